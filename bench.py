@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark: harmonic-trials/s of the coherent harmonic search on one MI355X.
+
+Workload (one "step"): one full search of a 2^24-sample series at dt = 64 us
+(5.3 ms von Mises pulsar, duty 0.03, S/N 12, realfft + rednoise in memory),
+nharms 60, m 16, maxdecim 6, blocksize 65536, hifreq 125 Hz, threshold 8 --
+interpolation of every filled harmonic of 16.1 M trial fundamentals, c2r
+profiles, boxcar S/N over all widths and decimations, candidate list.  The
+spectrum is resident in HBM before the timed region (uploaded once); each step
+is the fused kernel plus the candidate readback and host ordering.
+
+Prints one JSON line.  Extra keys: the kernel's HBM roofline from HIP events
+(algorithmic bytes of the reference GPU dataflow per launch / average kernel
+time), the CPU baseline (the oracle's restatement of the reference CPU path on
+a bounded sample of the same workload), the upload-inclusive rate and the
+parity of the step's candidates.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(spec, params_kw, budget_s=12.0):
+    """Oracle (reference CPU path restated, f32, blocksize 2048, OpenMP
+    round-robin chunks) on the leading chunks of the same spectrum until the
+    time budget is spent; harmonic-trials/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = len(os.sched_getaffinity(0))
+    cfg = po.config(blocksize=2048, **params_kw)
+    o = po.Oracle(spec.amps, spec.N, spec.T, cfg)
+    cs_ = o.chunk_starts()
+    # harmonic-trials per chunk from the fit predicate at the chunk ends
+    done_ht, elapsed, c0 = 0, 0.0, 0
+    step = max(threads * 4, 8)
+    # start mid-band so the sample has every harmonic filled like most chunks
+    c0 = len(cs_) // 4
+    t_start = time.perf_counter()
+    while elapsed < budget_s and c0 < o.nchunks:
+        c1 = min(c0 + step, o.nchunks)
+        t = time.perf_counter()
+        o.search(8.0, threads, c0, c1)
+        elapsed += time.perf_counter() - t
+        for c in range(c0, c1):
+            i0, n, _ = o.chunk(c)
+            nf = sum(po.lib().orc_harmonic_fits(o.h, h, i0) and
+                     po.lib().orc_harmonic_fits(o.h, h, i0 + n - 1)
+                     for h in range(1, cfg.nharms + 1))
+            done_ht += nf * n
+        c0 = c1
+    o.close()
+    return dict(value=done_ht / elapsed, unit="harmonic-trials/s", cores=threads, kind="port",
+                sample="chunks %d..%d of the same search (blocksize 2048), %.1f s, %.3g "
+                       "harmonic-trials" % (len(cs_) // 4, c0, elapsed, done_ht))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--samples", type=int, default=1 << 24)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the search shards by request or by "
+              "trial range across devices (see the design notes), --gpus %d is not run"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import coherentsearch_c_amd as cs
+    from coherentsearch_c_amd import synth
+
+    rc, name = cs.gpu_available(0)
+    if rc != 1:
+        log("no GPU:", name)
+        sys.exit(1)
+    log("device:", name)
+    t = time.perf_counter()
+    x, dt = synth.config2(N=args.samples)
+    spec = cs.redfft(x, dt)
+    del x
+    log("staging (synth + realfft + rednoise): %.2f s" % (time.perf_counter() - t))
+    pkw = dict(nharms=60, m=16, maxdecim=6)
+    params = cs.make_params(**pkw)
+    t = time.perf_counter()
+    sess = cs.Session(spec, params, blocksize=65536)
+    ht, trials, alg_bytes = sess.work()
+    log("session: %d trials, %d chunks, %.4g harmonic-trials, create %.2f s"
+        % (trials, sess.nchunks, ht, time.perf_counter() - t))
+    t = time.perf_counter()
+    sess.upload()
+    up_s = time.perf_counter() - t
+
+    for _ in range(args.warmup):
+        c = sess.search_resident(8.0)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        c = sess.search_resident(8.0)
+        kms.append(sess.kernel_ms())
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    value = ht * args.steps / wall
+    kavg = float(np.mean(kms))
+    achieved = alg_bytes / (kavg * 1e-3) / 1e9
+    top = cs.finalize(spec, params, cs.search_file(spec, params, blocksize=65536,
+                                                   threshold=8.0))
+    f0 = 1.0 / 5.3e-3
+    found = bool(top.size and any(abs(fr / f0 - round(fr / f0)) < 2e-3 and round(fr / f0) >= 1
+                                  for fr in top["freq"][:3]))
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_search_c2.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    out = {
+        "metric": "harmonic-trials/s at 1 GPU vs plain form; p50/p99 search latency; "
+                  "HBM GB/s frac",
+        "value": value,
+        "unit": "harmonic-trials/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian noise + injected von Mises pulsar)",
+        "config": {"workload": "single 2^%d-sample series, dt 64 us, realfft+rednoise; "
+                               "nharms 60, m 16, maxdecim 6, blocksize 65536, hifreq 125 Hz, "
+                               "threshold 8" % int(np.log2(args.samples)),
+                   "trials": trials, "harmonic_trials": ht, "chunks": int(sess.nchunks)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "kernel_ms": kavg,
+        "candidates_per_step": int(c.size),
+        "pulsar_in_top3": found,
+        "upload_inclusive_ht_s": ht / (wall / args.steps + up_s),
+        "upload_s": up_s,
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(spec, pkw, args.cpu_budget)
+        except Exception as e:  # the baseline is reported, never required
+            out["cpu_baseline"] = {"error": repr(e)}
+    sess.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

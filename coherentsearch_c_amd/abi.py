@@ -1,0 +1,207 @@
+"""ctypes mirror of the C-ABI (include/cs_types.h, cs_gpu.h, cs_host.h).
+
+The structs keep the reference's layouts (include/coherent_search.h:48-195);
+sizes are asserted at import.  The libraries are loaded from this package's
+directory and a missing or unbuildable library raises -- there is no fallback
+implementation.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(HERE, "libcs_hip.so")
+HOST_LIB = os.path.join(HERE, "libcs_host.so")
+
+
+class SimpleInf(C.Structure):
+    _fields_ = [("path", C.c_char_p), ("object", C.c_char_p), ("has_epoch", C.c_int),
+                ("epoch", C.c_double), ("N", C.c_longlong), ("has_N", C.c_int),
+                ("dt", C.c_double), ("has_dt", C.c_int), ("DM", C.c_double),
+                ("has_DM", C.c_int)]
+
+
+class FFTFile(C.Structure):
+    _fields_ = [("path", C.c_char_p), ("inf", SimpleInf), ("N", C.c_longlong),
+                ("T", C.c_double), ("df", C.c_double), ("dereddened", C.c_int),
+                ("detrended", C.c_int), ("DC", C.c_float), ("Nyquist", C.c_float),
+                ("amps", C.POINTER(C.c_float)), ("namps", C.c_longlong),
+                ("map", C.c_void_p), ("mapsize", C.c_size_t), ("owns_amps", C.c_int)]
+
+
+class SearchParams(C.Structure):
+    _fields_ = [("nharms", C.c_int), ("m", C.c_int), ("numbetween", C.c_int),
+                ("hidr", C.c_double), ("threshold", C.c_double),
+                ("boxcar_fsp", C.c_double), ("boxcar_maxfrac", C.c_double),
+                ("boxcar_gatemargin", C.c_double), ("ndecim", C.c_int),
+                ("decimations", C.c_int * 64), ("precision", C.c_int), ("sigma", C.c_int)]
+
+
+class DirectPlan(C.Structure):
+    _fields_ = [("h", C.c_int), ("m", C.c_int), ("q", C.c_int), ("s", C.c_int),
+                ("base_adv", C.c_int), ("pnum", C.c_int), ("rfloor0", C.c_int),
+                ("dr0", C.c_double), ("P", C.c_int),
+                ("row", C.POINTER(C.c_int32)), ("W", C.POINTER(C.c_float)),
+                ("A_re", C.POINTER(C.c_float)), ("A_im", C.POINTER(C.c_float)),
+                ("carry", C.POINTER(C.c_uint8)), ("ngrp", C.c_int),
+                ("grow", C.POINTER(C.c_int32)), ("goff", C.POINTER(C.c_int32)),
+                ("gnj", C.POINTER(C.c_int32)), ("gW", C.POINTER(C.c_float)),
+                ("gA_re", C.POINTER(C.c_float)), ("gA_im", C.POINTER(C.c_float))]
+
+
+class DirectPlanSet(C.Structure):
+    _fields_ = [("plans", C.POINTER(DirectPlan)), ("n", C.c_int)]
+
+
+class Candidate(C.Structure):
+    _fields_ = [("freq", C.c_double), ("metric", C.c_double), ("r", C.c_double),
+                ("nharm", C.c_int), ("ducy", C.c_double)]
+
+
+class CandidateList(C.Structure):
+    _fields_ = [("v", C.POINTER(Candidate)), ("n", C.c_size_t), ("cap", C.c_size_t)]
+
+
+class SearchCache(C.Structure):
+    _fields_ = [("Nprof", C.c_int), ("plan_valid", C.c_int),
+                ("params", C.POINTER(SearchParams)), ("plan_T", C.c_double),
+                ("plan_N", C.c_longlong), ("plan_namps", C.c_longlong),
+                ("plan_lofreq", C.c_double), ("plan_hifreq", C.c_double),
+                ("plan_lobin", C.c_int), ("plan_Nprof", C.c_int),
+                ("plan_total", C.c_longlong), ("dplans", DirectPlanSet),
+                ("cstarts", C.POINTER(C.c_longlong)), ("nchunks", C.c_longlong),
+                ("gpu_session", C.c_void_p)]
+
+
+assert C.sizeof(SimpleInf) == 80 and C.sizeof(FFTFile) == 168
+assert C.sizeof(SearchParams) == 328 and C.sizeof(DirectPlan) == 144
+assert C.sizeof(Candidate) == 40 and C.sizeof(CandidateList) == 24
+
+# Every symbol the public headers declare, per library.
+HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matches",
+               "cs_gpu_session_run", "cs_gpu_session_free", "cs_gpu_search_region",
+               "cs_gpu_session_dump_stack", "cs_gpu_session_last_kernel_ms",
+               "cs_gpu_session_upload", "cs_gpu_session_search_resident",
+               "cs_gpu_session_work"]
+HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
+                "direct_plans_build", "direct_plans_free", "cs_chunk_state",
+                "cs_harmonic_fits", "harmonic_change_trials", "chunk_starts", "cs_sinpi",
+                "cs_cospi", "cs_eps", "fourier_interp_complex", "boxcar_widths",
+                "ladder_boxcar_widths", "analytic_sigma", "boxcar_shape",
+                "boxcar_best_width", "candlist_init", "candlist_free", "candlist_push",
+                "candlist_append", "candlist_sort_metric_desc", "candlist_sort_freq",
+                "candlist_sort_r", "candlist_sort_metric_desc_stable",
+                "candlist_sort_freq_stable", "candlist_sort_r_stable", "remove_duplicates",
+                "remove_harmonics", "harmonically_related", "candidate_profile",
+                "measure_ducy", "write_candidates", "cs_format_double",
+                "read_candidates_file", "inf_read", "inf_free", "fftfile_open",
+                "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_redfft",
+                "datpipe_redfft_mem", "cs_open_input", "cs_write_inf", "cs_write_floats",
+                "search_file", "search_cache_free", "cs_main", "cs_main_list"]
+
+_host = None
+_hip = None
+
+
+def _declare(L):
+    P, I, D, LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+    Ps = C.POINTER
+    L.params_defaults.argtypes = [Ps(SearchParams)]
+    L.decimation_set.argtypes = [I, I, Ps(C.c_int), I]
+    L.decimation_set.restype = I
+    L.trial_grid_rational.argtypes = [D, Ps(I), Ps(I)]
+    L.trial_grid_rational.restype = I
+    L.direct_plans_build.argtypes = [Ps(DirectPlanSet), Ps(SearchParams), D]
+    L.direct_plans_build.restype = I
+    L.direct_plans_free.argtypes = [Ps(DirectPlanSet)]
+    L.chunk_starts.argtypes = [Ps(DirectPlanSet), Ps(FFTFile), Ps(SearchParams), LL, I,
+                               Ps(Ps(LL)), Ps(LL)]
+    L.chunk_starts.restype = I
+    L.cs_harmonic_fits.argtypes = [Ps(DirectPlan), Ps(FFTFile), Ps(SearchParams), LL]
+    L.cs_harmonic_fits.restype = I
+    L.ladder_boxcar_widths.argtypes = [I, I, Ps(SearchParams), Ps(C.c_int), I]
+    L.ladder_boxcar_widths.restype = I
+    L.boxcar_widths.argtypes = [I, D, D, Ps(C.c_int), I]
+    L.boxcar_widths.restype = I
+    L.analytic_sigma.argtypes = [P, I, I]
+    L.analytic_sigma.restype = D
+    L.candlist_free.argtypes = [Ps(CandidateList)]
+    L.remove_duplicates.argtypes = [Ps(CandidateList), D]
+    L.remove_duplicates.restype = CandidateList
+    L.remove_harmonics.argtypes = [Ps(CandidateList), I, D]
+    L.remove_harmonics.restype = CandidateList
+    L.candlist_sort_metric_desc.argtypes = [Ps(CandidateList)]
+    L.candlist_sort_freq_stable.argtypes = [Ps(CandidateList)]
+    L.measure_ducy.argtypes = [Ps(FFTFile), Ps(CandidateList), Ps(SearchParams)]
+    L.measure_ducy.restype = CandidateList
+    L.write_candidates.argtypes = [Ps(CandidateList), C.c_char_p, D]
+    L.write_candidates.restype = I
+    L.fftfile_open.argtypes = [Ps(FFTFile), C.c_char_p]
+    L.fftfile_open.restype = I
+    L.fftfile_close.argtypes = [Ps(FFTFile)]
+    L.datpipe_create.argtypes = [I, I, D]
+    L.datpipe_create.restype = P
+    L.datpipe_free.argtypes = [P]
+    L.datpipe_redfft_mem.argtypes = [P, P, LL, D, C.c_char_p, Ps(FFTFile)]
+    L.datpipe_redfft_mem.restype = I
+    L.cs_write_inf.argtypes = [C.c_char_p, LL, D, D, C.c_char_p]
+    L.cs_write_inf.restype = I
+    L.cs_write_floats.argtypes = [C.c_char_p, P, LL]
+    L.cs_write_floats.restype = I
+    L.search_file.argtypes = [Ps(FFTFile), Ps(SearchParams), D, D, I, I, D, I, D, I, I,
+                              P, I, I, I, C.c_char_p, Ps(SearchCache), D, I]
+    L.search_file.restype = CandidateList
+    L.search_cache_free.argtypes = [Ps(SearchCache)]
+    L.cs_main.argtypes = [I, Ps(C.c_char_p)]
+    L.cs_main.restype = I
+    L.cs_main_list.argtypes = [I, Ps(C.c_char_p)]
+    L.cs_main_list.restype = I
+    # GPU backend
+    L.cs_gpu_available.argtypes = [I, C.c_char_p, C.c_size_t]
+    L.cs_gpu_available.restype = I
+    L.cs_gpu_session_create.argtypes = [Ps(FFTFile), Ps(SearchParams), Ps(DirectPlanSet),
+                                        Ps(LL), LL, I, LL, D, D, I]
+    L.cs_gpu_session_create.restype = P
+    L.cs_gpu_session_matches.argtypes = [P, Ps(FFTFile), Ps(SearchParams), I, LL, D, D, I]
+    L.cs_gpu_session_matches.restype = I
+    L.cs_gpu_session_run.argtypes = [P, Ps(FFTFile), D]
+    L.cs_gpu_session_run.restype = CandidateList
+    L.cs_gpu_session_free.argtypes = [P]
+    L.cs_gpu_search_region.argtypes = [Ps(FFTFile), Ps(SearchParams), Ps(DirectPlanSet),
+                                       Ps(LL), LL, I, LL, D, D, D, D, I]
+    L.cs_gpu_search_region.restype = CandidateList
+    L.cs_gpu_session_dump_stack.argtypes = [P, Ps(FFTFile), LL, LL, P]
+    L.cs_gpu_session_dump_stack.restype = I
+    L.cs_gpu_session_upload.argtypes = [P, Ps(FFTFile)]
+    L.cs_gpu_session_upload.restype = I
+    L.cs_gpu_session_search_resident.argtypes = [P, D]
+    L.cs_gpu_session_search_resident.restype = CandidateList
+    L.cs_gpu_session_last_kernel_ms.argtypes = [P]
+    L.cs_gpu_session_last_kernel_ms.restype = D
+    L.cs_gpu_session_work.argtypes = [P, Ps(LL), Ps(LL), Ps(D)]
+    L.cs_gpu_session_work.restype = I
+    L.free_ = C.CDLL(None).free
+    L.free_.argtypes = [P]
+
+
+def host():
+    """libcs_host.so (also pulls in libcs_hip.so).  Raises if not built."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB) or not os.path.exists(HIP_LIB):
+            raise RuntimeError(
+                "coherentsearch_c_amd native libraries are not built (%s); run "
+                "__graft_entry__.build() or make -C coherentsearch_c_amd/csrc" % HERE)
+        L = C.CDLL(HOST_LIB, mode=C.RTLD_GLOBAL)
+        _declare(L)
+        _host = L
+    return _host
+
+
+def hip():
+    """libcs_hip.so on its own (for symbol checks)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise RuntimeError("libcs_hip.so is not built")
+        _hip = C.CDLL(HIP_LIB)
+    return _hip

@@ -1,0 +1,46 @@
+/*
+ * Device-side data layout shared by the host session code and the kernels.
+ *
+ * HBM layout of one search (all resident for the life of a session):
+ *   amps    float2[namps]                 the spectrum, interleaved re/im
+ *   wtab    float[nh*q*WS]                per (harmonic h, residue res) the m
+ *                                         interpolation weights W, then A.re,
+ *                                         A.im (WS = m + 4, 16-byte rows)
+ *   hinfo   int2[nh]                      (rfloor0, first residue with carry)
+ *   tw      float2[TW_SIZE]               twiddles, N(N-1)/2 + j for N <= TW_NMAX
+ *   chunks  ChunkDesc[nchunks]            i0, n, filled mask, per-slot nvalid,
+ *                                         per-slot per-width 1/(sigma*sd_w)
+ *   tiles   int4[ntiles]                  (chunk, first trial, count, 0)
+ *   recs    CandRec[cap] + counter        above-threshold records
+ */
+#ifndef CS_DEVICE_H
+#define CS_DEVICE_H
+
+#include <stdint.h>
+
+#define CS_TILE 256            /* trials per workgroup (one per lane) */
+#define CS_MAXSLOT 17          /* base + up to 16 decimations */
+#define CS_NSEQ 12             /* width candidates 1,2,3,4,6,9,13,19,28,42,63,94 */
+#define CS_TW_NMAX 257
+#define CS_TW_SIZE (CS_TW_NMAX * (CS_TW_NMAX - 1) / 2)
+#define CS_WIN_CAP 6144        /* float2 window slots in LDS per workgroup */
+
+typedef struct {
+    long long i0;
+    int n;
+    int pad;
+    unsigned long long filled;      /* bit h-1: harmonic h fits the chunk */
+    int nvalid[CS_MAXSLOT];         /* trials below the decimated Nyquist */
+    int valid[CS_MAXSLOT];          /* 1/sigma > 0 */
+    float invsw[CS_MAXSLOT][CS_NSEQ];
+} ChunkDesc;
+
+typedef struct {
+    unsigned long long key;         /* chunk << 40 | slot << 32 | j */
+    float metric;
+    unsigned int pad;
+} CandRec;
+
+static inline int cs_tw_off(int N) { return N * (N - 1) / 2; }
+
+#endif

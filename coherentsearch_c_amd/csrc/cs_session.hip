@@ -1,0 +1,621 @@
+/*
+ * libcs_hip.so session layer: the reference's cs_gpu_* C-ABI (src/cuda.cu
+ * :488-1266) re-designed for one MI355X.
+ *
+ * create  : plan arena (per harmonic x residue weight rows), twiddles, the
+ *           chunk table with per-chunk filled masks, Nyquist counts and
+ *           1/(sigma*sd_w) for every slot and width, and the tile table --
+ *           everything a search needs stays resident in HBM.
+ * run     : one upload of the spectrum, one fused kernel launch over every
+ *           tile of every chunk, one readback of the above-threshold records,
+ *           host sort into the reference's candidate order.
+ * The noise-scale shape follows the reference CPU path's count-keyed cache
+ * (src/search.c:36-46) walked in chunk order, i.e. one CPU thread.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "core.h"
+#include "cs_device.h"
+#include "cs_gpu.h"
+#include "fast_kernel.hpp"
+
+#define HIDDEN __attribute__((visibility("hidden")))
+
+typedef hipError_t (*fast_launch_fn)(const csk::FastArgs *, int, hipStream_t);
+
+#define CS_DECL_FAST(NH, M, K)                                                    \
+    extern "C" HIDDEN hipError_t cs_launch_fast_##NH##_##M##_##K(const csk::FastArgs *, \
+                                                                int, hipStream_t);
+#define CS_FAST_LIST(X) \
+    X(16, 16, 6) X(16, 32, 6) X(32, 16, 6) X(32, 32, 6) X(60, 16, 6) X(60, 32, 6)
+CS_FAST_LIST(CS_DECL_FAST)
+
+namespace {
+
+struct FastVariant {
+    int nh, m, kmax;
+    fast_launch_fn fn;
+};
+
+#define CS_ENTRY_FAST(NH, M, K) {NH, M, K, cs_launch_fast_##NH##_##M##_##K},
+const FastVariant kFast[] = {CS_FAST_LIST(CS_ENTRY_FAST)};
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            fprintf(stderr, "Error: HIP %s failed at %s:%d: %s\n", #x, __FILE__,   \
+                    __LINE__, hipGetErrorString(e_));                              \
+            goto fail;                                                             \
+        }                                                                          \
+    } while (0)
+
+int wseq_index(int w)
+{
+    int v = 1;
+    for (int i = 0; i < CS_NSEQ; i++) {
+        if (v == w) return i;
+        int g = (int)(1.5 * (double)v);
+        v = g > v + 1 ? g : v + 1;
+    }
+    return -1;
+}
+
+int wcap_of(int nb)
+{
+    int c = (int)(0.3 * (double)nb);
+    if (c < 1) c = 1;
+    if (c > nb - 1) c = nb - 1;
+    return c;
+}
+
+}  // namespace
+
+struct GpuSession {
+    const SearchParams *p;
+    int device;
+    int Nprof;
+    long long total, nchunks, N, namps;
+    double T, r_lo, lodr, nyq;
+    int nh, m, q, pnum;
+    int nslot;
+    int slot_k[CS_MAXSLOT], slot_Hk[CS_MAXSLOT];
+    std::vector<long long> cstarts;
+    std::vector<ChunkDesc> chunks;
+    std::vector<int> tiles;   /* 4 ints per tile */
+    long long ht;             /* filled harmonic-trials */
+    double alg_bytes;
+    const FastVariant *fast;
+    hipStream_t st;
+    hipEvent_t ev0, ev1;
+    float2 *d_amps;
+    float *d_wtab;
+    int2 *d_hinfo;
+    float2 *d_tw;
+    int4 *d_tiles;
+    ChunkDesc *d_chunks;
+    CandRec *d_recs;
+    unsigned int *d_counter;
+    float2 *d_stack;
+    unsigned int cap;
+    csk::FastArgs args;
+    double last_ms;
+};
+
+static void session_release(GpuSession *S)
+{
+    if (!S) return;
+    if (S->d_amps) hipFree(S->d_amps);
+    if (S->d_wtab) hipFree(S->d_wtab);
+    if (S->d_hinfo) hipFree(S->d_hinfo);
+    if (S->d_tw) hipFree(S->d_tw);
+    if (S->d_tiles) hipFree(S->d_tiles);
+    if (S->d_chunks) hipFree(S->d_chunks);
+    if (S->d_recs) hipFree(S->d_recs);
+    if (S->d_counter) hipFree(S->d_counter);
+    if (S->d_stack) hipFree(S->d_stack);
+    if (S->ev0) hipEventDestroy(S->ev0);
+    if (S->ev1) hipEventDestroy(S->ev1);
+    if (S->st) hipStreamDestroy(S->st);
+    delete S;
+}
+
+extern "C" CS_API int cs_gpu_available(int device, char *errbuf, size_t errlen)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) {
+        if (errbuf) snprintf(errbuf, errlen, "no HIP device available (%s)",
+                             e != hipSuccess ? hipGetErrorString(e) : "0 devices");
+        return -1;
+    }
+    if (device < 0 || device >= count) {
+        if (errbuf) snprintf(errbuf, errlen,
+                             "HIP device %d requested but %d device(s) present",
+                             device, count);
+        return -1;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        if (errbuf) snprintf(errbuf, errlen, "cannot select HIP device %d: %s", device,
+                             hipGetErrorString(e));
+        return -1;
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        if (errbuf) snprintf(errbuf, errlen, "hipGetDeviceProperties failed: %s",
+                             hipGetErrorString(e));
+        return -1;
+    }
+    if (errbuf) snprintf(errbuf, errlen, "%s (device %d, %s)", prop.name, device,
+                         prop.gcnArchName);
+    return 1;
+}
+
+/* Per-chunk slot data: filled mask, Nyquist counts, analytic 1/(sigma sd_w). */
+static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
+{
+    const SearchParams *p = S->p;
+    const int nh = S->nh;
+    std::vector<uint8_t> filled(nh);
+    int widths[CS_MAXSLOT][256], nw[CS_MAXSLOT];
+    int wsel[CS_MAXSLOT][256];
+    for (int s = 0; s < S->nslot; s++) {
+        const int nb = 2 * S->slot_Hk[s];
+        nw[s] = csx_ladder_widths(nb, S->slot_k[s], p, widths[s], 256);
+        unsigned mask = 0;
+        for (int i = 0; i < nw[s]; i++) {
+            const int wi = wseq_index(widths[s][i]);
+            if (wi < 0 || widths[s][i] > wcap_of(nb)) return -2;
+            wsel[s][i] = wi;
+            mask |= 1u << wi;
+        }
+        S->args.wmask[s] = mask;
+    }
+    /* count-keyed shape cache, one per slot (src/search.c:36-46) */
+    int shapeH[CS_MAXSLOT];
+    static thread_local double shape[CS_MAXSLOT][256];
+    for (int s = 0; s < CS_MAXSLOT; s++) shapeH[s] = -1;
+    S->chunks.resize((size_t)S->nchunks);
+    S->tiles.clear();
+    S->ht = 0;
+    for (long long c = 0; c < S->nchunks; c++) {
+        ChunkDesc &C = S->chunks[(size_t)c];
+        memset(&C, 0, sizeof(C));
+        const long long i0 = S->cstarts[(size_t)c];
+        const int n = (int)((c + 1 < S->nchunks ? S->cstarts[(size_t)c + 1] : S->total) - i0);
+        C.i0 = i0;
+        C.n = n;
+        const double rstart = S->r_lo + (double)i0 * S->lodr;
+        int nf_all = 0;
+        for (int h = 1; h <= nh; h++) {
+            const DirectPlan *pl = &dp->plans[h - 1];
+            const int f = csx_harmonic_fits(pl, S->lodr, S->namps, S->N, i0) &&
+                          csx_harmonic_fits(pl, S->lodr, S->namps, S->N, i0 + n - 1);
+            filled[h - 1] = (uint8_t)f;
+            if (f) C.filled |= 1ull << (h - 1);
+            nf_all += f;
+        }
+        S->ht += (long long)nf_all * n;
+        for (int s = 0; s < S->nslot; s++) {
+            const int k = S->slot_k[s], Hk = S->slot_Hk[s], nb = 2 * Hk;
+            int nvalid = n;
+            if (s > 0) {
+                int lo = 0, hi = n;
+                while (lo < hi) {
+                    const int mid = lo + ((hi - lo) >> 1);
+                    if ((double)k * (rstart + (double)mid * S->lodr) < S->nyq) lo = mid + 1;
+                    else hi = mid;
+                }
+                nvalid = lo;
+            }
+            C.nvalid[s] = nvalid;
+            int nf = 0;
+            for (int j = 1; j <= Hk; j++) nf += filled[(size_t)j * k - 1];
+            if (shapeH[s] != nf) {
+                csx_boxcar_shape(shape[s], nb, widths[s], nw[s], filled.data(), k, Hk);
+                shapeH[s] = nf;
+            }
+            const float sig = (float)csx_analytic_sigma(filled.data(), k, Hk);
+            const float invsigma = sig > 0.0f ? 1.0f / sig : 0.0f;
+            C.valid[s] = invsigma > 0.0f;
+            for (int i = 0; i < nw[s]; i++)
+                C.invsw[s][wsel[s][i]] = (float)((double)invsigma * shape[s][i]);
+        }
+        for (int j0 = 0; j0 < n; j0 += CS_TILE) {
+            S->tiles.push_back((int)c);
+            S->tiles.push_back(j0);
+            S->tiles.push_back(std::min(CS_TILE, n - j0));
+            S->tiles.push_back(0);
+        }
+    }
+    return 0;
+}
+
+extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const SearchParams *p,
+                                                    const DirectPlanSet *dplans,
+                                                    const long long *cstarts, long long total,
+                                                    int Nprof, long long nchunks,
+                                                    double r_lo, double lodr, int device)
+{
+    if (!ft || !p || !dplans || (!cstarts && nchunks > 0)) return NULL;
+    GpuSession *S = new GpuSession();
+    memset((void *)&S->args, 0, sizeof(S->args));
+    S->p = p;
+    S->device = device;
+    S->Nprof = Nprof;
+    S->total = total;
+    S->nchunks = nchunks;
+    S->N = ft->N;
+    S->namps = ft->namps;
+    S->T = ft->T;
+    S->r_lo = r_lo;
+    S->lodr = lodr;
+    S->nyq = (double)ft->N / 2.0;
+    S->nh = p->nharms;
+    S->m = p->m;
+    S->cstarts.assign(cstarts, cstarts + nchunks);
+    S->d_amps = NULL; S->d_wtab = NULL; S->d_hinfo = NULL; S->d_tw = NULL;
+    S->d_tiles = NULL; S->d_chunks = NULL; S->d_recs = NULL; S->d_counter = NULL;
+    S->d_stack = NULL; S->st = NULL; S->ev0 = NULL; S->ev1 = NULL;
+    S->last_ms = 0.0;
+    S->fast = NULL;
+
+    const int nh = p->nharms, m = p->m;
+    std::vector<float> wtab;
+    std::vector<int2> hinfo;
+    std::vector<float2> tw;
+    int ws;
+    int rc;
+
+    if (dplans->n != nh || nh < 1) {
+        fprintf(stderr, "Error: plan set has %d harmonics, expected %d\n", dplans->n, nh);
+        goto fail;
+    }
+    if (p->precision != CS_PREC_F32 || p->sigma != CS_SIGMA_ANALYTIC) {
+        fprintf(stderr, "Error: this build's GPU search implements --precision f32 with "
+                        "--sigma analytic\n");
+        goto fail;
+    }
+    /* slots: base, then decimations k > 1 in SearchParams order */
+    S->nslot = 1;
+    S->slot_k[0] = 1;
+    S->slot_Hk[0] = nh;
+    for (int i = 0; i < CS_MAXSLOT; i++) S->args.kslot[i] = -1;
+    for (int i = 0; i < p->ndecim; i++) {
+        const int k = p->decimations[i];
+        if (k <= 1) continue;
+        if (S->nslot >= CS_MAXSLOT || k >= CS_MAXSLOT || S->args.kslot[k] >= 0 || nh / k < 1) {
+            fprintf(stderr, "Error: decimation %d is not supported by the GPU search\n", k);
+            goto fail;
+        }
+        S->args.kslot[k] = S->nslot;
+        S->slot_k[S->nslot] = k;
+        S->slot_Hk[S->nslot] = nh / k;
+        S->nslot++;
+    }
+    for (const FastVariant &v : kFast) {
+        if (v.nh != nh || v.m != m) continue;
+        bool ok = true;
+        for (int s = 1; s < S->nslot; s++) ok = ok && S->slot_k[s] <= v.kmax;
+        if (ok) { S->fast = &v; break; }
+    }
+    if (!S->fast) {
+        fprintf(stderr, "Error: no GPU kernel for nharms=%d m=%d with these decimations\n",
+                nh, m);
+        goto fail;
+    }
+    S->q = dplans->plans[0].q;
+    S->pnum = dplans->plans[0].pnum;
+    for (int h = 0; h < nh; h++)
+        if (dplans->plans[h].P <= 0 || dplans->plans[h].q != S->q ||
+            dplans->plans[h].pnum != S->pnum || dplans->plans[h].m != m) {
+            fprintf(stderr, "Error: the GPU search needs a rational --hidr (tabulated plans)\n");
+            goto fail;
+        }
+    /* 32-bit residue walk bounds: h*rho < 2^31, h*qbase < 2^31 */
+    if ((double)nh * (double)S->q >= 2147483647.0 ||
+        (double)nh * ((double)total * S->pnum / S->q + 2.0) >= 2147483000.0) {
+        fprintf(stderr, "Error: trial grid too large for the GPU residue walk\n");
+        goto fail;
+    }
+    /* every harmonic window of a tile must fit the LDS buffer */
+    if ((double)CS_TILE * nh * lodr + m + 2 > (double)CS_WIN_CAP) {
+        fprintf(stderr, "Error: --hidr too large for the GPU tile window\n");
+        goto fail;
+    }
+
+    /* plan arena: row (h, res) = W[0..m-1], A.re, A.im, 0, 0 */
+    ws = m + 4;
+    wtab.assign((size_t)nh * S->q * ws, 0.0f);
+    hinfo.resize((size_t)nh);
+    for (int h = 1; h <= nh; h++) {
+        const DirectPlan *pl = &dplans->plans[h - 1];
+        int cthr = S->q;
+        for (int res = 0; res < S->q; res++) {
+            if (pl->row[res] == 0) continue;
+            const int ph = pl->row[res] - 1;
+            float *row = &wtab[((size_t)(h - 1) * S->q + res) * ws];
+            memcpy(row, pl->W + (size_t)ph * m, (size_t)m * sizeof(float));
+            row[m] = pl->A_re[ph];
+            row[m + 1] = pl->A_im[ph];
+            if (pl->carry[ph] && res < cthr) cthr = res;
+        }
+        for (int res = 0; res < S->q; res++) {
+            if (pl->row[res] == 0) continue;
+            if ((pl->carry[pl->row[res] - 1] != 0) != (res >= cthr)) {
+                fprintf(stderr, "Error: harmonic %d carry table is not monotone\n", h);
+                goto fail;
+            }
+        }
+        hinfo[(size_t)h - 1] = make_int2(pl->rfloor0, cthr);
+    }
+    /* twiddles tw(j, N) = (cos, sin)(2 pi j / N), the project c2r's table */
+    tw.resize(CS_TW_SIZE);
+    for (int Nn = 1; Nn < CS_TW_NMAX; Nn++)
+        for (int j = 0; j < Nn; j++) {
+            const double ang = 2.0 * M_PI * (double)j / (double)Nn;
+            tw[(size_t)cs_tw_off(Nn) + j] = make_float2((float)cos(ang), (float)sin(ang));
+        }
+
+    rc = build_chunks(S, dplans);
+    if (rc == -2) {
+        fprintf(stderr, "Error: boxcar width bank outside the GPU kernel's candidates "
+                        "(fsp 1.5, maxfrac <= 0.3)\n");
+        goto fail;
+    }
+    {
+        /* algorithmic bytes of the reference GPU dataflow (per trial) */
+        bool anyfold = false;
+        double b = 16.0 * (nh + 1);
+        for (int s = 1; s < S->nslot; s++) {
+            const int k = S->slot_k[s], Hk = S->slot_Hk[s];
+            if (nh % k == 0) anyfold = true;
+            else b += 8.0 * (Hk + 1) + 16.0 * Hk;
+        }
+        b += 8.0 * nh * (2 + (anyfold ? 1 : 0));
+        S->alg_bytes = b * (double)total + 8.0 * (double)S->namps;
+    }
+
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&S->ev0));
+    HIPCHK(hipEventCreate(&S->ev1));
+    HIPCHK(hipMalloc(&S->d_amps, (size_t)std::max<long long>(S->namps, 1) * sizeof(float2)));
+    HIPCHK(hipMalloc(&S->d_wtab, wtab.size() * sizeof(float)));
+    HIPCHK(hipMemcpy(S->d_wtab, wtab.data(), wtab.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&S->d_hinfo, hinfo.size() * sizeof(int2)));
+    HIPCHK(hipMemcpy(S->d_hinfo, hinfo.data(), hinfo.size() * sizeof(int2), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&S->d_tw, tw.size() * sizeof(float2)));
+    HIPCHK(hipMemcpy(S->d_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&S->d_chunks, std::max<size_t>(S->chunks.size(), 1) * sizeof(ChunkDesc)));
+    if (!S->chunks.empty())
+        HIPCHK(hipMemcpy(S->d_chunks, S->chunks.data(), S->chunks.size() * sizeof(ChunkDesc),
+                         hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&S->d_tiles, std::max<size_t>(S->tiles.size(), 4) * sizeof(int)));
+    if (!S->tiles.empty())
+        HIPCHK(hipMemcpy(S->d_tiles, S->tiles.data(), S->tiles.size() * sizeof(int),
+                         hipMemcpyHostToDevice));
+    S->cap = 1u << 20;
+    HIPCHK(hipMalloc(&S->d_recs, (size_t)S->cap * sizeof(CandRec)));
+    HIPCHK(hipMalloc(&S->d_counter, sizeof(unsigned int)));
+
+    S->args.amps = S->d_amps;
+    S->args.namps = S->namps;
+    S->args.wtab = S->d_wtab;
+    S->args.hinfo = S->d_hinfo;
+    S->args.tw = S->d_tw;
+    S->args.tiles = S->d_tiles;
+    S->args.ntiles = (int)(S->tiles.size() / 4);
+    S->args.q = S->q;
+    S->args.pnum = S->pnum;
+    S->args.ws = ws;
+    S->args.chunks = S->d_chunks;
+    S->args.recs = S->d_recs;
+    S->args.counter = S->d_counter;
+    S->args.cap = S->cap;
+    return S;
+fail:
+    session_release(S);
+    return NULL;
+}
+
+extern "C" CS_API int cs_gpu_session_matches(const GpuSession *S, const FFTFile *ft,
+                                             const SearchParams *p, int Nprof,
+                                             long long total, double r_lo, double lodr,
+                                             int device)
+{
+    if (!S) return 0;
+    return S->p == p && S->device == device && S->Nprof == Nprof && S->total == total &&
+           S->T == ft->T && S->N == ft->N && S->namps == ft->namps && S->r_lo == r_lo &&
+           S->lodr == lodr;
+}
+
+__global__ void cs_zero_u32(unsigned int *p) { p[0] = 0u; }
+
+static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
+{
+    S->args.stack_out = stack_out;
+    S->args.threshold = threshold;
+    S->args.recs = S->d_recs;
+    S->args.cap = S->cap;
+    const int ntiles = S->args.ntiles;
+    const int nblk = ((ntiles + 7) / 8) * 8;
+    hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
+    if (hipGetLastError() != hipSuccess) return -1;
+    hipEventRecord(S->ev0, S->st);
+    if (ntiles > 0 && S->fast->fn(&S->args, nblk, S->st) != hipSuccess) return -1;
+    hipEventRecord(S->ev1, S->st);
+    return 0;
+}
+
+extern "C" CS_API int cs_gpu_session_upload(GpuSession *S, const FFTFile *ft)
+{
+    if (!S || !ft) return -1;
+    if (ft->namps != S->namps || ft->N != S->N) {
+        fprintf(stderr, "Error: GPU session built for N=%lld/%lld amps but the input has "
+                        "N=%lld/%lld\n", S->N, S->namps, ft->N, ft->namps);
+        return -1;
+    }
+    if (hipSetDevice(S->device) != hipSuccess) return -1;
+    if (S->namps > 0 &&
+        hipMemcpy(S->d_amps, ft->amps, (size_t)S->namps * sizeof(float2),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        fprintf(stderr, "Error: spectrum upload failed\n");
+        return -1;
+    }
+    S->T = ft->T;
+    return 0;
+}
+
+extern "C" CS_API CandidateList cs_gpu_session_run(GpuSession *S, const FFTFile *ft,
+                                                   double threshold)
+{
+    CandidateList out;
+    out.v = NULL; out.n = 0; out.cap = 0;
+    if (!S || cs_gpu_session_upload(S, ft) != 0) return out;
+    return cs_gpu_session_search_resident(S, threshold);
+}
+
+extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
+                                                               double threshold)
+{
+    CandidateList out;
+    out.v = NULL; out.n = 0; out.cap = 0;
+    if (!S || hipSetDevice(S->device) != hipSuccess) return out;
+    unsigned int count = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (launch_search(S, threshold, NULL) != 0) {
+            fprintf(stderr, "Error: GPU search launch failed\n");
+            return out;
+        }
+        if (hipMemcpyAsync(&count, S->d_counter, sizeof(count), hipMemcpyDeviceToHost,
+                           S->st) != hipSuccess ||
+            hipStreamSynchronize(S->st) != hipSuccess) {
+            fprintf(stderr, "Error: GPU search failed: %s\n",
+                    hipGetErrorString(hipGetLastError()));
+            return out;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, S->ev0, S->ev1);
+        S->last_ms = ms;
+        if (count <= S->cap) break;
+        /* never truncate: grow to the exact count and search again */
+        hipFree(S->d_recs);
+        S->d_recs = NULL;
+        S->cap = count;
+        if (hipMalloc(&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
+            fprintf(stderr, "Error: cannot hold %u candidate records\n", count);
+            S->cap = 0;
+            return out;
+        }
+    }
+    if (count == 0) return out;
+    std::vector<CandRec> recs(count);
+    if (hipMemcpy(recs.data(), S->d_recs, (size_t)count * sizeof(CandRec),
+                  hipMemcpyDeviceToHost) != hipSuccess)
+        return out;
+    std::sort(recs.begin(), recs.end(),
+              [](const CandRec &a, const CandRec &b) { return a.key < b.key; });
+    out.v = (Candidate *)malloc((size_t)count * sizeof(Candidate));
+    if (!out.v) return out;
+    out.cap = count;
+    for (const CandRec &r : recs) {
+        const long long c = (long long)(r.key >> 40);
+        const int slot = (int)((r.key >> 32) & 0xff);
+        const long long j = (long long)(r.key & 0xffffffffull);
+        const double metric = (double)r.metric;
+        if (!(metric > threshold)) continue;
+        const double rstart = S->r_lo + (double)S->cstarts[(size_t)c] * S->lodr;
+        double rr;
+        int nharm;
+        if (slot == 0) {
+            rr = rstart + (double)j * S->lodr;
+            nharm = S->nh;
+        } else {
+            rr = (double)S->slot_k[slot] * (rstart + (double)j * S->lodr);
+            nharm = S->slot_Hk[slot];
+            if (!(rr < S->nyq)) continue;
+        }
+        Candidate cd;
+        cd.freq = rr / S->T;
+        cd.metric = metric;
+        cd.r = rr;
+        cd.nharm = nharm;
+        cd.ducy = NAN;
+        out.v[out.n++] = cd;
+    }
+    return out;
+}
+
+extern "C" CS_API void cs_gpu_session_free(GpuSession *S) { session_release(S); }
+
+extern "C" CS_API CandidateList cs_gpu_search_region(const FFTFile *ft, const SearchParams *p,
+                                                     const DirectPlanSet *dplans,
+                                                     const long long *cstarts,
+                                                     long long total, int Nprof,
+                                                     long long nchunks, double r_lo,
+                                                     double r_hi, double lodr,
+                                                     double threshold, int device)
+{
+    (void)r_hi;
+    GpuSession *S = cs_gpu_session_create(ft, p, dplans, cstarts, total, Nprof, nchunks,
+                                          r_lo, lodr, device);
+    if (!S) {
+        CandidateList e;
+        e.v = NULL; e.n = 0; e.cap = 0;
+        return e;
+    }
+    CandidateList c = cs_gpu_session_run(S, ft, threshold);
+    cs_gpu_session_free(S);
+    return c;
+}
+
+extern "C" CS_API int cs_gpu_session_dump_stack(GpuSession *S, const FFTFile *ft,
+                                                long long t0, long long n, float *out)
+{
+    if (!S || !ft || !out || n <= 0 || t0 < 0 || t0 + n > S->total) return -1;
+    if (ft->namps != S->namps || ft->N != S->N) return -1;
+    if (hipSetDevice(S->device) != hipSuccess) return -1;
+    const size_t bytes = (size_t)n * (S->nh + 1) * sizeof(float2);
+    float2 *d = NULL;
+    int rc = -1;
+    if (hipMemcpy(S->d_amps, ft->amps, (size_t)S->namps * sizeof(float2),
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    if (hipMalloc(&d, bytes) != hipSuccess) return -1;
+    if (hipMemsetAsync(d, 0, bytes, S->st) == hipSuccess) {
+        S->args.dump_t0 = t0;
+        S->args.dump_n = n;
+        if (launch_search(S, INFINITY, d) == 0 &&
+            hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, S->st) == hipSuccess &&
+            hipStreamSynchronize(S->st) == hipSuccess)
+            rc = 0;
+    }
+    S->args.stack_out = NULL;
+    hipFree(d);
+    return rc;
+}
+
+extern "C" CS_API double cs_gpu_session_last_kernel_ms(const GpuSession *S)
+{
+    return S ? S->last_ms : 0.0;
+}
+
+extern "C" CS_API int cs_gpu_session_work(const GpuSession *S, long long *ht,
+                                          long long *trials, double *alg_bytes)
+{
+    if (!S) return -1;
+    if (ht) *ht = S->ht;
+    if (trials) *trials = S->total;
+    if (alg_bytes) *alg_bytes = S->alg_bytes;
+    return 0;
+}

@@ -1,0 +1,477 @@
+/*
+ * Fused search kernel for gfx950, specialised on (nharms, m, max decimation).
+ *
+ * One workgroup = CS_TILE consecutive trials of one chunk, one trial per
+ * lane (wave64).  Per workgroup:
+ *   1. every filled harmonic's spectrum window for the tile is copied from
+ *      HBM into LDS with coalesced loads (batches if the windows exceed LDS);
+ *   2. each lane interpolates its trial at every harmonic h*r: m taps of
+ *      amps from LDS times the phase's f32 weights (an L2-resident table
+ *      indexed by the exact residue), explicit fmaf in tap order, then the
+ *      unfused rotation -- the reference's arithmetic (src/interp.c:686-811,
+ *      src/cuda.cu:121-135), so the stack is bit-identical to the CPU path;
+ *   3. the harmonic stack stays in registers and is turned into profiles by
+ *      a compile-time unrolled mixed-radix c2r (the project c2r, identical op
+ *      sequence to the oracle's), for the base and every non-divisor
+ *      decimation; divisor decimations fold the base profile
+ *      (src/search.c:294-331);
+ *   4. the boxcar matched filter runs on register-resident prefix sums over
+ *      the enabled widths (src/boxcar.c:203-252) and above-threshold trials
+ *      are appended with one atomic per wave.
+ * Nothing but the spectrum, the tables and the records touches HBM.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "cs_device.h"
+
+namespace csk {
+
+struct cf {
+    float x, y;
+};
+
+template <int... Is, class F>
+__device__ __forceinline__ void sfor_(std::integer_sequence<int, Is...>, F &&f)
+{
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void sfor(F &&f)
+{
+    sfor_(std::make_integer_sequence<int, N>{}, f);
+}
+
+constexpr int radix_of(int N)
+{
+    if (N % 4 == 0) return 4;
+    if (N % 2 == 0) return 2;
+    if (N % 3 == 0) return 3;
+    if (N % 5 == 0) return 5;
+    for (int p = 7; p * p <= N; p += 2)
+        if (N % p == 0) return p;
+    return N;
+}
+constexpr int twoff(int N) { return N * (N - 1) / 2; }
+
+/* width candidates: the fsp = 1.5 bank 1,2,3,4,6,9,13,... (src/boxcar.c:78-92) */
+constexpr int wseq(int i)
+{
+    int w = 1;
+    for (int k = 0; k < i; k++) {
+        int g = (int)(1.5 * (double)w);
+        w = g > w + 1 ? g : w + 1;
+    }
+    return w;
+}
+constexpr int wcap(int nb)
+{
+    int c = (int)(0.3 * (double)nb);
+    if (c < 1) c = 1;
+    if (c > nb - 1) c = nb - 1;
+    return c;
+}
+constexpr int ncand(int nb)
+{
+    int n = 0;
+    while (n < CS_NSEQ && wseq(n) <= wcap(nb)) n++;
+    return n;
+}
+constexpr int wcmax(int nb) { return ncand(nb) > 0 ? wseq(ncand(nb) - 1) : 0; }
+
+__device__ __forceinline__ cf cmul(cf a, float2 w)
+{
+    const float p1 = a.x * w.x, p2 = a.y * w.y, p3 = a.x * w.y, p4 = a.y * w.x;
+    return {p1 - p2, p3 + p4};
+}
+
+template <int R>
+__device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], const float2 *__restrict__ tw)
+{
+    if constexpr (R == 2) {
+        v[0] = {u[0].x + u[1].x, u[0].y + u[1].y};
+        v[1] = {u[0].x - u[1].x, u[0].y - u[1].y};
+    } else if constexpr (R == 4) {
+        const cf t0 = {u[0].x + u[2].x, u[0].y + u[2].y};
+        const cf t1 = {u[0].x - u[2].x, u[0].y - u[2].y};
+        const cf t2 = {u[1].x + u[3].x, u[1].y + u[3].y};
+        const cf t3 = {u[1].x - u[3].x, u[1].y - u[3].y};
+        v[0] = {t0.x + t2.x, t0.y + t2.y};
+        v[2] = {t0.x - t2.x, t0.y - t2.y};
+        v[1] = {t1.x - t3.y, t1.y + t3.x};
+        v[3] = {t1.x + t3.y, t1.y - t3.x};
+    } else {
+        constexpr int hh = (R - 1) / 2;
+        cf sp[hh + 1], sm[hh + 1];
+        sfor<hh>([&](auto I) {
+            constexpr int r = decltype(I)::value + 1;
+            sp[r] = {u[r].x + u[R - r].x, u[r].y + u[R - r].y};
+            sm[r] = {u[r].x - u[R - r].x, u[r].y - u[R - r].y};
+        });
+        cf x0 = u[0];
+        sfor<hh>([&](auto I) {
+            constexpr int r = decltype(I)::value + 1;
+            x0 = {x0.x + sp[r].x, x0.y + sp[r].y};
+        });
+        v[0] = x0;
+        sfor<hh>([&](auto S) {
+            constexpr int s = decltype(S)::value + 1;
+            cf A = u[0], B = {0.0f, 0.0f};
+            sfor<hh>([&](auto I) {
+                constexpr int r = decltype(I)::value + 1;
+                constexpr int j = (r * s) % R;
+                const float2 w = tw[twoff(R) + j];
+                const float pr = sp[r].x * w.x, pi = sp[r].y * w.x;
+                A = {A.x + pr, A.y + pi};
+                const float qr = sm[r].x * w.y, qi = sm[r].y * w.y;
+                if constexpr (r == 1) B = {qr, qi};
+                else B = {B.x + qr, B.y + qi};
+            });
+            v[s] = {A.x - B.y, A.y + B.x};
+            v[R - s] = {A.x + B.y, A.y - B.x};
+        });
+    }
+}
+
+/* out = IDFT_N(in[O + j*S]), decimation in time, radix order of radix_of(). */
+template <int N, int S, int O, int LEN>
+__device__ __forceinline__ void idft(const cf (&in)[LEN], cf (&out)[N],
+                                     const float2 *__restrict__ tw)
+{
+    if constexpr (N == 1) {
+        out[0] = in[O];
+    } else {
+        constexpr int R = radix_of(N), M = N / R;
+        cf Y[R][M];
+        sfor<R>([&](auto I) {
+            constexpr int r = decltype(I)::value;
+            idft<M, S * R, O + r * S, LEN>(in, Y[r], tw);
+        });
+        sfor<M>([&](auto K) {
+            constexpr int k = decltype(K)::value;
+            cf u[R], v[R];
+            u[0] = Y[0][k];
+            sfor<R - 1>([&](auto I) {
+                constexpr int r = decltype(I)::value + 1;
+                if constexpr (r * k == 0) u[r] = Y[r][k];
+                else u[r] = cmul(Y[r][k], tw[twoff(N) + r * k]);
+            });
+            bfly<R>(u, v, tw);
+            sfor<R>([&](auto I) {
+                constexpr int s = decltype(I)::value;
+                out[k + M * s] = v[s];
+            });
+        });
+    }
+}
+
+/* Project c2r of X[0], X[XS], ..., X[H*XS] into x[0..2H-1] (unnormalised). */
+template <int H, int XS, int LEN>
+__device__ __forceinline__ void c2r(const cf (&X)[LEN], float (&x)[2 * H],
+                                    const float2 *__restrict__ tw)
+{
+    cf Z[H];
+    {
+        const float x0 = X[0].x, xh = X[H * XS].x;
+        Z[0] = {x0 + xh, x0 - xh};
+    }
+    sfor<H - 1>([&](auto I) {
+        constexpr int k = decltype(I)::value + 1;
+        const cf a = X[k * XS], b = X[(H - k) * XS];
+        const float er = a.x + b.x, ei = a.y - b.y;
+        const float dr = a.x - b.x, di = a.y + b.y;
+        const float2 w = tw[twoff(2 * H) + k];
+        const float p1 = dr * w.x, p2 = di * w.y, p3 = dr * w.y, p4 = di * w.x;
+        const float o_r = p1 - p2, o_i = p3 + p4;
+        Z[k] = {er - o_i, ei + o_r};
+    });
+    cf z[H];
+    idft<H, 1, 0, H>(Z, z, tw);
+    sfor<H>([&](auto I) {
+        constexpr int s = decltype(I)::value;
+        x[2 * s] = z[s].x;
+        x[2 * s + 1] = z[s].y;
+    });
+}
+
+/* Boxcar S/N of one profile over the enabled widths (mask over the fsp=1.5
+ * candidates), src/boxcar.c:203-231: wrapped prefix sums, max window sum per
+ * width, (m - duty*total) * f32(invsigma*shape_w). */
+template <int NB>
+__device__ __forceinline__ float boxcar(const float (&x)[NB], unsigned wmask,
+                                        const float *__restrict__ invsw, int valid)
+{
+    constexpr int NC = ncand(NB);
+    constexpr int WC = wcmax(NB);
+    float P[NB + WC + 1];
+    P[0] = 0.0f;
+    sfor<NB + WC>([&](auto I) {
+        constexpr int i = decltype(I)::value + 1;
+        constexpr int idx = (i > NB ? i - NB : i) - 1;
+        P[i] = P[i - 1] + x[idx];
+    });
+    const float stot = P[NB] - P[0];
+    float best = -INFINITY;
+    sfor<NC>([&](auto I) {
+        constexpr int wi = decltype(I)::value;
+        constexpr int W = wseq(wi);
+        if (wmask & (1u << wi)) {
+            float m = P[W] - P[0];
+            sfor<NB - 1>([&](auto J) {
+                constexpr int p = decltype(J)::value + 2;
+                const float d = P[p + W - 1] - P[p - 1];
+                m = fmaxf(m, d);
+            });
+            constexpr float duty = (float)W / (float)NB;
+            const float dt = duty * stot;
+            const float c = (m - dt) * invsw[wi];
+            best = c > best ? c : best;
+        }
+    });
+    return valid ? best : 0.0f;
+}
+
+struct FastArgs {
+    const float2 *amps;
+    long long namps;
+    const float *wtab;
+    const int2 *hinfo;        /* (rfloor0, carry residue threshold) per h */
+    const float2 *tw;
+    const int4 *tiles;
+    int ntiles;
+    int q;
+    int pnum;
+    int ws;                   /* weight row stride (floats) */
+    const ChunkDesc *chunks;
+    CandRec *recs;
+    unsigned int *counter;
+    unsigned int cap;
+    double threshold;
+    int kslot[CS_MAXSLOT];    /* slot index of decimation k (k<=16), -1 off */
+    unsigned int wmask[CS_MAXSLOT];   /* per slot */
+    float2 *stack_out;        /* optional stack dump of trials [dump_t0, +dump_n) */
+    long long dump_t0;
+    long long dump_n;
+};
+
+__device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, float metric,
+                                     const FastArgs &a)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(a.counter, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) {
+        const unsigned int pos = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+        if (pos < a.cap) {
+            CandRec r;
+            r.key = ((unsigned long long)chunk << 40) | ((unsigned long long)slot << 32) |
+                    (unsigned int)j;
+            r.metric = metric;
+            r.pad = 0u;
+            a.recs[pos] = r;
+        }
+    }
+}
+
+template <int NH, int M, int KMAX>
+__global__ __launch_bounds__(CS_TILE) void fast_search_kernel(FastArgs a)
+{
+    static_assert(NH >= 1 && NH <= 64, "fast path holds up to 64 harmonics");
+    static_assert(M % 4 == 0, "tap count multiple of 4");
+    __shared__ float2 win[CS_WIN_CAP];
+    __shared__ int s_off[NH + 1];
+    __shared__ int s_lo[NH];
+    __shared__ int s_C[NH];
+
+    /* XCD-aware mapping: consecutive tiles land on the same XCD's L2. */
+    const int nblk = gridDim.x;
+    const int per = nblk >> 3;
+    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (tile >= a.ntiles) return;
+
+    const int4 T = a.tiles[tile];
+    const ChunkDesc *C = a.chunks + T.x;
+    const long long i0 = C->i0;
+    const unsigned long long filled = C->filled;
+    const int tid = threadIdx.x;
+    const bool active = tid < T.z;
+    const int lj = T.y + (active ? tid : T.z - 1);
+    const long long t = i0 + lj;
+    const int q = a.q;
+    constexpr int m2 = M / 2;
+
+    /* per-harmonic windows of this tile */
+    if (tid < NH) {
+        const int h = tid + 1;
+        int len = 0, lo = 0, cc = 0;
+        if ((filled >> tid) & 1ull) {
+            const int2 hi = a.hinfo[tid];
+            const long long tf = i0 + T.y, tl = i0 + T.y + T.z - 1;
+            const long long af = tf * a.pnum, al = tl * a.pnum;
+            const int rf = (int)(af % q), rl = (int)(al % q);
+            const long long qf = af / q, ql = al / q;
+            const int hrf = h * rf, hrl = h * rl;
+            const int resf = hrf % q, resl = hrl % q;
+            const long long bf = hi.x + (long long)h * qf + hrf / q + (resf >= hi.y ? 1 : 0);
+            const long long bl = hi.x + (long long)h * ql + hrl / q + (resl >= hi.y ? 1 : 0);
+            lo = (int)(bf + 1 - m2);
+            len = (int)(bl - bf) + M;
+            cc = hi.x + 1 - m2 - lo;
+        }
+        s_lo[tid] = lo;
+        s_C[tid] = cc;
+        s_off[tid + 1] = len;
+    }
+    if (tid == 0) s_off[0] = 0;
+    __syncthreads();
+    if (tid < 64) {
+        /* inclusive scan of lengths over the first wave */
+        int v = (tid < NH) ? s_off[tid + 1] : 0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(v, d);
+            if (tid >= d) v += u;
+        }
+        if (tid < NH) s_off[tid + 1] = v;
+    }
+    __syncthreads();
+    if (tid < NH) s_C[tid] += s_off[tid];
+    __syncthreads();
+
+    /* per-lane exact residue walk: t*pnum = qb*q + rho */
+    const long long tp = t * a.pnum;
+    const int rho = (int)(tp % q);
+    const int qb = (int)(tp / q);
+
+    cf X[NH + 1];
+    sfor<NH + 1>([&](auto I) { X[decltype(I)::value] = {0.0f, 0.0f}; });
+
+    int hb0 = 0;
+    while (hb0 < NH) {
+        const int base = s_off[hb0];
+        int hb1 = hb0;
+        while (hb1 < NH && s_off[hb1 + 1] - base <= CS_WIN_CAP) hb1++;
+        if (hb1 == hb0) break;  /* host guarantees every window fits */
+        const int cnt = s_off[hb1] - base;
+        for (int idx = tid; idx < cnt; idx += CS_TILE) {
+            int lo = hb0, hi = hb1 - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] - base <= idx) lo = mid;
+                else hi = mid - 1;
+            }
+            const long long ai = (long long)s_lo[lo] + (idx - (s_off[lo] - base));
+            float2 v = make_float2(0.0f, 0.0f);
+            if (ai >= 0 && ai < a.namps) v = a.amps[ai];
+            win[idx] = v;
+        }
+        __syncthreads();
+
+        int res = 0, qi = 0;
+        sfor<NH>([&](auto I) {
+            constexpr int h = decltype(I)::value + 1;
+            res += rho;
+            qi += qb;
+            if (res >= q) { res -= q; qi += 1; }
+            if (((filled >> (h - 1)) & 1ull) && (h - 1) >= hb0 && (h - 1) < hb1) {
+                const int carry = res >= a.hinfo[h - 1].y ? 1 : 0;
+                const int wi = s_C[h - 1] - base + qi + carry;
+                const float *wr = a.wtab + ((size_t)(h - 1) * q + res) * a.ws;
+                float ar = 0.0f, ai = 0.0f;
+                sfor<M / 4>([&](auto J) {
+                    constexpr int j = 4 * decltype(J)::value;
+                    const float4 w4 = *reinterpret_cast<const float4 *>(wr + j);
+                    const float2 a0 = win[wi + j], a1 = win[wi + j + 1];
+                    const float2 a2 = win[wi + j + 2], a3 = win[wi + j + 3];
+                    ar = __builtin_fmaf(w4.x, a0.x, ar); ai = __builtin_fmaf(w4.x, a0.y, ai);
+                    ar = __builtin_fmaf(w4.y, a1.x, ar); ai = __builtin_fmaf(w4.y, a1.y, ai);
+                    ar = __builtin_fmaf(w4.z, a2.x, ar); ai = __builtin_fmaf(w4.z, a2.y, ai);
+                    ar = __builtin_fmaf(w4.w, a3.x, ar); ai = __builtin_fmaf(w4.w, a3.y, ai);
+                });
+                const float2 A = *reinterpret_cast<const float2 *>(wr + M);
+                const float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
+                X[h] = {p1 - p2, p3 + p4};
+            }
+        });
+        hb0 = hb1;
+        __syncthreads();
+    }
+
+    if (a.stack_out != nullptr) {
+        const long long d = t - a.dump_t0;
+        if (active && d >= 0 && d < a.dump_n) {
+            sfor<NH + 1>([&](auto I) {
+                constexpr int h = decltype(I)::value;
+                a.stack_out[d * (NH + 1) + h] = make_float2(X[h].x, X[h].y);
+            });
+        }
+    }
+
+    const double thr = a.threshold;
+    const int chunk = T.x;
+
+    /* decimations k not dividing nharms: strided c2r of the stack */
+    sfor<KMAX - 1>([&](auto I) {
+        constexpr int k = decltype(I)::value + 2;
+        if constexpr (NH / k >= 1 && NH % k != 0) {
+            const int slot = a.kslot[k];
+            if (slot >= 0) {
+                constexpr int Hk = NH / k;
+                float xk[2 * Hk];
+                c2r<Hk, k, NH + 1>(X, xk, a.tw);
+                const float mv = boxcar<2 * Hk>(xk, a.wmask[slot], C->invsw[slot], C->valid[slot]);
+                emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
+            }
+        }
+    });
+
+    /* base profile */
+    float x[2 * NH];
+    c2r<NH, 1, NH + 1>(X, x, a.tw);
+
+    /* divisor decimations fold the base profile (src/search.c:298-331) */
+    sfor<KMAX - 1>([&](auto I) {
+        constexpr int k = decltype(I)::value + 2;
+        if constexpr (NH / k >= 1 && NH % k == 0) {
+            const int slot = a.kslot[k];
+            if (slot >= 0) {
+                constexpr int nk = 2 * (NH / k);
+                constexpr float invk = 1.0f / (float)k;
+                float xk[nk];
+                sfor<nk>([&](auto J) {
+                    constexpr int j = decltype(J)::value;
+                    float s = x[j];
+                    sfor<k - 1>([&](auto Q) {
+                        constexpr int qq = decltype(Q)::value + 1;
+                        s = s + x[j + qq * nk];
+                    });
+                    xk[j] = s * invk;
+                });
+                const float mv = boxcar<nk>(xk, a.wmask[slot], C->invsw[slot], C->valid[slot]);
+                emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
+            }
+        }
+    });
+
+    const float mb = boxcar<2 * NH>(x, a.wmask[0], C->invsw[0], C->valid[0]);
+    emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
+}
+
+}  // namespace csk
+
+/* Instantiation + launch helper, one per translation unit. */
+#define CS_FAST_INSTANTIATE(NH, M, KMAX)                                         \
+    extern "C" __attribute__((visibility("hidden"))) hipError_t                   \
+    cs_launch_fast_##NH##_##M##_##KMAX(const csk::FastArgs *a, int nblocks,      \
+                                       hipStream_t st)                           \
+    {                                                                            \
+        hipLaunchKernelGGL((csk::fast_search_kernel<NH, M, KMAX>), dim3(nblocks), \
+                           dim3(CS_TILE), 0, st, *a);                            \
+        return hipGetLastError();                                                \
+    }

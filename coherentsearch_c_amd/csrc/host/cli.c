@@ -1,0 +1,340 @@
+/*
+ * Option-compatible driver for coherent_search / coherent_search_list
+ * (src/cli.c:351-1116): the same options and defaults, output naming
+ * (<stem>.cohout per input unless a single input without --outdir, which
+ * goes to -o or stdout), output-collision check, list files relative to the
+ * list's directory, per-file search_file with one shared SearchCache, then
+ * sort by S/N, truncate to --ncands, duty cycles and .cohout.  The search
+ * always runs on the GPU (--gpu N picks the device, default 0).  Options that
+ * need the reference's CPU-only code paths (--metricstats, --normalize,
+ * --sigma measured is allowed) or its plotting/zapping tools are rejected
+ * with exit status 2; the rest of the CPU-only knobs are accepted and
+ * ignored.
+ */
+#define _GNU_SOURCE
+#include "host_internal.h"
+
+#include <errno.h>
+#include <sys/stat.h>
+#include <sys/types.h>
+
+typedef struct {
+    char **files;
+    int nfiles, files_owned;
+    double threshold;
+    const char *outfilenm, *outdir;
+    int nharms, ncands, lobin, m, blocksize, maxdecim, numharm;
+    double lofreq, hifreq, hidr, drtol;
+    cs_precision precision;
+    cs_sigma sigma;
+    int remove, harm_remove, verbose, gpu_device;
+} Opts;
+
+static void usage(FILE *f, int list)
+{
+    if (list) fprintf(f, "Usage: coherent_search_list [options] FILE.dat.list\n");
+    else fprintf(f, "Usage: coherent_search [options] FILE.fft|FILE.dat [FILE2 ...]\n");
+    fprintf(f,
+"\nCoherent harmonic-summing pulsar search on an AMD Instinct GPU.\n"
+"Options (defaults as the reference):\n"
+"  -t, --threshold X   S/N cutoff (8.0)        -o, --outputfilenm F  single output\n"
+"      --outdir D      per-file .cohout dir    -n, --nharms N        harmonics (60)\n"
+"      --ncands N      max candidates (100)        --lobin N         (100)\n"
+"      --lofreq F      (0.1 Hz)                    --hifreq F        (125 Hz)\n"
+"      --hidr F        (0.5)                       --m N             taps (16)\n"
+"      --precision P   f32|f64                     --sigma S         analytic|measured\n"
+"      --blocksize N   trials per chunk (65536)    --maxdecim N      (6)\n"
+"      --drtol F       (1.0)                       --numharm N       (16)\n"
+"      --noremove      --noharmremove              --gpu [N]         device (0)\n"
+"  -v, --verbose       --threads N  --nowisdom  --wisdomfile F  --progressbar\n"
+"      --noprogress    --noplot     --help\n");
+}
+
+static int parse_d(const char *s, double *v)
+{
+    char *e = NULL;
+    errno = 0;
+    const double x = strtod(s, &e);
+    if (errno || !e || *e) return -1;
+    *v = x;
+    return 0;
+}
+static int parse_i(const char *s, int *v)
+{
+    char *e = NULL;
+    errno = 0;
+    const long x = strtol(s, &e, 10);
+    if (errno || !e || *e) return -1;
+    *v = (int)x;
+    return 0;
+}
+
+static char *path_stem(const char *path)
+{
+    const size_t n = strlen(path);
+    size_t slash = 0, dot = n;
+    for (size_t i = 0; i < n; i++) if (path[i] == '/') slash = i + 1;
+    for (size_t i = slash; i < n; i++) if (path[i] == '.') dot = i;
+    if (dot == n || dot == slash) dot = n;
+    char *r = (char *)malloc(dot + 1);
+    if (!r) return NULL;
+    memcpy(r, path, dot);
+    r[dot] = '\0';
+    return r;
+}
+
+static char *output_path(const char *in, const char *outfilenm, const char *outdir, int nfiles)
+{
+    if (nfiles == 1 && (!outdir || !*outdir)) return strdup(outfilenm ? outfilenm : "");
+    char *stem = path_stem(in);
+    if (!stem) return NULL;
+    char *name;
+    if (outdir && *outdir) {
+        const char *bn = strrchr(stem, '/');
+        bn = bn ? bn + 1 : stem;
+        const size_t n = strlen(outdir) + strlen(bn) + 2;
+        name = (char *)malloc(n);
+        snprintf(name, n, "%s/%s", outdir, bn);
+        free(stem);
+    } else {
+        name = stem;
+    }
+    const size_t n2 = strlen(name) + 8;
+    char *out = (char *)malloc(n2);
+    snprintf(out, n2, "%s.cohout", name);
+    free(name);
+    return out;
+}
+
+static int cmp_str(const void *a, const void *b)
+{
+    return strcmp(*(char *const *)a, *(char *const *)b);
+}
+
+static int collisions(char **files, int n, const char *outdir)
+{
+    if (n <= 1) return 0;
+    char **o = (char **)malloc((size_t)n * sizeof(char *));
+    for (int i = 0; i < n; i++) o[i] = output_path(files[i], "", outdir, 2);
+    qsort(o, (size_t)n, sizeof(char *), cmp_str);
+    int bad = 0;
+    for (int i = 1; i < n && !bad; i++)
+        if (strcmp(o[i], o[i - 1]) == 0) {
+            fprintf(stderr, "Error: several inputs would write the same candidate file "
+                            "'%s'\n", o[i]);
+            bad = 1;
+        }
+    for (int i = 0; i < n; i++) free(o[i]);
+    free(o);
+    return bad;
+}
+
+static char **read_list(const char *listpath, int *nout)
+{
+    FILE *f = fopen(listpath, "r");
+    if (!f) {
+        fprintf(stderr, "Error: cannot open list file '%s'\n", listpath);
+        return NULL;
+    }
+    char *dir = strdup(listpath);
+    char *sl = strrchr(dir, '/');
+    if (sl) *sl = '\0';
+    else dir[0] = '\0';
+    char **v = NULL, *line = NULL;
+    int n = 0, cap = 0;
+    size_t lcap = 0;
+    while (getline(&line, &lcap, f) >= 0) {
+        char *s = line;
+        while (*s == ' ' || *s == '\t' || *s == '\r' || *s == '\n') s++;
+        char *e = s + strlen(s);
+        while (e > s && (e[-1] == ' ' || e[-1] == '\t' || e[-1] == '\r' || e[-1] == '\n'))
+            *--e = '\0';
+        if (!*s || *s == '#') continue;
+        char *full;
+        if (s[0] == '/' || dir[0] == '\0') full = strdup(s);
+        else {
+            const size_t need = strlen(dir) + strlen(s) + 2;
+            full = (char *)malloc(need);
+            snprintf(full, need, "%s/%s", dir, s);
+        }
+        if (n == cap) {
+            cap = cap ? 2 * cap : 64;
+            v = (char **)realloc(v, (size_t)cap * sizeof(char *));
+        }
+        v[n++] = full;
+    }
+    free(line);
+    free(dir);
+    fclose(f);
+    *nout = n;
+    return v;
+}
+
+static void mkdir_p(const char *path)
+{
+    char *p = strdup(path);
+    for (char *q = p + 1; *q; q++)
+        if (*q == '/') { *q = '\0'; mkdir(p, 0777); *q = '/'; }
+    mkdir(p, 0777);
+    free(p);
+}
+
+static int run(int argc, char **argv, int list)
+{
+    Opts o;
+    memset(&o, 0, sizeof(o));
+    o.threshold = 8.0; o.outfilenm = ""; o.outdir = ""; o.nharms = 60; o.ncands = 100;
+    o.lobin = 100; o.lofreq = 0.1; o.hifreq = 125.0; o.hidr = 0.5; o.m = 16;
+    o.precision = CS_PREC_F32; o.sigma = CS_SIGMA_ANALYTIC; o.maxdecim = 6; o.drtol = 1.0;
+    o.numharm = 16; o.remove = 1; o.harm_remove = 1;
+    o.files = (char **)calloc((size_t)argc + 1, sizeof(char *));
+    for (int i = 1; i < argc; i++) {
+        char *arg = argv[i];
+        if (!strcmp(arg, "--help") || !strcmp(arg, "-h")) { usage(stdout, list); return 0; }
+        if (arg[0] != '-' || !strcmp(arg, "-")) { o.files[o.nfiles++] = arg; continue; }
+        char *name = arg, *val = NULL;
+        if (!strncmp(arg, "--", 2)) {
+            char *eq = strchr(arg, '=');
+            if (eq) { *eq = '\0'; val = eq + 1; }
+        }
+#define NEED() do { if (!val) { if (i + 1 >= argc) { fprintf(stderr, "Error: %s needs a value\n", name); return 2; } val = argv[++i]; } } while (0)
+#define BAD(what) do { fprintf(stderr, "Error: bad %s '%s'\n", what, val); return 2; } while (0)
+        if (!strcmp(name, "-t") || !strcmp(name, "--threshold")) { NEED(); if (parse_d(val, &o.threshold)) BAD("threshold"); }
+        else if (!strcmp(name, "-o") || !strcmp(name, "--outputfilenm")) { NEED(); o.outfilenm = val; }
+        else if (!strcmp(name, "--outdir")) { NEED(); o.outdir = val; }
+        else if (!strcmp(name, "-n") || !strcmp(name, "--nharms")) { NEED(); if (parse_i(val, &o.nharms)) BAD("nharms"); }
+        else if (!strcmp(name, "--ncands")) { NEED(); if (parse_i(val, &o.ncands)) BAD("ncands"); }
+        else if (!strcmp(name, "--lobin")) { NEED(); if (parse_i(val, &o.lobin)) BAD("lobin"); }
+        else if (!strcmp(name, "--lofreq")) { NEED(); if (parse_d(val, &o.lofreq)) BAD("lofreq"); }
+        else if (!strcmp(name, "--hifreq")) { NEED(); if (parse_d(val, &o.hifreq)) BAD("hifreq"); }
+        else if (!strcmp(name, "--hidr")) { NEED(); if (parse_d(val, &o.hidr)) BAD("hidr"); }
+        else if (!strcmp(name, "--m")) { NEED(); if (parse_i(val, &o.m)) BAD("m"); }
+        else if (!strcmp(name, "--precision")) {
+            NEED();
+            if (!strcmp(val, "f32")) o.precision = CS_PREC_F32;
+            else if (!strcmp(val, "f64")) o.precision = CS_PREC_F64;
+            else { fprintf(stderr, "Error: precision must be f32 or f64\n"); return 2; }
+        } else if (!strcmp(name, "--sigma")) {
+            NEED();
+            if (!strcmp(val, "analytic")) o.sigma = CS_SIGMA_ANALYTIC;
+            else if (!strcmp(val, "measured")) o.sigma = CS_SIGMA_MEASURED;
+            else { fprintf(stderr, "Error: sigma must be analytic or measured\n"); return 2; }
+        } else if (!strcmp(name, "--blocksize")) { NEED(); if (parse_i(val, &o.blocksize)) BAD("blocksize"); }
+        else if (!strcmp(name, "--maxdecim")) { NEED(); if (parse_i(val, &o.maxdecim)) BAD("maxdecim"); }
+        else if (!strcmp(name, "--drtol")) { NEED(); if (parse_d(val, &o.drtol)) BAD("drtol"); }
+        else if (!strcmp(name, "--numharm")) { NEED(); if (parse_i(val, &o.numharm)) BAD("numharm"); }
+        else if (!strcmp(name, "--threads")) {
+            NEED();
+            int t = 0;
+            if (parse_i(val, &t) || t < 1) { fprintf(stderr, "Error: --threads expects a positive integer\n"); return 2; }
+        } else if (!strcmp(name, "--wisdomfile")) { NEED(); }
+        else if (!strcmp(name, "--noremove")) o.remove = 0;
+        else if (!strcmp(name, "--noharmremove")) o.harm_remove = 0;
+        else if (!strcmp(name, "-v") || !strcmp(name, "--verbose")) o.verbose = 1;
+        else if (!strcmp(name, "--progressbar") || !strcmp(name, "--noprogress") ||
+                 !strcmp(name, "--nowisdom") || !strcmp(name, "--noplot")) { }
+        else if (!strcmp(name, "--gpu")) {
+            if (val) {
+                if (parse_i(val, &o.gpu_device) || o.gpu_device < 0) {
+                    fprintf(stderr, "Error: --gpu expects a non-negative device index\n");
+                    return 2;
+                }
+            } else if (i + 1 < argc) {
+                char *e = NULL;
+                const long v = strtol(argv[i + 1], &e, 10);
+                if (e && e != argv[i + 1] && *e == '\0' && v >= 0) { o.gpu_device = (int)v; i++; }
+            }
+        } else if (!strcmp(name, "--metricstats") || !strcmp(name, "--normalize") ||
+                   !strcmp(name, "--plot") || !strcmp(name, "--plotstem") ||
+                   !strcmp(name, "--plotcols") || !strcmp(name, "--plotrows") ||
+                   !strcmp(name, "--plot-candidates") || !strcmp(name, "--zaplist") ||
+                   !strcmp(name, "--baryv") || !strcmp(name, "--prime-wisdom")) {
+            fprintf(stderr, "Error: option '%s' is not supported by this GPU build\n", name);
+            return 2;
+        } else {
+            fprintf(stderr, "Error: unknown option '%s'\n", name);
+            usage(stderr, list);
+            return 2;
+        }
+#undef NEED
+#undef BAD
+    }
+    if (list) {
+        if (o.nfiles != 1) { fprintf(stderr, "Error: coherent_search_list expects exactly one .list file\n"); return 2; }
+        if (o.outfilenm[0]) { fprintf(stderr, "Error: -o is not allowed with a list (use --outdir)\n"); return 2; }
+        int n = 0;
+        char **v = read_list(o.files[0], &n);
+        if (!v || n == 0) { fprintf(stderr, "Error: list file '%s' contains no inputs\n", o.files[0]); return 2; }
+        free(o.files);
+        o.files = v;
+        o.nfiles = n;
+        o.files_owned = 1;
+    } else if (o.nfiles == 0) {
+        fprintf(stderr, "Error: at least one FFT file is required\n");
+        usage(stderr, list);
+        return 2;
+    }
+    if (o.nfiles > 1 && o.outfilenm[0]) {
+        fprintf(stderr, "Error: -o/--outputfilenm names one output file but %d FFT files were "
+                        "given; omit it or use --outdir\n", o.nfiles);
+        return 2;
+    }
+    if (o.m <= 0 || (o.m & 1)) { fprintf(stderr, "Error: m must be a positive even integer\n"); return 2; }
+    if (o.nharms < 1) { fprintf(stderr, "Error: nharms must be positive\n"); return 2; }
+    char gerr[256] = {0};
+    if (cs_gpu_available(o.gpu_device, gerr, sizeof(gerr)) != 1) {
+        fprintf(stderr, "Error: %s\n", gerr);
+        return 2;
+    }
+    if (o.blocksize == 0) o.blocksize = CS_GPU_DEFAULT_BLOCKSIZE;
+    if (collisions(o.files, o.nfiles, o.outdir)) return 2;
+    if (o.outdir[0]) mkdir_p(o.outdir);
+
+    SearchParams params;
+    params_defaults(&params);
+    params.nharms = o.nharms;
+    params.m = o.m;
+    params.hidr = o.hidr;
+    params.threshold = o.threshold;
+    params.precision = o.precision;
+    params.sigma = o.sigma;
+    params.ndecim = decimation_set(o.nharms, o.maxdecim, params.decimations, 64);
+
+    SearchCache cache;
+    memset(&cache, 0, sizeof(cache));
+    DatPipe *pipe = datpipe_create(6, 100, 6.0);
+    int status = 0;
+    for (int fi = 0; fi < o.nfiles; fi++) {
+        const char *path = o.files[fi];
+        FFTFile ft;
+        int is_dat = 0;
+        if (cs_open_input(path, pipe, &ft, &is_dat) != 0) {
+            fprintf(stderr, "Error: failed to open '%s'\n", path);
+            status = 1;
+            continue;
+        }
+        CandidateList c = search_file(&ft, &params, o.lofreq, o.hifreq, o.lobin, o.blocksize,
+                                      o.threshold, o.remove, o.drtol, o.harm_remove,
+                                      o.numharm, NULL, 0, o.verbose, 0, NULL, &cache, 1.0,
+                                      o.gpu_device);
+        candlist_sort_metric_desc(&c);
+        if ((int)c.n > o.ncands) c.n = (size_t)o.ncands;
+        CandidateList d = measure_ducy(&ft, &c, &params);
+        candlist_free(&c);
+        const int names = list ? (o.nfiles > 1 ? o.nfiles : 2) : o.nfiles;
+        char *outfile = output_path(path, o.outfilenm, o.outdir, names);
+        if (write_candidates(&d, outfile ? outfile : "", o.threshold) != 0) status = 1;
+        free(outfile);
+        candlist_free(&d);
+        fftfile_close(&ft);
+    }
+    search_cache_free(&cache);
+    datpipe_free(pipe);
+    if (o.files_owned)
+        for (int i = 0; i < o.nfiles; i++) free(o.files[i]);
+    free(o.files);
+    return status;
+}
+
+int cs_main(int argc, char **argv) { return run(argc, argv, 0); }
+int cs_main_list(int argc, char **argv) { return run(argc, argv, 1); }

@@ -1,0 +1,210 @@
+"""Python view of the host interface (reference src/search.c search_file,
+src/interp.c plans/chunks, src/cuda.cu sessions), over the C-ABI libraries.
+
+Everything here forwards to libcs_host.so / libcs_hip.so; the numerical work
+happens on the GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+CAND_DTYPE = np.dtype([("freq", "f8"), ("metric", "f8"), ("r", "f8"),
+                       ("nharm", "i4"), ("pad", "i4"), ("ducy", "f8")])
+
+
+def make_params(nharms=60, m=16, hidr=0.5, maxdecim=6, precision="f32",
+                sigma="analytic", threshold=8.0, decimations=None):
+    """SearchParams as cli.c:943-951 builds them."""
+    L = abi.host()
+    p = abi.SearchParams()
+    L.params_defaults(C.byref(p))
+    p.nharms, p.m, p.hidr, p.threshold = nharms, m, hidr, threshold
+    p.precision = 1 if precision == "f64" else 0
+    p.sigma = 1 if sigma == "measured" else 0
+    if decimations is None:
+        p.ndecim = L.decimation_set(nharms, maxdecim, p.decimations, 64)
+    else:
+        p.ndecim = len(decimations)
+        for i, k in enumerate(decimations):
+            p.decimations[i] = k
+    return p
+
+
+class Spectrum:
+    """An in-memory .fft: namps complex amplitudes (interleaved float32),
+    N time samples of width dt (T = N*dt)."""
+
+    def __init__(self, amps, N, dt, path="memory"):
+        a = np.asarray(amps)
+        if np.iscomplexobj(a):
+            a = a.astype(np.complex64).view(np.float32)
+        self.amps = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+        self.N, self.dt = int(N), float(dt)
+        self.T = self.N * self.dt
+        self._path = path.encode()
+        f = abi.FFTFile()
+        f.path = self._path
+        f.N = self.N
+        f.T = self.T
+        f.df = 1.0 / self.T
+        f.amps = self.amps.ctypes.data_as(C.POINTER(C.c_float))
+        f.namps = self.amps.size // 2
+        f.inf.N, f.inf.has_N, f.inf.dt, f.inf.has_dt = self.N, 1, self.dt, 1
+        f.DC = float(self.amps[0]) if self.amps.size else 0.0
+        f.Nyquist = float(self.amps[1]) if self.amps.size > 1 else 0.0
+        self.ft = f
+
+    @property
+    def namps(self):
+        return self.amps.size // 2
+
+
+def redfft(samples, dt, path="memory"):
+    """realfft + rednoise of a float32 series in memory (datpipe_redfft_mem)."""
+    L = abi.host()
+    s = np.ascontiguousarray(samples, dtype=np.float32)
+    pipe = L.datpipe_create(6, 100, 6.0)
+    ft = abi.FFTFile()
+    rc = L.datpipe_redfft_mem(pipe, s.ctypes.data_as(C.c_void_p), s.size, dt,
+                              path.encode(), C.byref(ft))
+    if rc != 0:
+        L.datpipe_free(pipe)
+        raise ValueError("realfft/rednoise failed (N must be even and > 2*startwidth)")
+    amps = np.ctypeslib.as_array(ft.amps, shape=(2 * ft.namps,)).copy()
+    L.fftfile_close(C.byref(ft))
+    L.datpipe_free(pipe)
+    return Spectrum(amps, s.size, dt, path)
+
+
+def take_list(cl):
+    """Copy a CandidateList into a numpy array and free it."""
+    L = abi.host()
+    n = int(cl.n)
+    if n:
+        buf = C.string_at(C.cast(cl.v, C.c_void_p), n * C.sizeof(abi.Candidate))
+        arr = np.frombuffer(buf, dtype=CAND_DTYPE).copy()
+    else:
+        arr = np.zeros(0, dtype=CAND_DTYPE)
+    L.candlist_free(C.byref(cl))
+    return arr
+
+
+def to_list(arr):
+    """numpy candidates -> CandidateList owning malloc'ed memory (free with
+    candlist_free)."""
+    arr = np.ascontiguousarray(arr, dtype=CAND_DTYPE)
+    cl = abi.CandidateList()
+    n = arr.size
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    p = libc.malloc(max(n, 1) * C.sizeof(abi.Candidate))
+    C.memmove(p, arr.ctypes.data, n * C.sizeof(abi.Candidate))
+    cl.v = C.cast(p, C.POINTER(abi.Candidate))
+    cl.n = n
+    cl.cap = n
+    return cl
+
+
+class Session:
+    """Plans + chunk table (host) and a resident GPU session for one
+    spectrum geometry (search.c:1257-1341 then cuda.cu session_create)."""
+
+    def __init__(self, spec, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=65536,
+                 device=0):
+        L = abi.host()
+        self.L, self.spec, self.params = L, spec, params
+        self.lodr = params.hidr / params.nharms
+        self.r_lo = lofreq * spec.T if lobin == 100 else float(lobin)
+        r_hi = hifreq * spec.T
+        self.total = max(int(np.floor((r_hi - self.r_lo) / self.lodr)) + 1, 0)
+        self.Nprof = blocksize if blocksize > 0 else 65536
+        self.plans = abi.DirectPlanSet()
+        if L.direct_plans_build(C.byref(self.plans), C.byref(params), self.r_lo) != 0:
+            raise ValueError("plan build failed")
+        cs = C.POINTER(C.c_longlong)()
+        nc = C.c_longlong()
+        if L.chunk_starts(C.byref(self.plans), C.byref(spec.ft), C.byref(params), self.total,
+                          self.Nprof, C.byref(cs), C.byref(nc)) != 0:
+            raise ValueError("chunk table failed")
+        self.nchunks = nc.value
+        self.cstarts = np.ctypeslib.as_array(cs, shape=(max(self.nchunks, 1),))[:self.nchunks].copy()
+        L.free_(C.cast(cs, C.c_void_p))
+        self._cs = np.ascontiguousarray(self.cstarts, dtype=np.int64)
+        self.h = L.cs_gpu_session_create(
+            C.byref(spec.ft), C.byref(params), C.byref(self.plans),
+            self._cs.ctypes.data_as(C.POINTER(C.c_longlong)), self.total, self.Nprof,
+            self.nchunks, self.r_lo, self.lodr, device)
+        if not self.h:
+            L.direct_plans_free(C.byref(self.plans))
+            raise RuntimeError("cs_gpu_session_create failed (see stderr)")
+
+    def run(self, threshold):
+        return take_list(self.L.cs_gpu_session_run(self.h, C.byref(self.spec.ft), threshold))
+
+    def upload(self, spec=None):
+        spec = spec or self.spec
+        if self.L.cs_gpu_session_upload(self.h, C.byref(spec.ft)) != 0:
+            raise RuntimeError("spectrum upload failed")
+
+    def search_resident(self, threshold):
+        return take_list(self.L.cs_gpu_session_search_resident(self.h, threshold))
+
+    def dump_stack(self, t0, n):
+        out = np.zeros((n, self.params.nharms + 1, 2), dtype=np.float32)
+        rc = self.L.cs_gpu_session_dump_stack(self.h, C.byref(self.spec.ft), t0, n,
+                                              out.ctypes.data_as(C.c_void_p))
+        if rc != 0:
+            raise RuntimeError("stack dump failed")
+        return out
+
+    def kernel_ms(self):
+        return self.L.cs_gpu_session_last_kernel_ms(self.h)
+
+    def work(self):
+        ht, tr, b = C.c_longlong(), C.c_longlong(), C.c_double()
+        self.L.cs_gpu_session_work(self.h, C.byref(ht), C.byref(tr), C.byref(b))
+        return ht.value, tr.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cs_gpu_session_free(self.h)
+            self.h = None
+            self.L.direct_plans_free(C.byref(self.plans))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def search_file(spec, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
+                threshold=8.0, remove=True, dr_tol=1.0, harm_remove=True, numharm=16,
+                device=0, cache=None, harm_tol=1.0):
+    """search.c:1247-1486 (GPU path): raw search then collapse."""
+    L = abi.host()
+    cl = L.search_file(C.byref(spec.ft), C.byref(params), lofreq, hifreq, lobin, blocksize,
+                       threshold, int(remove), dr_tol, int(harm_remove), numharm, None, 0, 0,
+                       0, None, C.byref(cache) if cache is not None else None, harm_tol,
+                       device)
+    return take_list(cl)
+
+
+def finalize(spec, params, cands, ncands=100):
+    """cli.c:1059-1061: sort by S/N, truncate, measure duty cycles."""
+    L = abi.host()
+    cl = to_list(cands)
+    L.candlist_sort_metric_desc(C.byref(cl))
+    if cl.n > ncands:
+        cl.n = ncands
+    d = L.measure_ducy(C.byref(spec.ft), C.byref(cl), C.byref(params))
+    L.candlist_free(C.byref(cl))
+    return take_list(d)
+
+
+def gpu_available(device=0):
+    buf = C.create_string_buffer(256)
+    rc = abi.host().cs_gpu_available(device, buf, 256)
+    return rc, buf.value.decode()
