@@ -23,7 +23,9 @@
 #define CS_NSEQ 12             /* width candidates 1,2,3,4,6,9,13,19,28,42,63,94 */
 #define CS_TW_NMAX 257
 #define CS_TW_SIZE (CS_TW_NMAX * (CS_TW_NMAX - 1) / 2)
-#define CS_WIN_CAP 6144        /* float2 window slots in LDS per workgroup */
+/* LDS window slots (float2) per workgroup for nh harmonics of m taps: covers
+ * sum_h (ceil(255*h*lodr) + m + 2) whenever hidr <= 0.5, rounded to 256. */
+#define CS_WIN_CAP(nh, m) (((64 * ((nh) + 1) + (nh) * ((m) + 4)) + 255) / 256 * 256)
 
 typedef struct {
     long long i0;

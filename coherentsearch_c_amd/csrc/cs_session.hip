@@ -327,10 +327,14 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
         fprintf(stderr, "Error: trial grid too large for the GPU residue walk\n");
         goto fail;
     }
-    /* every harmonic window of a tile must fit the LDS buffer */
-    if ((double)CS_TILE * nh * lodr + m + 2 > (double)CS_WIN_CAP) {
-        fprintf(stderr, "Error: --hidr too large for the GPU tile window\n");
-        goto fail;
+    /* all harmonic windows of a tile must fit the kernel's LDS buffer */
+    {
+        double need = 0.0;
+        for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + m + 2;
+        if (need > (double)CS_WIN_CAP(nh, m)) {
+            fprintf(stderr, "Error: --hidr too large for the GPU tile window\n");
+            goto fail;
+        }
     }
 
     /* plan arena: row (h, res) = W[0..m-1], A.re, A.im, 0, 0 */

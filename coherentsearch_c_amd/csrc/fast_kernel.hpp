@@ -280,12 +280,16 @@ __device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, floa
     }
 }
 
+/* Occupancy target: 2 waves/SIMD for the largest stacks, 3 below. */
+constexpr int waves_for(int nh) { return nh > 32 ? 2 : 3; }
+
 template <int NH, int M, int KMAX>
-__global__ __launch_bounds__(CS_TILE) void fast_search_kernel(FastArgs a)
+__global__ __launch_bounds__(CS_TILE) __attribute__((amdgpu_waves_per_eu(waves_for(NH))))
+void fast_search_kernel(FastArgs a)
 {
     static_assert(NH >= 1 && NH <= 64, "fast path holds up to 64 harmonics");
     static_assert(M % 4 == 0, "tap count multiple of 4");
-    __shared__ float2 win[CS_WIN_CAP];
+    __shared__ float2 win[CS_WIN_CAP(NH, M)];
     __shared__ int s_off[NH + 1];
     __shared__ int s_lo[NH];
     __shared__ int s_C[NH];
@@ -352,36 +356,35 @@ __global__ __launch_bounds__(CS_TILE) void fast_search_kernel(FastArgs a)
     cf X[NH + 1];
     sfor<NH + 1>([&](auto I) { X[decltype(I)::value] = {0.0f, 0.0f}; });
 
-    int hb0 = 0;
-    while (hb0 < NH) {
-        const int base = s_off[hb0];
-        int hb1 = hb0;
-        while (hb1 < NH && s_off[hb1 + 1] - base <= CS_WIN_CAP) hb1++;
-        if (hb1 == hb0) break;  /* host guarantees every window fits */
-        const int cnt = s_off[hb1] - base;
-        for (int idx = tid; idx < cnt; idx += CS_TILE) {
-            int lo = hb0, hi = hb1 - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] - base <= idx) lo = mid;
-                else hi = mid - 1;
-            }
-            const long long ai = (long long)s_lo[lo] + (idx - (s_off[lo] - base));
-            float2 v = make_float2(0.0f, 0.0f);
-            if (ai >= 0 && ai < a.namps) v = a.amps[ai];
-            win[idx] = v;
+    /* All windows of the tile at once: the host admits a configuration only
+     * when sum_h (ceil(255*h*lodr) + m + 2) <= CS_WIN_CAP(NH, M), so one pass
+     * suffices (a batched loop would let the compiler hoist every harmonic's
+     * residue walk out of it and hold it in registers). */
+    const int cnt = s_off[NH];
+    for (int idx = tid; idx < cnt; idx += CS_TILE) {
+        int lo = 0, hi = NH - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= idx) lo = mid;
+            else hi = mid - 1;
         }
-        __syncthreads();
+        const long long ai = (long long)s_lo[lo] + (idx - s_off[lo]);
+        float2 v = make_float2(0.0f, 0.0f);
+        if (ai >= 0 && ai < a.namps) v = a.amps[ai];
+        win[idx] = v;
+    }
+    __syncthreads();
 
+    {
         int res = 0, qi = 0;
         sfor<NH>([&](auto I) {
             constexpr int h = decltype(I)::value + 1;
             res += rho;
             qi += qb;
             if (res >= q) { res -= q; qi += 1; }
-            if (((filled >> (h - 1)) & 1ull) && (h - 1) >= hb0 && (h - 1) < hb1) {
+            if ((filled >> (h - 1)) & 1ull) {
                 const int carry = res >= a.hinfo[h - 1].y ? 1 : 0;
-                const int wi = s_C[h - 1] - base + qi + carry;
+                const int wi = s_C[h - 1] + qi + carry;
                 const float *wr = a.wtab + ((size_t)(h - 1) * q + res) * a.ws;
                 float ar = 0.0f, ai = 0.0f;
                 sfor<M / 4>([&](auto J) {
@@ -399,8 +402,6 @@ __global__ __launch_bounds__(CS_TILE) void fast_search_kernel(FastArgs a)
                 X[h] = {p1 - p2, p3 + p4};
             }
         });
-        hb0 = hb1;
-        __syncthreads();
     }
 
     if (a.stack_out != nullptr) {
