@@ -26,6 +26,10 @@
 /* LDS window slots (float2) per workgroup for nh harmonics of m taps: covers
  * sum_h (ceil(255*h*lodr) + m + 2) whenever hidr <= 0.5, rounded to 256. */
 #define CS_WIN_CAP(nh, m) (((64 * ((nh) + 1) + (nh) * ((m) + 4)) + 255) / 256 * 256)
+/* Weight-table row stride in floats: m weights, A.re, A.im, padded to an odd
+ * number of 16-byte units, so that the same float4 of different rows falls in
+ * different L1 banks when a wave gathers rows at scattered residues. */
+#define CS_WSP(m) ((((m) + 5) / 4) % 2 ? ((m) + 5) / 4 * 4 : (((m) + 5) / 4 + 1) * 4)
 
 typedef struct {
     long long i0;

@@ -337,8 +337,13 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
         }
     }
 
-    /* plan arena: row (h, res) = W[0..m-1], A.re, A.im, 0, 0 */
-    ws = m + 4;
+    /* plan arena: row (h, res) = W[0..m-1], A.re, A.im, zero padding to
+     * CS_WSP(m) floats; row index (h-1)*q + res must fit 32-bit offsets */
+    ws = CS_WSP(m);
+    if ((double)nh * S->q * ws >= 4294967295.0 / 4) {
+        fprintf(stderr, "Error: weight table too large\n");
+        goto fail;
+    }
     wtab.assign((size_t)nh * S->q * ws, 0.0f);
     hinfo.resize((size_t)nh);
     for (int h = 1; h <= nh; h++) {

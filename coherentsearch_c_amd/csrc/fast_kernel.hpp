@@ -45,15 +45,27 @@ __device__ __forceinline__ void sfor(F &&f)
     sfor_(std::make_integer_sequence<int, N>{}, f);
 }
 
-constexpr int radix_of(int N)
+/* smallest prime factor; the power of it dividing N (the prime-factor split's
+ * first factor); the Cooley-Tukey radix of a prime power */
+constexpr int spf(int N)
 {
-    if (N % 4 == 0) return 4;
     if (N % 2 == 0) return 2;
-    if (N % 3 == 0) return 3;
-    if (N % 5 == 0) return 5;
-    for (int p = 7; p * p <= N; p += 2)
+    for (int p = 3; p * p <= N; p += 2)
         if (N % p == 0) return p;
     return N;
+}
+constexpr int pfa_n1(int N)
+{
+    int n1 = 1;
+    while (N % (n1 * spf(N)) == 0) n1 *= spf(N);
+    return n1;
+}
+constexpr int radix_of(int N) { return N % 4 == 0 ? 4 : (N % 2 == 0 ? 2 : spf(N)); }
+constexpr int modinv(int a, int m)
+{
+    for (int x = 1; x < m; x++)
+        if (a % m * x % m == 1) return x;
+    return 0;
 }
 constexpr int twoff(int N) { return N * (N - 1) / 2; }
 
@@ -82,14 +94,25 @@ constexpr int ncand(int nb)
 }
 constexpr int wcmax(int nb) { return ncand(nb) > 0 ? wseq(ncand(nb) - 1) : 0; }
 
+/* Twiddles are read through the constant address space: every index is a
+ * compile-time constant, so they become scalar loads and SGPR operands. */
+typedef const __attribute__((address_space(4))) float twc_t;
+
+__device__ __forceinline__ float2 ldtw(twc_t *__restrict__ tw, int i)
+{
+    return make_float2(tw[2 * i], tw[2 * i + 1]);
+}
+
+/* y * w with the project c2r's fused form:
+ * (fma(y.re, w.re, -(y.im*w.im)), fma(y.re, w.im, y.im*w.re)) */
 __device__ __forceinline__ cf cmul(cf a, float2 w)
 {
-    const float p1 = a.x * w.x, p2 = a.y * w.y, p3 = a.x * w.y, p4 = a.y * w.x;
-    return {p1 - p2, p3 + p4};
+    const float p2 = a.y * w.y, p4 = a.y * w.x;
+    return {__builtin_fmaf(a.x, w.x, -p2), __builtin_fmaf(a.x, w.y, p4)};
 }
 
 template <int R>
-__device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], const float2 *__restrict__ tw)
+__device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], twc_t *__restrict__ tw)
 {
     if constexpr (R == 2) {
         v[0] = {u[0].x + u[1].x, u[0].y + u[1].y};
@@ -104,6 +127,9 @@ __device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], const float2 
         v[1] = {t1.x - t3.y, t1.y + t3.x};
         v[3] = {t1.x + t3.y, t1.y - t3.x};
     } else {
+        /* odd prime: sums and differences of mirrored inputs, then per
+         * output pair A = u0 + sum sp_r c (fused, left to right) and
+         * B = sum sm_r sn (first term a product, the rest fused) */
         constexpr int hh = (R - 1) / 2;
         cf sp[hh + 1], sm[hh + 1];
         sfor<hh>([&](auto I) {
@@ -123,12 +149,10 @@ __device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], const float2 
             sfor<hh>([&](auto I) {
                 constexpr int r = decltype(I)::value + 1;
                 constexpr int j = (r * s) % R;
-                const float2 w = tw[twoff(R) + j];
-                const float pr = sp[r].x * w.x, pi = sp[r].y * w.x;
-                A = {A.x + pr, A.y + pi};
-                const float qr = sm[r].x * w.y, qi = sm[r].y * w.y;
-                if constexpr (r == 1) B = {qr, qi};
-                else B = {B.x + qr, B.y + qi};
+                const float2 w = ldtw(tw, twoff(R) + j);
+                A = {__builtin_fmaf(sp[r].x, w.x, A.x), __builtin_fmaf(sp[r].y, w.x, A.y)};
+                if constexpr (r == 1) B = {sm[r].x * w.y, sm[r].y * w.y};
+                else B = {__builtin_fmaf(sm[r].x, w.y, B.x), __builtin_fmaf(sm[r].y, w.y, B.y)};
             });
             v[s] = {A.x - B.y, A.y + B.x};
             v[R - s] = {A.x + B.y, A.y - B.x};
@@ -136,19 +160,53 @@ __device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], const float2 
     }
 }
 
-/* out = IDFT_N(in[O + j*S]), decimation in time, radix order of radix_of(). */
-template <int N, int S, int O, int LEN>
-__device__ __forceinline__ void idft(const cf (&in)[LEN], cf (&out)[N],
-                                     const float2 *__restrict__ tw)
+/* X = IDFT_N(x) (e^{+i}, unnormalised).  N with more than one prime: the
+ * prime-factor split N = N1*N2 (N1 the full power of the smallest prime):
+ * DFT_N1 over x[(N2*n1 + N1*n2) mod N] for every n2, then DFT_N2 over n2 for
+ * every k1, written to the CRT index -- no twiddles, only renaming.  A prime
+ * power: decimation in time with radix 4, 2 or the prime, twiddles cmul'd. */
+template <int N>
+__device__ __forceinline__ void idft(const cf (&x)[N], cf (&X)[N], twc_t *__restrict__ tw)
 {
     if constexpr (N == 1) {
-        out[0] = in[O];
+        X[0] = x[0];
+    } else if constexpr (pfa_n1(N) < N) {
+        constexpr int N1 = pfa_n1(N), N2 = N / N1;
+        constexpr int E1 = N2 * modinv(N2, N1), E2 = N1 * modinv(N1, N2);
+        cf T[N2][N1];
+        sfor<N2>([&](auto I2) {
+            constexpr int n2 = decltype(I2)::value;
+            cf g[N1];
+            sfor<N1>([&](auto I1) {
+                constexpr int n1 = decltype(I1)::value;
+                g[n1] = x[(N2 * n1 + N1 * n2) % N];
+            });
+            idft<N1>(g, T[n2], tw);
+        });
+        sfor<N1>([&](auto K1) {
+            constexpr int k1 = decltype(K1)::value;
+            cf c[N2], o[N2];
+            sfor<N2>([&](auto I2) {
+                constexpr int n2 = decltype(I2)::value;
+                c[n2] = T[n2][k1];
+            });
+            idft<N2>(c, o, tw);
+            sfor<N2>([&](auto K2) {
+                constexpr int k2 = decltype(K2)::value;
+                X[(k1 * E1 + k2 * E2) % N] = o[k2];
+            });
+        });
     } else {
         constexpr int R = radix_of(N), M = N / R;
         cf Y[R][M];
         sfor<R>([&](auto I) {
             constexpr int r = decltype(I)::value;
-            idft<M, S * R, O + r * S, LEN>(in, Y[r], tw);
+            cf g[M];
+            sfor<M>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                g[j] = x[r + j * R];
+            });
+            idft<M>(g, Y[r], tw);
         });
         sfor<M>([&](auto K) {
             constexpr int k = decltype(K)::value;
@@ -157,12 +215,12 @@ __device__ __forceinline__ void idft(const cf (&in)[LEN], cf (&out)[N],
             sfor<R - 1>([&](auto I) {
                 constexpr int r = decltype(I)::value + 1;
                 if constexpr (r * k == 0) u[r] = Y[r][k];
-                else u[r] = cmul(Y[r][k], tw[twoff(N) + r * k]);
+                else u[r] = cmul(Y[r][k], ldtw(tw, twoff(N) + r * k));
             });
             bfly<R>(u, v, tw);
             sfor<R>([&](auto I) {
                 constexpr int s = decltype(I)::value;
-                out[k + M * s] = v[s];
+                X[k + M * s] = v[s];
             });
         });
     }
@@ -171,7 +229,7 @@ __device__ __forceinline__ void idft(const cf (&in)[LEN], cf (&out)[N],
 /* Project c2r of X[0], X[XS], ..., X[H*XS] into x[0..2H-1] (unnormalised). */
 template <int H, int XS, int LEN>
 __device__ __forceinline__ void c2r(const cf (&X)[LEN], float (&x)[2 * H],
-                                    const float2 *__restrict__ tw)
+                                    twc_t *__restrict__ tw)
 {
     cf Z[H];
     {
@@ -182,14 +240,11 @@ __device__ __forceinline__ void c2r(const cf (&X)[LEN], float (&x)[2 * H],
         constexpr int k = decltype(I)::value + 1;
         const cf a = X[k * XS], b = X[(H - k) * XS];
         const float er = a.x + b.x, ei = a.y - b.y;
-        const float dr = a.x - b.x, di = a.y + b.y;
-        const float2 w = tw[twoff(2 * H) + k];
-        const float p1 = dr * w.x, p2 = di * w.y, p3 = dr * w.y, p4 = di * w.x;
-        const float o_r = p1 - p2, o_i = p3 + p4;
-        Z[k] = {er - o_i, ei + o_r};
+        const cf o = cmul({a.x - b.x, a.y + b.y}, ldtw(tw, twoff(2 * H) + k));
+        Z[k] = {er - o.y, ei + o.x};
     });
     cf z[H];
-    idft<H, 1, 0, H>(Z, z, tw);
+    idft<H>(Z, z, tw);
     sfor<H>([&](auto I) {
         constexpr int s = decltype(I)::value;
         x[2 * s] = z[s].x;
@@ -311,24 +366,23 @@ void fast_search_kernel(FastArgs a)
     const int q = a.q;
     constexpr int m2 = M / 2;
 
-    /* per-harmonic windows of this tile */
+    /* per-harmonic windows of this tile; harmonics that do not fit the chunk
+     * get a window too (zero outside the spectrum) so that the interpolation
+     * below is branch-free and their result is masked to zero */
     if (tid < NH) {
         const int h = tid + 1;
-        int len = 0, lo = 0, cc = 0;
-        if ((filled >> tid) & 1ull) {
-            const int2 hi = a.hinfo[tid];
-            const long long tf = i0 + T.y, tl = i0 + T.y + T.z - 1;
-            const long long af = tf * a.pnum, al = tl * a.pnum;
-            const int rf = (int)(af % q), rl = (int)(al % q);
-            const long long qf = af / q, ql = al / q;
-            const int hrf = h * rf, hrl = h * rl;
-            const int resf = hrf % q, resl = hrl % q;
-            const long long bf = hi.x + (long long)h * qf + hrf / q + (resf >= hi.y ? 1 : 0);
-            const long long bl = hi.x + (long long)h * ql + hrl / q + (resl >= hi.y ? 1 : 0);
-            lo = (int)(bf + 1 - m2);
-            len = (int)(bl - bf) + M;
-            cc = hi.x + 1 - m2 - lo;
-        }
+        const int2 hi = a.hinfo[tid];
+        const long long tf = i0 + T.y, tl = i0 + T.y + T.z - 1;
+        const long long af = tf * a.pnum, al = tl * a.pnum;
+        const int rf = (int)(af % q), rl = (int)(al % q);
+        const long long qf = af / q, ql = al / q;
+        const int hrf = h * rf, hrl = h * rl;
+        const int resf = hrf % q, resl = hrl % q;
+        const long long bf = hi.x + (long long)h * qf + hrf / q + (resf >= hi.y ? 1 : 0);
+        const long long bl = hi.x + (long long)h * ql + hrl / q + (resl >= hi.y ? 1 : 0);
+        const int lo = (int)(bf + 1 - m2);
+        const int len = (int)(bl - bf) + M;
+        const int cc = hi.x + 1 - m2 - lo;
         s_lo[tid] = lo;
         s_C[tid] = cc;
         s_off[tid + 1] = len;
@@ -376,31 +430,37 @@ void fast_search_kernel(FastArgs a)
     __syncthreads();
 
     {
+        /* Branch-free over harmonics so loads of later harmonics can be
+         * scheduled under the FMA chains of earlier ones.  Weight rows are
+         * CS_WSP(M) floats (an odd number of 16-byte units), addressed with a
+         * 32-bit offset from the table base. */
+        constexpr int WS4 = CS_WSP(M) / 4;
+        const char *__restrict__ wbase = reinterpret_cast<const char *>(a.wtab);
         int res = 0, qi = 0;
         sfor<NH>([&](auto I) {
             constexpr int h = decltype(I)::value + 1;
             res += rho;
             qi += qb;
             if (res >= q) { res -= q; qi += 1; }
-            if ((filled >> (h - 1)) & 1ull) {
-                const int carry = res >= a.hinfo[h - 1].y ? 1 : 0;
-                const int wi = s_C[h - 1] + qi + carry;
-                const float *wr = a.wtab + ((size_t)(h - 1) * q + res) * a.ws;
-                float ar = 0.0f, ai = 0.0f;
-                sfor<M / 4>([&](auto J) {
-                    constexpr int j = 4 * decltype(J)::value;
-                    const float4 w4 = *reinterpret_cast<const float4 *>(wr + j);
-                    const float2 a0 = win[wi + j], a1 = win[wi + j + 1];
-                    const float2 a2 = win[wi + j + 2], a3 = win[wi + j + 3];
-                    ar = __builtin_fmaf(w4.x, a0.x, ar); ai = __builtin_fmaf(w4.x, a0.y, ai);
-                    ar = __builtin_fmaf(w4.y, a1.x, ar); ai = __builtin_fmaf(w4.y, a1.y, ai);
-                    ar = __builtin_fmaf(w4.z, a2.x, ar); ai = __builtin_fmaf(w4.z, a2.y, ai);
-                    ar = __builtin_fmaf(w4.w, a3.x, ar); ai = __builtin_fmaf(w4.w, a3.y, ai);
-                });
-                const float2 A = *reinterpret_cast<const float2 *>(wr + M);
-                const float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
-                X[h] = {p1 - p2, p3 + p4};
-            }
+            const int carry = res >= a.hinfo[h - 1].y ? 1 : 0;
+            const int wi = s_C[h - 1] + qi + carry;
+            const unsigned wrow = (unsigned)((h - 1) * q + res) * (unsigned)(16 * WS4);
+            const float4 *__restrict__ wr = reinterpret_cast<const float4 *>(wbase + wrow);
+            float ar = 0.0f, ai = 0.0f;
+            sfor<M / 4>([&](auto J) {
+                constexpr int j = 4 * decltype(J)::value;
+                const float4 w4 = wr[j / 4];
+                const float2 a0 = win[wi + j], a1 = win[wi + j + 1];
+                const float2 a2 = win[wi + j + 2], a3 = win[wi + j + 3];
+                ar = __builtin_fmaf(w4.x, a0.x, ar); ai = __builtin_fmaf(w4.x, a0.y, ai);
+                ar = __builtin_fmaf(w4.y, a1.x, ar); ai = __builtin_fmaf(w4.y, a1.y, ai);
+                ar = __builtin_fmaf(w4.z, a2.x, ar); ai = __builtin_fmaf(w4.z, a2.y, ai);
+                ar = __builtin_fmaf(w4.w, a3.x, ar); ai = __builtin_fmaf(w4.w, a3.y, ai);
+            });
+            const float4 A = wr[M / 4];
+            const float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
+            const bool fit = (filled >> (h - 1)) & 1ull;
+            X[h] = {fit ? p1 - p2 : 0.0f, fit ? p3 + p4 : 0.0f};
         });
     }
 
@@ -425,7 +485,7 @@ void fast_search_kernel(FastArgs a)
             if (slot >= 0) {
                 constexpr int Hk = NH / k;
                 float xk[2 * Hk];
-                c2r<Hk, k, NH + 1>(X, xk, a.tw);
+                c2r<Hk, k, NH + 1>(X, xk, (twc_t *)a.tw);
                 const float mv = boxcar<2 * Hk>(xk, a.wmask[slot], C->invsw[slot], C->valid[slot]);
                 emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
             }
@@ -434,7 +494,7 @@ void fast_search_kernel(FastArgs a)
 
     /* base profile */
     float x[2 * NH];
-    c2r<NH, 1, NH + 1>(X, x, a.tw);
+    c2r<NH, 1, NH + 1>(X, x, (twc_t *)a.tw);
 
     /* divisor decimations fold the base profile (src/search.c:298-331) */
     sfor<KMAX - 1>([&](auto I) {

@@ -545,30 +545,41 @@ int orc_harmonic_fits(const OrcCtx *c, int h, long long t)
 /* Project c2r.                                                       */
 /*                                                                    */
 /* Definition (both the oracle and the HIP kernels follow it op for   */
-/* op, in IEEE single precision with no contraction, so their profiles */
-/* are bit-identical; the reference uses FFTW/cuFFT instead):         */
-/*  tw(j,N) = ((float)cos(a), (float)sin(a)), a = 2*pi*(double)j/N    */
-/*  cmul(y,w) = (y.re*w.re - y.im*w.im, y.re*w.im + y.im*w.re)        */
+/* op in IEEE arithmetic, with fused multiply-adds exactly where it    */
+/* writes fma and no other contraction, so their profiles are         */
+/* bit-identical; the reference uses FFTW/cuFFT instead):             */
+/*  tw(j,N) = ((T)cos(a), (T)sin(a)), a = 2*pi*(double)j/N            */
+/*  cmul(y,w) = (fma(y.re, w.re, -(y.im*w.im)),                       */
+/*               fma(y.re, w.im,   y.im*w.re))                        */
 /*  c2r(X[0..H]) -> x[0..2H-1]:                                        */
 /*    Z_0 = (X0.re + XH.re, X0.re - XH.re)                             */
-/*    k=1..H-1: a=X_k, b=conj(X_{H-k}); e=a+b; d=a-b; o=cmul(d,tw(k,2H))*/
+/*    k=1..H-1: a=X_k, b=X_{H-k}; e=(a.re+b.re, a.im-b.im);            */
+/*              d=(a.re-b.re, a.im+b.im); o=cmul(d,tw(k,2H))           */
 /*              Z_k = (e.re - o.im, e.im + o.re)                       */
 /*    z = IDFT_H(Z); x[2s] = z_s.re; x[2s+1] = z_s.im                  */
-/*  IDFT_N (e^{+i}), decimation in time: R = 4 if 4|N, else 2 if 2|N, */
-/*    else 3, 5, else the smallest prime factor; M = N/R;             */
-/*    Y_r = IDFT_M(x_r, x_{r+R}, ...); for k<M: u_0 = Y_0[k],         */
-/*    u_r = Y_r[k] if r*k==0 else cmul(Y_r[k], tw(r*k, N));           */
-/*    X[k + M*s] = butterfly_R(u)_s, with                             */
+/*  IDFT_N (e^{+i}) of x[0..N-1]:                                      */
+/*   N = 1: copy.                                                      */
+/*   p = smallest prime factor, N1 = the power of p dividing N.        */
+/*   N1 < N (prime-factor split, no twiddles), N2 = N/N1:              */
+/*     T[n2] = IDFT_N1(x[(N2*n1 + N1*n2) mod N], n1 = 0..N1-1);       */
+/*     for k1 < N1: o = IDFT_N2(T[n2][k1], n2 = 0..N2-1);             */
+/*       X[(k1*E1 + k2*E2) mod N] = o[k2], E1 = N2*(N2^-1 mod N1),     */
+/*       E2 = N1*(N1^-1 mod N2)  (the CRT map).                        */
+/*   N1 = N (a prime power), decimation in time: R = 4 if 4|N, else   */
+/*     2 if 2|N, else p; M = N/R; Y_r = IDFT_M(x_r, x_{r+R}, ...);    */
+/*     for k<M: u_0 = Y_0[k],                                          */
+/*     u_r = Y_r[k] if r*k==0 else cmul(Y_r[k], tw(r*k, N));           */
+/*     X[k + M*s] = butterfly_R(u)_s, with                             */
 /*    R=2: (u0+u1, u0-u1)                                             */
 /*    R=4: t0=u0+u2 t1=u0-u2 t2=u1+u3 t3=u1-u3;                       */
 /*         (t0+t2, (t1.re-t3.im, t1.im+t3.re), t0-t2,                 */
 /*          (t1.re+t3.im, t1.im-t3.re))                               */
 /*    R odd (p): sp_r = u_r+u_{p-r}, sm_r = u_r-u_{p-r}, r=1..(p-1)/2;*/
 /*         X_0 = ((u0+sp_1)+sp_2)+...;                                 */
-/*         s=1..(p-1)/2: A = u0 + sp_1*c + sp_2*c' ... (left to right),*/
-/*         B = sm_1*sn + sm_2*sn' ... (starting from the first term), */
-/*         c,sn = tw(r*s mod p, p); X_s = (A.re-B.im, A.im+B.re),     */
-/*         X_{p-s} = (A.re+B.im, A.im-B.re)                           */
+/*         s=1..(p-1)/2, c,sn = tw(r*s mod p, p):                     */
+/*         A = u0; A = fma(sp_r, c, A) for r = 1, 2, ... (per part);  */
+/*         B = sm_1*sn; B = fma(sm_r, sn, B) for r = 2, 3, ...;       */
+/*         X_s = (A.re-B.im, A.im+B.re), X_{p-s} = (A.re+B.im, A.im-B.re) */
 /* ------------------------------------------------------------------ */
 static inline void otw(int j, int N, double *c, double *s)
 {
@@ -577,41 +588,98 @@ static inline void otw(int j, int N, double *c, double *s)
     *s = sin(a);
 }
 
-static int oradix(int N)
+static int ospf(int N)
 {
-    if (N % 4 == 0) return 4;
     if (N % 2 == 0) return 2;
-    if (N % 3 == 0) return 3;
-    if (N % 5 == 0) return 5;
-    for (int p = 7; p * p <= N; p += 2)
+    for (int p = 3; p * p <= N; p += 2)
         if (N % p == 0) return p;
     return N;
 }
 
-#define DEFINE_DFT(T, SUF, CVT)                                                \
-static void dft_##SUF(int N, const T *ir, const T *ii, long is,               \
-                      T *or_, T *oi, long lo, int B)                          \
+static int opfa_n1(int N)
+{
+    const int p = ospf(N);
+    int n1 = 1;
+    while (N % (n1 * p) == 0) n1 *= p;
+    return n1;
+}
+
+static int oradix(int N)
+{
+    if (N % 4 == 0) return 4;
+    if (N % 2 == 0) return 2;
+    return ospf(N);
+}
+
+static long omodinv(long a, long m)
+{
+    a %= m;
+    for (long x = 1; x < m; x++)
+        if (a * x % m == 1) return x;
+    return 0;
+}
+
+/* dft_SUF: output k (B lanes at or_/oi + ooff[k]) = IDFT_N of the inputs
+ * n (B lanes at ir/ii + ioff[n]); the index lists carry the prime-factor
+ * and decimation-in-time reorderings, which involve no arithmetic. */
+#define DEFINE_DFT(T, SUF, CVT, FMA)                                           \
+static void dft_##SUF(int N, const T *ir, const T *ii, const long *ioff,      \
+                      T *or_, T *oi, const long *ooff, int B)                 \
 {                                                                             \
     if (N == 1) {                                                             \
-        for (int b = 0; b < B; b++) { or_[b] = ir[b]; oi[b] = ii[b]; }        \
+        for (int b = 0; b < B; b++) {                                         \
+            or_[ooff[0] + b] = ir[ioff[0] + b];                               \
+            oi[ooff[0] + b] = ii[ioff[0] + b];                                \
+        }                                                                     \
+        return;                                                               \
+    }                                                                         \
+    const int N1 = opfa_n1(N);                                                \
+    T *tr = malloc(sizeof(T) * (size_t)N * B), *ti = malloc(sizeof(T) * (size_t)N * B); \
+    long *si = malloc(sizeof(long) * (size_t)N), *so = malloc(sizeof(long) * (size_t)N); \
+    if (N1 < N) {                                                             \
+        const int N2 = N / N1;                                                \
+        for (int n2 = 0; n2 < N2; n2++) {                                     \
+            for (int n1 = 0; n1 < N1; n1++) {                                 \
+                si[n1] = ioff[((long)N2 * n1 + (long)N1 * n2) % N];           \
+                so[n1] = ((long)n2 * N1 + n1) * B;                            \
+            }                                                                 \
+            dft_##SUF(N1, ir, ii, si, tr, ti, so, B);                         \
+        }                                                                     \
+        const long E1 = N2 * omodinv(N2, N1), E2 = N1 * omodinv(N1, N2);      \
+        for (int k1 = 0; k1 < N1; k1++) {                                     \
+            for (int n2 = 0; n2 < N2; n2++) {                                 \
+                si[n2] = ((long)n2 * N1 + k1) * B;                            \
+                so[n2] = ooff[(k1 * E1 + n2 * E2) % N];                       \
+            }                                                                 \
+            dft_##SUF(N2, tr, ti, si, or_, oi, so, B);                        \
+        }                                                                     \
+        free(tr); free(ti); free(si); free(so);                               \
         return;                                                               \
     }                                                                         \
     const int R = oradix(N), M = N / R;                                       \
-    for (int r = 0; r < R; r++)                                               \
-        dft_##SUF(M, ir + (long)r * is, ii + (long)r * is, is * R,            \
-                  or_ + (long)r * M * lo, oi + (long)r * M * lo, lo, B);      \
+    /* Y_r[j] at tr/ti + (r*M + j)*B */                                       \
+    for (int r = 0; r < R; r++) {                                             \
+        for (int j = 0; j < M; j++) {                                         \
+            si[j] = ioff[r + (long)j * R];                                    \
+            so[j] = ((long)r * M + j) * B;                                    \
+        }                                                                     \
+        dft_##SUF(M, ir, ii, si, tr, ti, so, B);                              \
+    }                                                                         \
     T *ur = malloc(sizeof(T) * (size_t)R * B);                                \
     T *ui = malloc(sizeof(T) * (size_t)R * B);                                \
-    T *sp = malloc(sizeof(T) * (size_t)(R + 1) * B);                          \
-    T *sm = malloc(sizeof(T) * (size_t)(R + 1) * B);                          \
+    T *sp = malloc(sizeof(T) * (size_t)(R + 1) * 2 * B);                      \
+    T *sm = malloc(sizeof(T) * (size_t)(R + 1) * 2 * B);                      \
     T *twc = malloc(sizeof(T) * (size_t)R), *tws = malloc(sizeof(T) * (size_t)R); \
-    T *accr = malloc(sizeof(T) * B), *acci = malloc(sizeof(T) * B);           \
     T *Ar = malloc(sizeof(T) * B), *Ai = malloc(sizeof(T) * B);               \
     T *Br = malloc(sizeof(T) * B), *Bi = malloc(sizeof(T) * B);               \
+    for (int j = 0; j < R; j++) {                                             \
+        double c_, s_; otw(j, R, &c_, &s_);                                   \
+        twc[j] = CVT(c_); tws[j] = CVT(s_);                                   \
+    }                                                                         \
     for (int k = 0; k < M; k++) {                                             \
         for (int r = 0; r < R; r++) {                                         \
-            const T *yr = or_ + (long)(r * M + k) * lo;                       \
-            const T *yi = oi + (long)(r * M + k) * lo;                        \
+            const T *yr = tr + ((long)r * M + k) * B;                         \
+            const T *yi = ti + ((long)r * M + k) * B;                         \
             T *dr = ur + (size_t)r * B, *di = ui + (size_t)r * B;            \
             if (r * k == 0) {                                                 \
                 for (int b = 0; b < B; b++) { dr[b] = yr[b]; di[b] = yi[b]; } \
@@ -619,26 +687,24 @@ static void dft_##SUF(int N, const T *ir, const T *ii, long is,               \
                 double c_, s_; otw(r * k, N, &c_, &s_);                       \
                 const T wc = CVT(c_), ws = CVT(s_);                           \
                 for (int b = 0; b < B; b++) {                                 \
-                    T a = yr[b] * wc, bb = yi[b] * ws;                        \
-                    T cc = yr[b] * ws, dd = yi[b] * wc;                       \
-                    dr[b] = a - bb;                                           \
-                    di[b] = cc + dd;                                          \
+                    const T bb = yi[b] * ws, dd = yi[b] * wc;                 \
+                    dr[b] = FMA(yr[b], wc, -bb);                              \
+                    di[b] = FMA(yr[b], ws, dd);                               \
                 }                                                             \
             }                                                                 \
         }                                                                     \
+        T *o0r = or_ + ooff[k], *o0i = oi + ooff[k];                          \
         if (R == 2) {                                                         \
-            T *o0r = or_ + (long)k * lo, *o0i = oi + (long)k * lo;            \
-            T *o1r = or_ + (long)(k + M) * lo, *o1i = oi + (long)(k + M) * lo;\
+            T *o1r = or_ + ooff[k + M], *o1i = oi + ooff[k + M];              \
             for (int b = 0; b < B; b++) {                                     \
                 T a0 = ur[b], a1 = ui[b], b0 = ur[B + b], b1 = ui[B + b];     \
                 o0r[b] = a0 + b0; o0i[b] = a1 + b1;                           \
                 o1r[b] = a0 - b0; o1i[b] = a1 - b1;                           \
             }                                                                 \
         } else if (R == 4) {                                                  \
-            T *p0r = or_ + (long)k * lo, *p0i = oi + (long)k * lo;            \
-            T *p1r = or_ + (long)(k + M) * lo, *p1i = oi + (long)(k + M) * lo;\
-            T *p2r = or_ + (long)(k + 2 * M) * lo, *p2i = oi + (long)(k + 2 * M) * lo; \
-            T *p3r = or_ + (long)(k + 3 * M) * lo, *p3i = oi + (long)(k + 3 * M) * lo; \
+            T *p1r = or_ + ooff[k + M], *p1i = oi + ooff[k + M];              \
+            T *p2r = or_ + ooff[k + 2 * M], *p2i = oi + ooff[k + 2 * M];      \
+            T *p3r = or_ + ooff[k + 3 * M], *p3i = oi + ooff[k + 3 * M];      \
             for (int b = 0; b < B; b++) {                                     \
                 T u0r = ur[b], u0i = ui[b], u1r = ur[B + b], u1i = ui[B + b]; \
                 T u2r = ur[2 * B + b], u2i = ui[2 * B + b];                   \
@@ -647,18 +713,14 @@ static void dft_##SUF(int N, const T *ir, const T *ii, long is,               \
                 T t1r = u0r - u2r, t1i = u0i - u2i;                           \
                 T t2r = u1r + u3r, t2i = u1i + u3i;                           \
                 T t3r = u1r - u3r, t3i = u1i - u3i;                           \
-                p0r[b] = t0r + t2r; p0i[b] = t0i + t2i;                       \
+                o0r[b] = t0r + t2r; o0i[b] = t0i + t2i;                       \
                 p2r[b] = t0r - t2r; p2i[b] = t0i - t2i;                       \
                 p1r[b] = t1r - t3i; p1i[b] = t1i + t3r;                       \
                 p3r[b] = t1r + t3i; p3i[b] = t1i - t3r;                       \
             }                                                                 \
         } else {                                                              \
             const int hh = (R - 1) / 2;                                       \
-            for (int j = 0; j < R; j++) {                                     \
-                double c_, s_; otw(j, R, &c_, &s_);                           \
-                twc[j] = CVT(c_); tws[j] = CVT(s_);                           \
-            }                                                                 \
-            /* sp/sm planes: [r][b], r = 1..hh */                             \
+            /* sp/sm planes: [2r + part][b], r = 1..hh */                     \
             for (int r = 1; r <= hh; r++) {                                   \
                 const T *arr = ur + (size_t)r * B, *aii = ui + (size_t)r * B; \
                 const T *brr = ur + (size_t)(R - r) * B, *bii = ui + (size_t)(R - r) * B; \
@@ -669,53 +731,49 @@ static void dft_##SUF(int N, const T *ir, const T *ii, long is,               \
                     m0[b] = arr[b] - brr[b]; m1[b] = aii[b] - bii[b];         \
                 }                                                             \
             }                                                                 \
-            {                                                                 \
-                T *x0r = or_ + (long)k * lo, *x0i = oi + (long)k * lo;        \
-                for (int b = 0; b < B; b++) { accr[b] = ur[b]; acci[b] = ui[b]; } \
+            for (int s = 1; s <= hh; s++) {                                   \
+                for (int b = 0; b < B; b++) { Ar[b] = ur[b]; Ai[b] = ui[b]; } \
                 for (int r = 1; r <= hh; r++) {                               \
+                    const int j = (r * s) % R;                                \
+                    const T c = twc[j], sn = tws[j];                          \
                     const T *p0 = sp + (size_t)(2 * r) * B, *p1 = sp + (size_t)(2 * r + 1) * B; \
-                    for (int b = 0; b < B; b++) { accr[b] = accr[b] + p0[b]; acci[b] = acci[b] + p1[b]; } \
-                }                                                             \
-                for (int s = 1; s <= hh; s++) {                               \
-                    for (int b = 0; b < B; b++) { Ar[b] = ur[b]; Ai[b] = ui[b]; } \
-                    for (int r = 1; r <= hh; r++) {                           \
-                        const int j = (r * s) % R;                            \
-                        const T c = twc[j], sn = tws[j];                      \
-                        const T *p0 = sp + (size_t)(2 * r) * B, *p1 = sp + (size_t)(2 * r + 1) * B; \
-                        const T *m0 = sm + (size_t)(2 * r) * B, *m1 = sm + (size_t)(2 * r + 1) * B; \
-                        if (r == 1) {                                         \
-                            for (int b = 0; b < B; b++) {                     \
-                                T pr = p0[b] * c, pi = p1[b] * c;             \
-                                Ar[b] = Ar[b] + pr; Ai[b] = Ai[b] + pi;       \
-                                Br[b] = m0[b] * sn; Bi[b] = m1[b] * sn;       \
-                            }                                                 \
-                        } else {                                              \
-                            for (int b = 0; b < B; b++) {                     \
-                                T pr = p0[b] * c, pi = p1[b] * c;             \
-                                Ar[b] = Ar[b] + pr; Ai[b] = Ai[b] + pi;       \
-                                T qr = m0[b] * sn, qi = m1[b] * sn;           \
-                                Br[b] = Br[b] + qr; Bi[b] = Bi[b] + qi;       \
-                            }                                                 \
-                        }                                                     \
-                    }                                                         \
-                    T *osr = or_ + (long)(k + M * s) * lo, *osi = oi + (long)(k + M * s) * lo; \
-                    T *omr = or_ + (long)(k + M * (R - s)) * lo, *omi = oi + (long)(k + M * (R - s)) * lo; \
+                    const T *m0 = sm + (size_t)(2 * r) * B, *m1 = sm + (size_t)(2 * r + 1) * B; \
                     for (int b = 0; b < B; b++) {                             \
-                        osr[b] = Ar[b] - Bi[b]; osi[b] = Ai[b] + Br[b];       \
-                        omr[b] = Ar[b] + Bi[b]; omi[b] = Ai[b] - Br[b];       \
+                        Ar[b] = FMA(p0[b], c, Ar[b]);                         \
+                        Ai[b] = FMA(p1[b], c, Ai[b]);                         \
+                        if (r == 1) { Br[b] = m0[b] * sn; Bi[b] = m1[b] * sn; } \
+                        else { Br[b] = FMA(m0[b], sn, Br[b]);                 \
+                               Bi[b] = FMA(m1[b], sn, Bi[b]); }               \
                     }                                                         \
                 }                                                             \
-                for (int b = 0; b < B; b++) { x0r[b] = accr[b]; x0i[b] = acci[b]; } \
+                T *osr = or_ + ooff[k + M * s], *osi = oi + ooff[k + M * s];  \
+                T *omr = or_ + ooff[k + M * (R - s)], *omi = oi + ooff[k + M * (R - s)]; \
+                for (int b = 0; b < B; b++) {                                 \
+                    osr[b] = Ar[b] - Bi[b]; osi[b] = Ai[b] + Br[b];           \
+                    omr[b] = Ar[b] + Bi[b]; omi[b] = Ai[b] - Br[b];           \
+                }                                                             \
+            }                                                                 \
+            /* X_0 last: with an in-place output list it must not clobber */ \
+            /* u0 before the s loop reads it (ur is a copy, so it cannot) */ \
+            for (int b = 0; b < B; b++) {                                     \
+                T acr = ur[b], aci = ui[b];                                   \
+                for (int r = 1; r <= hh; r++) {                               \
+                    acr = acr + sp[(size_t)(2 * r) * B + b];                  \
+                    aci = aci + sp[(size_t)(2 * r + 1) * B + b];              \
+                }                                                             \
+                o0r[b] = acr; o0i[b] = aci;                                   \
             }                                                                 \
         }                                                                     \
     }                                                                         \
     free(ur); free(ui); free(sp); free(sm); free(twc); free(tws);             \
-    free(accr); free(acci); free(Ar); free(Ai); free(Br); free(Bi);           \
+    free(Ar); free(Ai); free(Br); free(Bi);                                   \
+    free(tr); free(ti); free(si); free(so);                                   \
 }                                                                             \
 /* Batched c2r: X bins h=0..H at (xr,xi)[h*xs + b]; out bins x[i*lo + b]. */ \
 static void c2r_##SUF(int H, const T *xr, const T *xi, long xs, T *out,      \
                       long lo, int B, T *wk)                                  \
 {                                                                             \
+    if (H < 1 || B < 1) return;                                               \
     T *zr = wk, *zi = wk + (size_t)H * B;                                     \
     T *yr = wk + (size_t)2 * H * B, *yi = wk + (size_t)3 * H * B;             \
     for (int b = 0; b < B; b++) {                                             \
@@ -729,15 +787,18 @@ static void c2r_##SUF(int H, const T *xr, const T *xi, long xs, T *out,      \
         const T *ar = xr + (long)k * xs, *ai = xi + (long)k * xs;             \
         const T *br = xr + (long)(H - k) * xs, *bi = xi + (long)(H - k) * xs; \
         for (int b = 0; b < B; b++) {                                         \
-            T er = ar[b] + br[b], ei = ai[b] + (-bi[b]);                      \
-            T dr = ar[b] - br[b], di = ai[b] - (-bi[b]);                      \
-            T p1 = dr * wc, p2 = di * ws, p3 = dr * ws, p4 = di * wc;         \
-            T or_ = p1 - p2, oi = p3 + p4;                                    \
+            T er = ar[b] + br[b], ei = ai[b] - bi[b];                         \
+            T dr = ar[b] - br[b], di = ai[b] + bi[b];                         \
+            T p2 = di * ws, p4 = di * wc;                                     \
+            T or_ = FMA(dr, wc, -p2), oi = FMA(dr, ws, p4);                   \
             zr[(size_t)k * B + b] = er - oi;                                  \
             zi[(size_t)k * B + b] = ei + or_;                                 \
         }                                                                     \
     }                                                                         \
-    dft_##SUF(H, zr, zi, B, yr, yi, B, B);                                    \
+    long *io = calloc((size_t)H, sizeof(long));                                \
+    for (int s = 0; s < H; s++) io[s] = (long)s * B;                          \
+    dft_##SUF(H, zr, zi, io, yr, yi, io, B);                                  \
+    free(io);                                                                 \
     for (int s = 0; s < H; s++)                                               \
         for (int b = 0; b < B; b++) {                                         \
             out[(long)(2 * s) * lo + b] = yr[(size_t)s * B + b];              \
@@ -747,23 +808,27 @@ static void c2r_##SUF(int H, const T *xr, const T *xi, long xs, T *out,      \
 
 #define CVT_F(x) ((float)(x))
 #define CVT_D(x) ((double)(x))
-DEFINE_DFT(float, f32, CVT_F)
-DEFINE_DFT(double, f64, CVT_D)
+DEFINE_DFT(float, f32, CVT_F, fmaf)
+DEFINE_DFT(double, f64, CVT_D, fma)
 
 void orc_c2r_f32(int H, const float *X, float *x)
 {
     float *xr = malloc(sizeof(float) * (H + 1)), *xi = malloc(sizeof(float) * (H + 1));
     float *wk = malloc(sizeof(float) * 4 * (H > 0 ? H : 1));
-    for (int h = 0; h <= H; h++) { xr[h] = X[2 * h]; xi[h] = X[2 * h + 1]; }
-    c2r_f32(H, xr, xi, 1, x, 1, 1, wk);
+    if (H >= 1) {
+        for (int h = 0; h <= H; h++) { xr[h] = X[2 * h]; xi[h] = X[2 * h + 1]; }
+        c2r_f32(H, xr, xi, 1, x, 1, 1, wk);
+    }
     free(xr); free(xi); free(wk);
 }
 void orc_c2r_f64(int H, const double *X, double *x)
 {
     double *xr = malloc(sizeof(double) * (H + 1)), *xi = malloc(sizeof(double) * (H + 1));
     double *wk = malloc(sizeof(double) * 4 * (H > 0 ? H : 1));
-    for (int h = 0; h <= H; h++) { xr[h] = X[2 * h]; xi[h] = X[2 * h + 1]; }
-    c2r_f64(H, xr, xi, 1, x, 1, 1, wk);
+    if (H >= 1) {
+        for (int h = 0; h <= H; h++) { xr[h] = X[2 * h]; xi[h] = X[2 * h + 1]; }
+        c2r_f64(H, xr, xi, 1, x, 1, 1, wk);
+    }
     free(xr); free(xi); free(wk);
 }
 

@@ -17,7 +17,10 @@ from coherentsearch_c_amd import synth
 
 def test_c2r_matches_numpy_irfft(oracle):
     rng = np.random.default_rng(3)
-    for H in [1, 2, 3, 4, 5, 6, 7, 8, 10, 11, 12, 13, 15, 16, 20, 24, 30, 32, 37, 49, 60, 64]:
+    # prime powers (Cooley-Tukey with twiddles), coprime products (prime-factor
+    # split), and mixtures of both
+    for H in [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 20, 24, 25, 27, 30, 32,
+              36, 37, 45, 48, 49, 56, 60, 63, 64]:
         X = rng.standard_normal(H + 1) + 1j * rng.standard_normal(H + 1)
         ref = np.fft.irfft(X, 2 * H) * (2 * H)
         x64 = oracle.c2r_f64(X)
