@@ -427,6 +427,7 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
     S->args.q = S->q;
     S->args.pnum = S->pnum;
     S->args.ws = ws;
+    S->args.grp = S->q < CS_TILE ? CS_TILE / S->q : 1;
     S->args.chunks = S->d_chunks;
     S->args.recs = S->d_recs;
     S->args.counter = S->d_counter;

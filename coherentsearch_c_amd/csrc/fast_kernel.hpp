@@ -300,6 +300,7 @@ struct FastArgs {
     int q;
     int pnum;
     int ws;                   /* weight row stride (floats) */
+    int grp;                  /* trials of the tile per residue class, max(1, CS_TILE/q) */
     const ChunkDesc *chunks;
     CandRec *recs;
     unsigned int *counter;
@@ -360,8 +361,15 @@ void fast_search_kernel(FastArgs a)
     const long long i0 = C->i0;
     const unsigned long long filled = C->filled;
     const int tid = threadIdx.x;
-    const bool active = tid < T.z;
-    const int lj = T.y + (active ? tid : T.z - 1);
+    /* Trials t and t+q have the same residue t*h*p mod q at every harmonic,
+     * hence the same weight rows.  The first grp*q lanes take the tile's
+     * trials in residue-class order (lane L -> (L % grp)*q + L / grp), so
+     * neighbouring lanes gather the same rows; the LDS windows cover the
+     * whole tile either way. */
+    const int G = a.grp;
+    const int slot_l = tid < G * a.q ? (tid % G) * a.q + tid / G : tid;
+    const bool active = slot_l < T.z;
+    const int lj = T.y + (active ? slot_l : T.z - 1);
     const long long t = i0 + lj;
     const int q = a.q;
     constexpr int m2 = M / 2;
