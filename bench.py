@@ -35,36 +35,53 @@ def log(*a):
 
 def cpu_baseline(spec, params_kw, budget_s=12.0):
     """Oracle (reference CPU path restated, f32, blocksize 2048, OpenMP
-    round-robin chunks) on the leading chunks of the same spectrum until the
-    time budget is spent; harmonic-trials/s."""
+    round-robin chunks, every host thread) on the same spectrum: passes over
+    the chunks from a quarter of the band to its end (all harmonics filled,
+    like most of the search) until the time budget is spent; harmonic-trials/s
+    over the timed searches only."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     threads = len(os.sched_getaffinity(0))
     cfg = po.config(blocksize=2048, **params_kw)
     o = po.Oracle(spec.amps, spec.N, spec.T, cfg)
-    cs_ = o.chunk_starts()
+    nchunks = o.nchunks
+    c_start = nchunks // 4
     # harmonic-trials per chunk from the fit predicate at the chunk ends
-    done_ht, elapsed, c0 = 0, 0.0, 0
+    # (fits are fixed within a chunk by construction of the chunk starts)
+    ht = []
+    for c in range(c_start, nchunks):
+        i0, n, _ = o.chunk(c)
+        nf = sum(po.lib().orc_harmonic_fits(o.h, h, i0) and
+                 po.lib().orc_harmonic_fits(o.h, h, i0 + n - 1)
+                 for h in range(1, cfg.nharms + 1))
+        ht.append(nf * n)
     step = max(threads * 4, 8)
-    # start mid-band so the sample has every harmonic filled like most chunks
-    c0 = len(cs_) // 4
-    t_start = time.perf_counter()
-    while elapsed < budget_s and c0 < o.nchunks:
-        c1 = min(c0 + step, o.nchunks)
+    done_ht, elapsed, passes, c0 = 0, 0.0, 0, c_start
+    while elapsed < budget_s:
+        c1 = min(c0 + step, nchunks)
         t = time.perf_counter()
         o.search(8.0, threads, c0, c1)
         elapsed += time.perf_counter() - t
-        for c in range(c0, c1):
-            i0, n, _ = o.chunk(c)
-            nf = sum(po.lib().orc_harmonic_fits(o.h, h, i0) and
-                     po.lib().orc_harmonic_fits(o.h, h, i0 + n - 1)
-                     for h in range(1, cfg.nharms + 1))
-            done_ht += nf * n
+        done_ht += sum(ht[c0 - c_start:c1 - c_start])
         c0 = c1
+        if c0 >= nchunks:
+            c0, passes = c_start, passes + 1
     o.close()
     return dict(value=done_ht / elapsed, unit="harmonic-trials/s", cores=threads, kind="port",
-                sample="chunks %d..%d of the same search (blocksize 2048), %.1f s, %.3g "
-                       "harmonic-trials" % (len(cs_) // 4, c0, elapsed, done_ht))
+                sample="chunks %d..%d of the same search (blocksize 2048) searched %.2f times "
+                       "over, %.1f s of CPU search, %.3g harmonic-trials"
+                       % (c_start, nchunks, done_ht / max(sum(ht), 1), elapsed, done_ht))
+
+
+def calibrate_copy(nbytes=256 << 20):
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 launches of %d bytes read + %d bytes written" % (nbytes, nbytes))
 
 
 def main():
@@ -75,7 +92,13 @@ def main():
     ap.add_argument("--samples", type=int, default=1 << 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 256 MiB device-to-device copies (a plain copy "
+                         "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards by request or by "
               "trial range across devices (see the design notes), --gpus %d is not run"
