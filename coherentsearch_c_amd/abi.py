@@ -81,7 +81,8 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_run", "cs_gpu_session_free", "cs_gpu_search_region",
                "cs_gpu_session_dump_stack", "cs_gpu_session_last_kernel_ms",
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
-               "cs_gpu_session_work"]
+               "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
+               "cs_gpu_stage_run_host", "cs_gpu_session_stage"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "direct_plans_build", "direct_plans_free", "cs_chunk_state",
                 "cs_harmonic_fits", "harmonic_change_trials", "chunk_starts", "cs_sinpi",
@@ -179,6 +180,14 @@ def _declare(L):
     L.cs_gpu_session_last_kernel_ms.restype = D
     L.cs_gpu_session_work.argtypes = [P, Ps(LL), Ps(LL), Ps(D)]
     L.cs_gpu_session_work.restype = I
+    L.cs_gpu_stage_create.argtypes = [LL, D, I, I, D, I]
+    L.cs_gpu_stage_create.restype = P
+    L.cs_gpu_stage_free.argtypes = [P]
+    L.cs_gpu_stage_free.restype = None
+    L.cs_gpu_stage_run_host.argtypes = [P, P, P, P]
+    L.cs_gpu_stage_run_host.restype = I
+    L.cs_gpu_session_stage.argtypes = [P, P, P, P]
+    L.cs_gpu_session_stage.restype = I
     L.free_ = C.CDLL(None).free
     L.free_.argtypes = [P]
 

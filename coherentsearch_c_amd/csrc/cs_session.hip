@@ -24,6 +24,7 @@
 #include "core.h"
 #include "cs_device.h"
 #include "cs_gpu.h"
+#include "cs_stage.h"
 #include "fast_kernel.hpp"
 
 #define HIDDEN __attribute__((visibility("hidden")))
@@ -483,6 +484,28 @@ extern "C" CS_API int cs_gpu_session_upload(GpuSession *S, const FFTFile *ft)
         return -1;
     }
     S->T = ft->T;
+    return 0;
+}
+
+extern "C" CS_API int cs_gpu_session_stage(GpuSession *S, CsStage *g, const float *samples,
+                                           float *amps_host)
+{
+    if (!S || !g || !samples) return -1;
+    if (cs_stage_N(g) != S->N || S->N / 2 != S->namps) {
+        fprintf(stderr, "Error: stage plan for N=%lld does not match the session's N=%lld/%lld "
+                        "amps\n", cs_stage_N(g), S->N, S->namps);
+        return -1;
+    }
+    if (hipSetDevice(S->device) != hipSuccess) return -1;
+    if (cs_stage_enqueue(g, samples, S->d_amps, S->st) != 0) return -1;
+    if (amps_host &&
+        hipMemcpyAsync(amps_host, S->d_amps, (size_t)S->namps * sizeof(float2),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(S->st) != hipSuccess) {
+        fprintf(stderr, "Error: device staging failed: %s\n", hipGetErrorString(hipGetLastError()));
+        return -1;
+    }
     return 0;
 }
 

@@ -1,0 +1,634 @@
+/*
+ * Device staging of a raw series: PRESTO's realfft followed by rednoise
+ * (src/datpipe.c:52-253), so a .dat goes from host samples to a resident
+ * spectrum without the host FFT.
+ *
+ *   1. samples (N float) -> N/2 complex doubles;
+ *   2. forward complex FFT of N/2 points in double precision: a multi-level
+ *      four-step transform, each level a batch of <= 512-point sub-FFTs done
+ *      in LDS (Stockham stages of radix 8/7/5/4/3/2), the inter-level
+ *      twiddles exp(-2 pi i (i*k mod LS)/LS) from sincospi of an exact integer
+ *      phase, the last level writing in transposed (natural) order;
+ *   3. Numerical-Recipes unpack (datpipe.c:104-137): the FFT rounded to float,
+ *      the half-sum/half-difference in float, the twiddle products in double,
+ *      stored as float; DC and Nyquist packed into bin 0.  The twiddle of bin
+ *      il is cos/sin(-2 pi il/N) evaluated directly, where the reference walks
+ *      a trigonometric recurrence;
+ *   4. rednoise (datpipe.c:143-253): the block layout depends only on N, dt
+ *      and the widths, so the host tabulates it; one wave per block takes the
+ *      exact lower median of the block's powers (element (n-1)/2 of the sorted
+ *      block, by ranking) and mean = (float)(median / ln 2); one workgroup per
+ *      output segment applies norm = invsqrtf(mean_old + dslope*(float)ii) with
+ *      PRESTO's bit-trick inverse square root, in the reference's float
+ *      operation order (no contraction).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cs_gpu.h"
+#include "cs_stage.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kLmax = 512;        /* longest sub-FFT of one level */
+constexpr int kCL = 4096;         /* columns x length held in LDS (64 KB of double2) */
+constexpr int kMaxLevels = 4;
+constexpr int kMaxStages = 10;
+
+struct LevelArgs {
+    int L;                        /* this level's length */
+    int C;                        /* sub-FFTs per workgroup */
+    int nrad;
+    int rad[kMaxStages];
+    long long S;                  /* product of the later levels (1 on the last) */
+    long long B;                  /* product of the earlier levels */
+    int twoff;                    /* offset of exp(-2 pi i t/L), t < L, in the table */
+    /* last level only: digits of the transposed index r = k1 + L1*k2 + ... */
+    int nd;
+    int dL[kMaxLevels];
+    long long dstride[kMaxLevels];
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+/* forward R-point DFT of v in place; w(t) = exp(-2 pi i t/R) from the table */
+template <int R>
+__device__ __forceinline__ void dft_r(double2 (&v)[R], const double2 *__restrict__ tw, int tstep)
+{
+    if constexpr (R == 2) {
+        const double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 4) {
+        const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+        const double2 t2 = cadd(v[1], v[3]), t3 = csub(v[1], v[3]);
+        v[0] = cadd(t0, t2);
+        v[2] = csub(t0, t2);
+        v[1] = make_double2(t1.x + t3.y, t1.y - t3.x);   /* t1 - i t3 */
+        v[3] = make_double2(t1.x - t3.y, t1.y + t3.x);   /* t1 + i t3 */
+    } else {
+        double2 y[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            double2 acc = v[0];
+#pragma unroll
+            for (int j = 1; j < R; j++) acc = cadd(acc, cmul(v[j], tw[((j * k) % R) * tstep]));
+            y[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < R; k++) v[k] = y[k];
+    }
+}
+
+/* one Stockham stage of radix R over the C columns in LDS (layout a*C + c):
+ * butterfly j of column c reads a = j + r*L/R, twiddles by
+ * exp(-2 pi i r (j mod Ns)/(Ns R)) and writes a = (j/Ns)*Ns*R + j mod Ns + r*Ns */
+template <int R>
+__device__ void stage(double2 *lds, int L, int C, int Ns, const double2 *__restrict__ tw)
+{
+    constexpr int BPT = (kCL / R + kThreads - 1) / kThreads;
+    const int nb = (L / R) * C;
+    const int LR = L / R;
+    double2 v[BPT][R];
+    int dst[BPT];
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        const int u = threadIdx.x + q * kThreads;
+        dst[q] = -1;
+        if (u < nb) {
+            const int c = u % C, j = u / C;
+            const int k = j % Ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                double2 x = lds[(j + r * LR) * C + c];
+                if (r > 0 && k > 0) x = cmul(x, tw[(long long)r * k * (L / (Ns * R)) % L]);
+                v[q][r] = x;
+            }
+            dft_r<R>(v[q], tw, L / R);
+            dst[q] = ((j / Ns) * Ns * R + k) * C + c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < BPT; q++) {
+        if (dst[q] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[dst[q] + r * Ns * C] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ double2 phase(long long m, long long n)
+{
+    /* exp(-2 pi i m/n) for 0 <= m < n, reduced to |m/n| <= 1/2 */
+    if (2 * m > n) m -= n;
+    double s, c;
+    sincospi(-2.0 * (double)m / (double)n, &s, &c);
+    return make_double2(c, s);
+}
+
+__global__ __launch_bounds__(kThreads) void fft_level_kernel(const double2 *__restrict__ in,
+                                                             double2 *__restrict__ out,
+                                                             const double2 *__restrict__ twall,
+                                                             LevelArgs a, int last)
+{
+    __shared__ double2 lds[kCL];
+    const int L = a.L, C = a.C;
+    const double2 *tw = twall + a.twoff;
+    const int tid = threadIdx.x;
+    if (!last) {
+        /* block b, columns i0 .. i0+C-1 of the S inner indices */
+        const long long nic = (a.S + C - 1) / C;
+        const long long b = blockIdx.x / nic;
+        const long long i0 = (blockIdx.x % nic) * C;
+        const long long base = b * (long long)L * a.S;
+        for (int e = tid; e < L * C; e += kThreads) {
+            const int ai = e / C, c = e % C;
+            const long long i = i0 + c;
+            lds[ai * C + c] = i < a.S ? in[base + ai * a.S + i] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        int Ns = 1;
+        for (int s = 0; s < a.nrad; s++) {
+            const int R = a.rad[s];
+            if (R == 8) stage<8>(lds, L, C, Ns, tw);
+            else if (R == 4) stage<4>(lds, L, C, Ns, tw);
+            else if (R == 2) stage<2>(lds, L, C, Ns, tw);
+            else if (R == 3) stage<3>(lds, L, C, Ns, tw);
+            else if (R == 5) stage<5>(lds, L, C, Ns, tw);
+            else stage<7>(lds, L, C, Ns, tw);
+            Ns *= R;
+        }
+        const long long LS = (long long)L * a.S;
+        for (int e = tid; e < L * C; e += kThreads) {
+            const int k = e / C, c = e % C;
+            const long long i = i0 + c;
+            if (i < a.S) {
+                double2 x = lds[k * C + c];
+                const long long m = (i * k) % LS;
+                if (m) x = cmul(x, phase(m, LS));
+                out[base + k * a.S + i] = x;
+            }
+        }
+    } else {
+        /* transposed indices r0 .. r0+C-1; r's digits give the block b */
+        const long long r0 = (long long)blockIdx.x * C;
+        __shared__ long long sb[16];
+        if (tid < C) {
+            long long r = r0 + tid, b = 0;
+            for (int d = 0; d < a.nd; d++) {
+                b += (r % a.dL[d]) * a.dstride[d];
+                r /= a.dL[d];
+            }
+            sb[tid] = (r0 + tid < a.B) ? b : -1;
+        }
+        __syncthreads();
+        for (int e = tid; e < L * C; e += kThreads) {
+            const int c = e / L, ai = e % L;
+            const long long b = sb[c];
+            lds[ai * C + c] = b >= 0 ? in[b * L + ai] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        int Ns = 1;
+        for (int s = 0; s < a.nrad; s++) {
+            const int R = a.rad[s];
+            if (R == 8) stage<8>(lds, L, C, Ns, tw);
+            else if (R == 4) stage<4>(lds, L, C, Ns, tw);
+            else if (R == 2) stage<2>(lds, L, C, Ns, tw);
+            else if (R == 3) stage<3>(lds, L, C, Ns, tw);
+            else if (R == 5) stage<5>(lds, L, C, Ns, tw);
+            else stage<7>(lds, L, C, Ns, tw);
+            Ns *= R;
+        }
+        for (int e = tid; e < L * C; e += kThreads) {
+            const int k = e / C, c = e % C;
+            if (r0 + c < a.B) out[r0 + c + a.B * k] = lds[k * C + c];
+        }
+    }
+}
+
+__global__ void to_complex_kernel(const float *__restrict__ x, double2 *__restrict__ z, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = make_double2((double)x[2 * i], (double)x[2 * i + 1]);
+}
+
+/* NR unpack of the N/2-point transform W into the packed real spectrum d */
+__global__ void unpack_kernel(const double2 *__restrict__ W, float2 *__restrict__ d, long long N)
+{
+    const long long n2 = N / 2;
+    const long long il = 1 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (il == 1) {
+        const float d0r = (float)W[0].x, d0i = (float)W[0].y;
+        const double t = (double)d0r;
+        d[0] = make_float2((float)(t + (double)d0i), (float)(t - (double)d0i));
+    }
+    if (il > (N >> 2)) return;
+    const long long ih = n2 - il;
+    const float lr = (float)W[il].x, li = (float)W[il].y;
+    const float hr = (float)W[ih].x, hi = (float)W[ih].y;
+    const double cc = -0.5;
+    const double h1r = 0.5 * (double)(lr + hr);
+    const double h1i = 0.5 * (double)(li - hi);
+    const double h2r = -cc * (double)(li + hi);
+    const double h2i = cc * (double)(lr - hr);
+    double wi, wr;
+    sincospi(-2.0 * (double)il / (double)N, &wi, &wr);
+    const double h2rwr = h2r * wr, h2rwi = h2r * wi;
+    const double h2iwr = h2i * wr, h2iwi = h2i * wi;
+    if (il != ih)
+        d[il] = make_float2((float)(h1r + h2rwr - h2iwi), (float)(h1i + h2iwr + h2rwi));
+    d[ih] = make_float2((float)(h1r - h2rwr + h2iwi), (float)(-h1i + h2iwr + h2rwi));
+}
+
+/* lower median of each block's powers, mean = (float)(median / ln 2) */
+__global__ __launch_bounds__(64) void block_mean_kernel(const float2 *__restrict__ d,
+                                                        const long long *__restrict__ bstart,
+                                                        const int *__restrict__ blen,
+                                                        float *__restrict__ mean, int nblocks)
+{
+    __shared__ float p[CS_STAGE_MAXBLK];
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    const long long s = bstart[b];
+    const int n = blen[b];
+    for (int e = threadIdx.x; e < n; e += 64) {
+        const float2 v = d[s + e];
+        const float a = v.x * v.x, c = v.y * v.y;
+        p[e] = a + c;
+    }
+    __syncthreads();
+    const int t = (n - 1) / 2;
+    for (int e = threadIdx.x; e < n; e += 64) {
+        const float x = p[e];
+        int lt = 0, eqb = 0;
+        for (int f = 0; f < n; f++) {
+            const float y = p[f];
+            lt += y < x ? 1 : 0;
+            eqb += (y == x && f < e) ? 1 : 0;
+        }
+        if (lt + eqb == t) mean[b] = (float)((double)x / 0.69314718055994530942);
+    }
+}
+
+__device__ __forceinline__ float presto_invsqrtf(float x)
+{
+    const int i = 0x5f3759df - (__float_as_int(x) >> 1);
+    const float y = __int_as_float(i);
+    const float hx = 0.5f * x;
+    const float hxy = hx * y;
+    const float hxyy = hxy * y;
+    return y * (1.5f - hxyy);
+}
+
+/* segment s: bins [sstart, sstart+slen) normalised by
+ * invsqrtf(mean[mo] + dslope*(float)ii), ii = bin - sstart, where dslope comes
+ * from blocks (sa, sb) with denominator sden, or is 1.0f (sa < 0), or the
+ * segment is the first half block with the constant invsqrtf(mean[mo]) */
+__global__ __launch_bounds__(128) void rednoise_kernel(const float2 *__restrict__ d,
+                                                       float2 *__restrict__ out,
+                                                       const long long *__restrict__ sstart,
+                                                       const int *__restrict__ slen,
+                                                       const int4 *__restrict__ sinfo,
+                                                       const double *__restrict__ sden,
+                                                       const float *__restrict__ mean, int nseg)
+{
+    const int sg = blockIdx.x;
+    if (sg >= nseg) return;
+    if (sg == 0 && threadIdx.x == 0) out[0] = make_float2(1.0f, 0.0f);
+    const int4 inf = sinfo[sg];          /* (mo, sa, sb, first) */
+    const float mo = mean[inf.x];
+    float slope = 1.0f;
+    if (inf.y >= 0) {
+        const float diff = mean[inf.z] - mean[inf.y];
+        slope = (float)((double)diff / sden[sg]);
+    }
+    const long long s0 = sstart[sg];
+    const int n = slen[sg];
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+        float norm;
+        if (inf.w) {
+            norm = presto_invsqrtf(mo);
+        } else {
+            const float sl = slope * (float)e;
+            norm = presto_invsqrtf(mo + sl);
+        }
+        const float2 v = d[s0 + e];
+        out[s0 + e] = make_float2(v.x * norm, v.y * norm);
+    }
+}
+
+bool factor7(long long n, std::vector<int> &pf)
+{
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) { pf.push_back(p); n /= p; }
+    return n == 1;
+}
+
+/* split the length into levels of at most kLmax, as balanced as possible */
+bool plan_levels(long long n, std::vector<int> &Ls)
+{
+    std::vector<int> pf;
+    if (!factor7(n, pf)) return false;
+    std::sort(pf.begin(), pf.end(), [](int a, int b) { return a > b; });
+    for (int m = 1; m <= kMaxLevels; m++) {
+        std::vector<long long> prod(m, 1);
+        bool ok = true;
+        for (int p : pf) {
+            int best = -1;
+            for (int i = 0; i < m; i++)
+                if (prod[i] * p <= kLmax && (best < 0 || prod[i] < prod[best])) best = i;
+            if (best < 0) { ok = false; break; }
+            prod[best] *= p;
+        }
+        if (ok) {
+            Ls.assign(prod.begin(), prod.end());
+            return true;
+        }
+    }
+    return false;
+}
+
+void radices(int L, std::vector<int> &r)
+{
+    while (L % 8 == 0) { r.push_back(8); L /= 8; }
+    while (L % 4 == 0) { r.push_back(4); L /= 4; }
+    while (L % 2 == 0) { r.push_back(2); L /= 2; }
+    for (int p : {3, 5, 7})
+        while (L % p == 0) { r.push_back(p); L /= p; }
+}
+
+}  // namespace
+
+struct CsStage {
+    long long N, n2;
+    double dt;
+    int device;
+    int nlev;
+    LevelArgs lev[kMaxLevels];
+    long long grid[kMaxLevels];
+    double2 *d_z0, *d_z1;         /* ping-pong, n2 each */
+    double2 *d_tw;
+    float *d_x;                   /* samples */
+    float2 *d_packed;             /* unpacked realfft output */
+    long long *d_bstart, *d_sstart;
+    int *d_blen, *d_slen;
+    int4 *d_sinfo;
+    double *d_sden;
+    float *d_mean;
+    int nblocks, nseg;
+    hipStream_t st;
+    int own_stream;
+};
+
+extern "C" __attribute__((visibility("hidden"))) long long cs_stage_N(const CsStage *g)
+{
+    return g ? g->N : -1;
+}
+
+extern "C" CS_API void cs_gpu_stage_free(CsStage *g)
+{
+    if (!g) return;
+    hipSetDevice(g->device);
+    hipFree(g->d_z0); hipFree(g->d_z1); hipFree(g->d_tw); hipFree(g->d_x);
+    hipFree(g->d_packed); hipFree(g->d_bstart); hipFree(g->d_sstart); hipFree(g->d_blen);
+    hipFree(g->d_slen); hipFree(g->d_sinfo); hipFree(g->d_sden); hipFree(g->d_mean);
+    if (g->own_stream && g->st) hipStreamDestroy(g->st);
+    delete g;
+}
+
+#define SCHK(x)                                                                    \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            fprintf(stderr, "Error: HIP %s failed at %s:%d: %s\n", #x, __FILE__,   \
+                    __LINE__, hipGetErrorString(e_));                              \
+            goto fail;                                                             \
+        }                                                                          \
+    } while (0)
+
+extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int startwidth,
+                                               int endwidth, double endfreq, int device)
+{
+    if (startwidth <= 0) startwidth = 6;
+    if (endwidth <= 0) endwidth = 100;
+    if (endfreq <= 0.0) endfreq = 6.0;
+    if (N <= 0 || (N & 1)) {
+        fprintf(stderr, "Error: device staging needs an even number of samples (%lld)\n", N);
+        return nullptr;
+    }
+    if (endwidth > CS_STAGE_MAXBLK) {
+        fprintf(stderr, "Error: rednoise endwidth %d exceeds the device limit %d\n", endwidth,
+                CS_STAGE_MAXBLK);
+        return nullptr;
+    }
+    std::vector<int> Ls;
+    const long long n2 = N / 2;
+    if (!plan_levels(n2, Ls)) {
+        fprintf(stderr, "Error: device realfft needs N/2 with prime factors <= 7 and at most "
+                        "%d^%d points (N = %lld)\n", kLmax, kMaxLevels, N);
+        return nullptr;
+    }
+    CsStage *g = new CsStage();
+    memset((void *)g, 0, sizeof(*g));
+    g->N = N;
+    g->n2 = n2;
+    g->dt = dt;
+    g->device = device;
+    g->nlev = (int)Ls.size();
+    {
+        std::vector<double2> tw;
+        long long B = 1;
+        for (int l = 0; l < g->nlev; l++) {
+            LevelArgs &a = g->lev[l];
+            a.L = Ls[l];
+            a.C = std::max(1, std::min(16, kCL / a.L));
+            std::vector<int> r;
+            radices(a.L, r);
+            a.nrad = (int)r.size();
+            for (int i = 0; i < a.nrad; i++) a.rad[i] = r[i];
+            long long S = 1;
+            for (int q = l + 1; q < g->nlev; q++) S *= Ls[q];
+            a.S = S;
+            a.B = B;
+            a.twoff = (int)tw.size();
+            for (int t = 0; t < a.L; t++) {
+                const double ang = -2.0 * M_PI * (double)t / (double)a.L;
+                tw.push_back(make_double2(cos(ang), sin(ang)));
+            }
+            if (l == g->nlev - 1) {
+                /* r = k1 + L1 k2 + ... + (L1..L_{m-2}) k_{m-1};
+                 * b = k1 (L2..L_{m-1}) + k2 (L3..L_{m-1}) + ... + k_{m-1} */
+                a.nd = g->nlev - 1;
+                for (int d = 0; d < a.nd; d++) {
+                    a.dL[d] = Ls[d];
+                    long long st = 1;
+                    for (int q = d + 1; q < g->nlev - 1; q++) st *= Ls[q];
+                    a.dstride[d] = st;
+                }
+                g->grid[l] = (B + a.C - 1) / a.C;
+            } else {
+                g->grid[l] = B * ((S + a.C - 1) / a.C);
+            }
+            B *= a.L;
+        }
+        SCHK(hipSetDevice(device));
+        SCHK(hipMalloc(&g->d_tw, tw.size() * sizeof(double2)));
+        SCHK(hipMemcpy(g->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+    }
+    {
+        /* rednoise block layout (datpipe.c:143-226): depends on N, dt, widths only */
+        std::vector<long long> bstart;
+        std::vector<int> blen;
+        const long long nc = n2;
+        const float T = (float)((double)N * dt);
+        int bufflen = startwidth;
+        if (nc - 1 < bufflen) {
+            fprintf(stderr, "Error: rednoise: too few Fourier bins (%lld)\n", nc);
+            goto fail;
+        }
+        long long pos = 1;
+        bstart.push_back(pos);
+        blen.push_back(bufflen);
+        pos += bufflen;
+        long long binnum = 1 + bufflen;
+        bufflen = (int)(startwidth * log((double)binnum));
+        for (;;) {
+            const long long avail = nc - pos;
+            if (avail <= 0) break;
+            const int nb = avail < bufflen ? (int)avail : bufflen;
+            bstart.push_back(pos);
+            blen.push_back(nb);
+            pos += nb;
+            binnum += nb;
+            if ((float)binnum / T < endfreq) bufflen = (int)(startwidth * log((double)binnum));
+            else bufflen = endwidth;
+            if (bufflen > endwidth) {
+                fprintf(stderr, "Error: rednoise block length %d exceeds buffer %d\n", bufflen,
+                        endwidth);
+                goto fail;
+            }
+        }
+        const int nbk = (int)bstart.size();
+        std::vector<long long> sstart;
+        std::vector<int> slen;
+        std::vector<int4> sinfo;
+        std::vector<double> sden;
+        /* first half block: constant norm */
+        sstart.push_back(1);
+        slen.push_back(blen[0] / 2);
+        sinfo.push_back(make_int4(0, -1, -1, 1));
+        sden.push_back(1.0);
+        for (int k = 1; k < nbk; k++) {
+            const int mo = blen[k - 1] / 2, mn = blen[k] / 2;
+            sstart.push_back(bstart[k - 1] + mo);
+            slen.push_back(blen[k - 1] - mo + mn);
+            sinfo.push_back(make_int4(k - 1, k - 1, k, 0));
+            sden.push_back(0.5 * (double)(blen[k - 1] + blen[k]));
+        }
+        {
+            const int k = nbk - 1, mo = blen[k] / 2;
+            sstart.push_back(bstart[k] + mo);
+            slen.push_back(blen[k] - mo);
+            if (nbk >= 2) {
+                sinfo.push_back(make_int4(k, k - 1, k, 0));
+                sden.push_back(0.5 * (double)(blen[k - 1] + blen[k]));
+            } else {
+                sinfo.push_back(make_int4(k, -1, -1, 0));
+                sden.push_back(1.0);
+            }
+        }
+        g->nblocks = nbk;
+        g->nseg = (int)sstart.size();
+        SCHK(hipMalloc(&g->d_bstart, nbk * sizeof(long long)));
+        SCHK(hipMalloc(&g->d_blen, nbk * sizeof(int)));
+        SCHK(hipMalloc(&g->d_mean, nbk * sizeof(float)));
+        SCHK(hipMalloc(&g->d_sstart, g->nseg * sizeof(long long)));
+        SCHK(hipMalloc(&g->d_slen, g->nseg * sizeof(int)));
+        SCHK(hipMalloc(&g->d_sinfo, g->nseg * sizeof(int4)));
+        SCHK(hipMalloc(&g->d_sden, g->nseg * sizeof(double)));
+        SCHK(hipMemcpy(g->d_bstart, bstart.data(), nbk * sizeof(long long), hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(g->d_blen, blen.data(), nbk * sizeof(int), hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(g->d_sstart, sstart.data(), g->nseg * sizeof(long long), hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(g->d_slen, slen.data(), g->nseg * sizeof(int), hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(g->d_sinfo, sinfo.data(), g->nseg * sizeof(int4), hipMemcpyHostToDevice));
+        SCHK(hipMemcpy(g->d_sden, sden.data(), g->nseg * sizeof(double), hipMemcpyHostToDevice));
+    }
+    SCHK(hipMalloc(&g->d_z0, n2 * sizeof(double2)));
+    SCHK(hipMalloc(&g->d_z1, n2 * sizeof(double2)));
+    SCHK(hipMalloc(&g->d_x, N * sizeof(float)));
+    SCHK(hipMalloc(&g->d_packed, n2 * sizeof(float2)));
+    SCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    g->own_stream = 1;
+    return g;
+fail:
+    cs_gpu_stage_free(g);
+    return nullptr;
+}
+
+/* Enqueue staging of N host samples into d_out (float2[N/2]) on stream st
+ * (the stage's own stream when st is null).  0 on success. */
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g, const float *samples,
+                                                                       float2 *d_out, hipStream_t st)
+{
+    if (!st) st = g->st;
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    {
+        const long long n2 = g->n2;
+        to_complex_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, st>>>(g->d_x, g->d_z0, n2);
+        SCHK(hipGetLastError());
+        double2 *src = g->d_z0, *dst = g->d_z1;
+        for (int l = 0; l < g->nlev; l++) {
+            const int last = l == g->nlev - 1;
+            fft_level_kernel<<<(unsigned)g->grid[l], kThreads, 0, st>>>(src, dst, g->d_tw, g->lev[l], last);
+            SCHK(hipGetLastError());
+            std::swap(src, dst);
+        }
+        const long long nq = g->N >> 2;
+        const long long nthr = nq > 0 ? nq : 1;
+        unpack_kernel<<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(src, g->d_packed, g->N);
+        SCHK(hipGetLastError());
+        block_mean_kernel<<<(unsigned)g->nblocks, 64, 0, st>>>(g->d_packed, g->d_bstart, g->d_blen,
+                                                              g->d_mean, g->nblocks);
+        SCHK(hipGetLastError());
+        rednoise_kernel<<<(unsigned)g->nseg, 128, 0, st>>>(g->d_packed, d_out, g->d_sstart, g->d_slen,
+                                                           g->d_sinfo, g->d_sden, g->d_mean, g->nseg);
+        SCHK(hipGetLastError());
+    }
+    return 0;
+fail:
+    return -1;
+}
+
+extern "C" CS_API int cs_gpu_stage_run_host(CsStage *g, const float *samples, float *amps,
+                                            float *packed)
+{
+    if (!g || !samples || !amps) return -1;
+    float2 *d_out = nullptr;
+    SCHK(hipSetDevice(g->device));
+    SCHK(hipMalloc(&d_out, g->n2 * sizeof(float2)));
+    if (cs_stage_enqueue(g, samples, d_out, g->st) != 0) goto fail;
+    SCHK(hipMemcpyAsync(amps, d_out, g->n2 * sizeof(float2), hipMemcpyDeviceToHost, g->st));
+    if (packed)
+        SCHK(hipMemcpyAsync(packed, g->d_packed, g->n2 * sizeof(float2), hipMemcpyDeviceToHost,
+                            g->st));
+    SCHK(hipStreamSynchronize(g->st));
+    hipFree(d_out);
+    return 0;
+fail:
+    if (d_out) hipFree(d_out);
+    return -1;
+}

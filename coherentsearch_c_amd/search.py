@@ -61,6 +61,52 @@ class Spectrum:
         return self.amps.size // 2
 
 
+class Stage:
+    """Device realfft + rednoise plan for series of N samples at dt
+    (cs_gpu_stage_create; datpipe_create's widths)."""
+
+    def __init__(self, N, dt, device=0, startwidth=6, endwidth=100, endfreq=6.0):
+        self.L = abi.host()
+        self.N, self.dt = int(N), float(dt)
+        self.h = self.L.cs_gpu_stage_create(self.N, self.dt, startwidth, endwidth, endfreq,
+                                            device)
+        if not self.h:
+            raise RuntimeError("cs_gpu_stage_create failed (see stderr)")
+
+    def run_host(self, samples, packed=False):
+        """Stage on the device and copy the spectrum (and optionally the
+        realfft output before rednoise) back: float32 arrays of N values."""
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        if s.size != self.N:
+            raise ValueError("series of %d samples for a plan of %d" % (s.size, self.N))
+        amps = np.zeros(self.N, dtype=np.float32)
+        pk = np.zeros(self.N, dtype=np.float32) if packed else None
+        rc = self.L.cs_gpu_stage_run_host(self.h, s.ctypes.data, amps.ctypes.data,
+                                          pk.ctypes.data if packed else None)
+        if rc != 0:
+            raise RuntimeError("device staging failed")
+        return (amps, pk) if packed else amps
+
+    def close(self):
+        if self.h:
+            self.L.cs_gpu_stage_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def empty_spectrum(N, dt, path="memory"):
+    """A Spectrum of N samples whose amplitudes are filled later by device
+    staging (bin 0 is (1, 0) after rednoise)."""
+    sp = Spectrum(np.zeros(int(N), dtype=np.float32), N, dt, path)
+    sp.ft.DC, sp.ft.Nyquist = 1.0, 0.0
+    return sp
+
+
 def redfft(samples, dt, path="memory"):
     """realfft + rednoise of a float32 series in memory (datpipe_redfft_mem)."""
     L = abi.host()
@@ -147,6 +193,21 @@ class Session:
         spec = spec or self.spec
         if self.L.cs_gpu_session_upload(self.h, C.byref(spec.ft)) != 0:
             raise RuntimeError("spectrum upload failed")
+
+    def stage(self, stage, samples, host_copy=True):
+        """realfft + rednoise of the raw series on the device, straight into
+        the resident spectrum; with host_copy the session's Spectrum receives
+        the staged amplitudes for host post-processing."""
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        if s.size != self.spec.N:
+            raise ValueError("series of %d samples for a session of N=%d" % (s.size, self.spec.N))
+        rc = self.L.cs_gpu_session_stage(self.h, stage.h, s.ctypes.data,
+                                         self.spec.amps.ctypes.data if host_copy else None)
+        if rc != 0:
+            raise RuntimeError("device staging failed")
+        if host_copy:
+            self.spec.ft.DC = float(self.spec.amps[0])
+            self.spec.ft.Nyquist = float(self.spec.amps[1])
 
     def search_resident(self, threshold):
         return take_list(self.L.cs_gpu_session_search_resident(self.h, threshold))

@@ -101,6 +101,32 @@ CS_API double cs_gpu_session_last_kernel_ms(const GpuSession *s);
 CS_API int cs_gpu_session_work(const GpuSession *s, long long *harmonic_trials,
                                long long *trials, double *alg_bytes);
 
+/* ------------------------------------------------------------------ */
+/* Device staging of raw series (no reference counterpart on the GPU: the
+ * reference stages on the host, src/datpipe.c:52-253 -- realfft_forward and
+ * rednoise_apply).  A stage plan holds the FFT plan and the rednoise block
+ * layout of one series length N at sample time dt, with datpipe_create's
+ * startwidth/endwidth/endfreq (0 selects 6, 100, 6 Hz). */
+typedef struct CsStage CsStage;
+
+/* NULL (with a message) if N is odd, N/2 has a prime factor above 7, or the
+ * rednoise widths exceed the device limit. */
+CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int startwidth, int endwidth,
+                                    double endfreq, int device);
+CS_API void cs_gpu_stage_free(CsStage *g);
+
+/* samples (host, N floats) -> rednoised spectrum amps (host, N floats = N/2
+ * complex, bin 0 = (1, 0)); packed (optional, N floats) receives the realfft
+ * output before rednoise.  0 on success. */
+CS_API int cs_gpu_stage_run_host(CsStage *g, const float *samples, float *amps, float *packed);
+
+/* Stage samples (host, N floats) straight into the session's resident
+ * spectrum (the device-side equivalent of datpipe_redfft_mem followed by
+ * cs_gpu_session_upload); amps_host (optional, N floats) receives a copy for
+ * host post-processing.  -1 if the plan's N does not match the session. */
+CS_API int cs_gpu_session_stage(GpuSession *s, CsStage *g, const float *samples,
+                                float *amps_host);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2106,6 +2106,11 @@ static int rednoise_o(const float *in, float *out, long long nc, long long N, do
     return numwrote == nc ? 0 : -1;
 }
 
+int orc_rednoise(const float *in, long long N, double dt, float *out)
+{
+    return rednoise_o(in, out, N / 2, N, dt);
+}
+
 int orc_realfft_rednoise(const float *samples, long long N, double dt, float *out)
 {
     float *tmp = malloc(sizeof(float) * (size_t)N);

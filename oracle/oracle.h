@@ -165,6 +165,8 @@ int orc_realfft_rednoise(const float *samples, long long N, double dt,
                          float *out);
 /* Forward realfft only (PRESTO packing: out[0]=DC, out[1]=Nyquist). */
 int orc_realfft(const float *samples, long long N, float *out);
+/* rednoise alone (datpipe.c:143-253) of a packed realfft output in[N]. */
+int orc_rednoise(const float *in, long long N, double dt, float *out);
 
 #ifdef __cplusplus
 }

@@ -103,6 +103,7 @@ def lib():
         L.orc_format_candidates.restype = C.c_size_t
         L.orc_realfft_rednoise.argtypes = [P, LL, D, P]; L.orc_realfft_rednoise.restype = I
         L.orc_realfft.argtypes = [P, LL, P]; L.orc_realfft.restype = I
+        L.orc_rednoise.argtypes = [P, LL, D, P]; L.orc_rednoise.restype = I
         L.free = C.CDLL(None).free
         L.free.argtypes = [P]
         _lib = L
@@ -295,4 +296,13 @@ def realfft(samples):
     out = np.zeros(s.size, dtype=np.float32)
     if lib().orc_realfft(ptr(s), s.size, ptr(out)) != 0:
         raise ValueError("realfft failed")
+    return out
+
+
+def rednoise(packed, dt):
+    """rednoise alone of a packed realfft output (N floats)."""
+    s = np.ascontiguousarray(packed, dtype=np.float32)
+    out = np.zeros(s.size, dtype=np.float32)
+    if lib().orc_rednoise(ptr(s), s.size, dt, ptr(out)) != 0:
+        raise ValueError("rednoise failed")
     return out
