@@ -72,6 +72,27 @@ class SearchCache(C.Structure):
                 ("gpu_session", C.c_void_p)]
 
 
+class CsSearchRequest(C.Structure):
+    _fields_ = [("samples", C.c_void_p), ("N", C.c_longlong), ("dt", C.c_double),
+                ("p", C.POINTER(SearchParams)), ("lofreq", C.c_double), ("hifreq", C.c_double),
+                ("lobin", C.c_int), ("blocksize", C.c_int), ("threshold", C.c_double),
+                ("amps_out", C.c_void_p)]
+
+
+class CsSearchInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("pad", C.c_int), ("harmonic_trials", C.c_longlong),
+                ("trials", C.c_longlong), ("footprint", C.c_longlong),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
+                ("kernel_ms", C.c_double)]
+
+
+class CsEngineStats(C.Structure):
+    _fields_ = [("budget", C.c_longlong), ("in_use", C.c_longlong),
+                ("peak_in_use", C.c_longlong), ("nslots", C.c_int), ("running", C.c_int),
+                ("peak_running", C.c_int), ("pad", C.c_int), ("queued", C.c_longlong),
+                ("completed", C.c_longlong)]
+
+
 assert C.sizeof(SimpleInf) == 80 and C.sizeof(FFTFile) == 168
 assert C.sizeof(SearchParams) == 328 and C.sizeof(DirectPlan) == 144
 assert C.sizeof(Candidate) == 40 and C.sizeof(CandidateList) == 24
@@ -82,7 +103,9 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_dump_stack", "cs_gpu_session_last_kernel_ms",
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
                "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
-               "cs_gpu_stage_run_host", "cs_gpu_session_stage"]
+               "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_engine_create",
+               "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
+               "cs_engine_stats", "cs_engine_free", "cs_host_alloc", "cs_host_free"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "direct_plans_build", "direct_plans_free", "cs_chunk_state",
                 "cs_harmonic_fits", "harmonic_change_trials", "chunk_starts", "cs_sinpi",
@@ -188,6 +211,24 @@ def _declare(L):
     L.cs_gpu_stage_run_host.restype = I
     L.cs_gpu_session_stage.argtypes = [P, P, P, P]
     L.cs_gpu_session_stage.restype = I
+    L.cs_engine_create.argtypes = [I, LL, I]
+    L.cs_engine_create.restype = P
+    L.cs_engine_submit.argtypes = [P, Ps(CsSearchRequest)]
+    L.cs_engine_submit.restype = LL
+    L.cs_engine_poll.argtypes = [P, LL]
+    L.cs_engine_poll.restype = I
+    L.cs_engine_next_done.argtypes = [P, D]
+    L.cs_engine_next_done.restype = LL
+    L.cs_engine_wait.argtypes = [P, LL, Ps(CandidateList), Ps(CsSearchInfo)]
+    L.cs_engine_wait.restype = I
+    L.cs_engine_stats.argtypes = [P, Ps(CsEngineStats)]
+    L.cs_engine_stats.restype = None
+    L.cs_engine_free.argtypes = [P]
+    L.cs_engine_free.restype = None
+    L.cs_host_alloc.argtypes = [LL]
+    L.cs_host_alloc.restype = P
+    L.cs_host_free.argtypes = [P]
+    L.cs_host_free.restype = None
     L.free_ = C.CDLL(None).free
     L.free_.argtypes = [P]
 

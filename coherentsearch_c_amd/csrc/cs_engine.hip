@@ -1,0 +1,336 @@
+/*
+ * Search engine: continuous batching of whole searches on one GPU.
+ *
+ * A request is a raw series in host memory plus the search options of
+ * search_file (src/search.c:1247-1406 -- grid, plans, chunk table) and the
+ * datpipe staging (src/datpipe.c:52-253).  The engine keeps a FIFO of
+ * submitted requests and `nslots` worker threads, each owning one HIP
+ * stream.  A worker admits the oldest request as soon as its HBM footprint
+ * fits the budget next to the requests already running, then, on its own
+ * stream:
+ *   host   plans (per-harmonic residue tables) and chunk starts;
+ *   device stream-ordered allocation of the session and staging buffers,
+ *          realfft + rednoise of the uploaded series, the fused search
+ *          kernel, the record readback, stream-ordered frees.
+ * Requests that arrive while others run are admitted as slots and memory free
+ * up, and their transfers and kernels overlap on the device.  Each request's
+ * result is the raw candidate list of cs_gpu_session_run -- the same
+ * arithmetic whether it ran alone or beside others (no state is shared
+ * between requests except the constant twiddle table).
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "core.h"
+#include "cs_gpu.h"
+#include "cs_stage.h"
+
+static_assert(sizeof(CsSearchRequest) == 72, "CsSearchRequest layout");
+static_assert(sizeof(CsSearchInfo) == 64, "CsSearchInfo layout");
+static_assert(sizeof(CsEngineStats) == 56, "CsEngineStats layout");
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+double ms_between(clk::time_point a, clk::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct EngReq {
+    long long id = 0;
+    CsSearchRequest in{};
+    SearchParams p{};
+    long long footprint = 0;
+    int state = 0;                /* 0 queued, 1 running, 2 done */
+    CandidateList out{};
+    CsSearchInfo info{};
+    clk::time_point t_submit, t_start, t_end;
+};
+
+}  // namespace
+
+struct CsEngine {
+    int device = 0;
+    long long budget = 0, in_use = 0, peak_in_use = 0;
+    int nslots = 0, running = 0, peak_running = 0;
+    long long completed = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<EngReq *> queue;
+    std::unordered_map<long long, EngReq *> reqs;
+    std::deque<long long> done_order;
+    std::vector<std::thread> workers;
+    bool stop = false;
+    long long next_id = 0;
+};
+
+namespace {
+
+/* HBM bytes one request holds while it runs: series, FFT ping-pong, realfft
+ * output, spectrum, rednoise tables, plan arena, chunk/tile tables, records */
+long long footprint_of(const CsSearchRequest *r, const SearchParams *p)
+{
+    const long long N = r->N;
+    long long b = 4 * N + 16 * N + 4 * N + 4 * N + N / 2;
+    int pnum = 0, q = 0;
+    if (!csx_trial_grid_rational(p->hidr / (double)p->nharms, &pnum, &q)) q = 1;
+    b += (long long)p->nharms * q * 4 * (p->m + 12);
+    const double T = (double)N * r->dt;
+    const double lodr = p->hidr / (double)p->nharms;
+    double r_lo = r->lofreq * T;
+    if (r->lobin != 100) r_lo = (double)r->lobin;
+    const double trials = std::max(0.0, (r->hifreq * T - r_lo) / lodr + 1.0);
+    const int nprof = r->blocksize > 0 ? r->blocksize : CS_GPU_DEFAULT_BLOCKSIZE;
+    const double nchunks = trials / nprof + 2.0 * p->nharms + 2.0;
+    b += (long long)(trials / 256.0 * 16.0 + nchunks * (16.0 + 1024.0));
+    b += (1 << 16) * 16 + (1 << 20);
+    return b;
+}
+
+int run_request(CsEngine *E, EngReq *r, hipStream_t st)
+{
+    const SearchParams *p = &r->p;
+    const long long N = r->in.N;
+    const double dt = r->in.dt, T = (double)N * dt;
+    const double lodr = p->hidr / (double)p->nharms;
+    double r_lo = r->in.lofreq * T;
+    if (r->in.lobin != 100) r_lo = (double)r->in.lobin;          /* src/search.c:1259-1260 */
+    const double r_hi = r->in.hifreq * T;
+    long long total = (long long)floor((r_hi - r_lo) / lodr) + 1;
+    if (total < 0) total = 0;
+    const int Nprof = r->in.blocksize > 0 ? r->in.blocksize : CS_GPU_DEFAULT_BLOCKSIZE;
+
+    FFTFile ft;
+    memset(&ft, 0, sizeof(ft));
+    ft.N = N;
+    ft.T = T;
+    ft.df = 1.0 / T;
+    ft.namps = N / 2;
+    ft.DC = 1.0;
+    DirectPlanSet set;
+    memset(&set, 0, sizeof(set));
+    long long *cstarts = NULL, nchunks = 0;
+    GpuSession *S = NULL;
+    CsStage *g = NULL;
+    int rc = -1;
+    if (csx_plans_build(&set, p, r_lo) != 0) goto done;
+    if (csx_chunk_starts(&set, lodr, ft.namps, N, total, Nprof, &cstarts, &nchunks) != 0)
+        goto done;
+    S = cs_session_create_on(&ft, p, &set, cstarts, total, Nprof, nchunks, r_lo, lodr,
+                             E->device, st);
+    if (!S) goto done;
+    g = cs_stage_create_on(N, dt, 6, 100, 6.0, E->device, st);
+    if (!g) goto done;
+    if (cs_gpu_session_stage(S, g, r->in.samples, r->in.amps_out) != 0) goto done;
+    r->out = cs_gpu_session_search_resident(S, r->in.threshold);
+    r->info.kernel_ms = cs_gpu_session_last_kernel_ms(S);
+    {
+        long long ht = 0, tr = 0;
+        double ab = 0.0;
+        cs_gpu_session_work(S, &ht, &tr, &ab);
+        r->info.harmonic_trials = ht;
+        r->info.trials = tr;
+    }
+    rc = 0;
+done:
+    if (g) cs_gpu_stage_free(g);
+    if (S) cs_gpu_session_free(S);
+    csx_plans_free(&set);
+    free(cstarts);
+    return rc;
+}
+
+void worker(CsEngine *E)
+{
+    hipSetDevice(E->device);
+    hipStream_t st = NULL;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = NULL;
+    for (;;) {
+        EngReq *r = NULL;
+        {
+            std::unique_lock<std::mutex> lk(E->mu);
+            E->cv_work.wait(lk, [&] {
+                if (E->queue.empty()) return E->stop;
+                const EngReq *h = E->queue.front();
+                return E->in_use == 0 || E->in_use + h->footprint <= E->budget;
+            });
+            if (E->queue.empty()) break;
+            r = E->queue.front();
+            E->queue.pop_front();
+            E->in_use += r->footprint;
+            E->peak_in_use = std::max(E->peak_in_use, E->in_use);
+            E->running++;
+            E->peak_running = std::max(E->peak_running, E->running);
+            r->state = 1;
+            r->t_start = clk::now();
+        }
+        const int rc = st ? run_request(E, r, st) : -1;
+        if (st) hipStreamSynchronize(st);
+        {
+            std::lock_guard<std::mutex> lk(E->mu);
+            r->t_end = clk::now();
+            r->info.status = rc;
+            r->info.footprint = r->footprint;
+            r->info.queue_ms = ms_between(r->t_submit, r->t_start);
+            r->info.run_ms = ms_between(r->t_start, r->t_end);
+            r->info.latency_ms = ms_between(r->t_submit, r->t_end);
+            r->state = 2;
+            E->in_use -= r->footprint;
+            E->running--;
+            E->completed++;
+            E->done_order.push_back(r->id);
+        }
+        E->cv_work.notify_all();
+        E->cv_done.notify_all();
+    }
+    if (st) hipStreamDestroy(st);
+}
+
+}  // namespace
+
+extern "C" CS_API CsEngine *cs_engine_create(int device, long long hbm_budget, int nslots)
+{
+    if (hipSetDevice(device) != hipSuccess) {
+        fprintf(stderr, "Error: cannot select HIP device %d\n", device);
+        return NULL;
+    }
+    if (hbm_budget <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return NULL;
+        hbm_budget = (long long)(0.8 * (double)fr);
+    }
+    CsEngine *E = new CsEngine();
+    E->device = device;
+    E->budget = hbm_budget;
+    E->nslots = nslots > 0 ? nslots : 4;
+    for (int i = 0; i < E->nslots; i++) E->workers.emplace_back(worker, E);
+    return E;
+}
+
+extern "C" CS_API long long cs_engine_submit(CsEngine *E, const CsSearchRequest *r)
+{
+    if (!E || !r || !r->samples || !r->p || r->N <= 0 || (r->N & 1) || !(r->dt > 0.0))
+        return -1;
+    EngReq *q = new EngReq();
+    q->in = *r;
+    q->p = *r->p;
+    q->in.p = &q->p;
+    q->footprint = footprint_of(r, &q->p);
+    q->t_submit = clk::now();
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        if (E->stop) {
+            delete q;
+            return -1;
+        }
+        q->id = E->next_id++;
+        E->reqs[q->id] = q;
+        E->queue.push_back(q);
+    }
+    E->cv_work.notify_all();
+    return q->id;
+}
+
+extern "C" CS_API int cs_engine_poll(CsEngine *E, long long ticket)
+{
+    if (!E) return -1;
+    std::lock_guard<std::mutex> lk(E->mu);
+    auto it = E->reqs.find(ticket);
+    if (it == E->reqs.end()) return -1;
+    return it->second->state == 2 ? 1 : 0;
+}
+
+extern "C" CS_API long long cs_engine_next_done(CsEngine *E, double timeout_ms)
+{
+    if (!E) return -1;
+    std::unique_lock<std::mutex> lk(E->mu);
+    auto ready = [&] { return !E->done_order.empty(); };
+    if (timeout_ms < 0) E->cv_done.wait(lk, ready);
+    else E->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready);
+    if (E->done_order.empty()) return -1;
+    const long long id = E->done_order.front();
+    E->done_order.pop_front();
+    return id;
+}
+
+extern "C" CS_API int cs_engine_wait(CsEngine *E, long long ticket, CandidateList *out,
+                                     CsSearchInfo *info)
+{
+    if (!E) return -1;
+    EngReq *r = NULL;
+    {
+        std::unique_lock<std::mutex> lk(E->mu);
+        auto it = E->reqs.find(ticket);
+        if (it == E->reqs.end()) return -1;
+        r = it->second;
+        E->cv_done.wait(lk, [&] { return r->state == 2; });
+        E->reqs.erase(it);
+        for (auto d = E->done_order.begin(); d != E->done_order.end(); ++d)
+            if (*d == ticket) {
+                E->done_order.erase(d);
+                break;
+            }
+    }
+    if (out) *out = r->out;
+    else free(r->out.v);
+    if (info) *info = r->info;
+    const int st = r->info.status;
+    delete r;
+    return st;
+}
+
+extern "C" CS_API void cs_engine_stats(CsEngine *E, CsEngineStats *s)
+{
+    if (!E || !s) return;
+    std::lock_guard<std::mutex> lk(E->mu);
+    s->budget = E->budget;
+    s->in_use = E->in_use;
+    s->peak_in_use = E->peak_in_use;
+    s->nslots = E->nslots;
+    s->running = E->running;
+    s->peak_running = E->peak_running;
+    s->queued = (long long)E->queue.size();
+    s->completed = E->completed;
+}
+
+extern "C" CS_API void cs_engine_free(CsEngine *E)
+{
+    if (!E) return;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->stop = true;
+    }
+    E->cv_work.notify_all();
+    for (std::thread &t : E->workers) t.join();
+    for (auto &kv : E->reqs) {
+        free(kv.second->out.v);
+        delete kv.second;
+    }
+    delete E;
+}
+
+extern "C" CS_API void *cs_host_alloc(long long bytes)
+{
+    void *p = NULL;
+    if (bytes <= 0 || hipHostMalloc(&p, (size_t)bytes, hipHostMallocDefault) != hipSuccess)
+        return NULL;
+    return p;
+}
+
+extern "C" CS_API void cs_host_free(void *p)
+{
+    if (p) hipHostFree(p);
+}

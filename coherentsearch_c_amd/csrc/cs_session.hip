@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "core.h"
@@ -108,26 +109,74 @@ struct GpuSession {
     unsigned int cap;
     csk::FastArgs args;
     double last_ms;
+    bool async_alloc;             /* stream-ordered allocations on st (engine sessions) */
+    bool own_stream;
+    std::vector<float> h_wtab;    /* host copies kept alive for asynchronous uploads */
+    std::vector<int2> h_hinfo;
 };
+
+/* The project c2r's twiddles tw(j, N) = (cos, sin)(2 pi j / N), N < CS_TW_NMAX:
+ * one table per device, shared by every session and held until the process
+ * ends. */
+static float2 *shared_twiddles(int device)
+{
+    static std::mutex mu;
+    static float2 *tab[64] = {};
+    if (device < 0 || device >= 64) return NULL;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!tab[device]) {
+        std::vector<float2> tw(CS_TW_SIZE);
+        for (int Nn = 1; Nn < CS_TW_NMAX; Nn++)
+            for (int j = 0; j < Nn; j++) {
+                const double ang = 2.0 * M_PI * (double)j / (double)Nn;
+                tw[(size_t)cs_tw_off(Nn) + j] = make_float2((float)cos(ang), (float)sin(ang));
+            }
+        float2 *d = NULL;
+        if (hipMalloc(&d, tw.size() * sizeof(float2)) != hipSuccess) return NULL;
+        if (hipMemcpy(d, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+            hipFree(d);
+            return NULL;
+        }
+        tab[device] = d;
+    }
+    return tab[device];
+}
+
+static hipError_t dalloc(GpuSession *S, void **p, size_t bytes)
+{
+    return S->async_alloc ? hipMallocAsync(p, bytes, S->st) : hipMalloc(p, bytes);
+}
+
+static hipError_t dcopy(GpuSession *S, void *dst, const void *src, size_t bytes)
+{
+    return S->async_alloc ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S->st)
+                          : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+static void dfree(GpuSession *S, void *p)
+{
+    if (!p) return;
+    if (S->async_alloc) hipFreeAsync(p, S->st);
+    else hipFree(p);
+}
 
 static void session_release(GpuSession *S)
 {
     if (!S) return;
-    if (S->d_amps) hipFree(S->d_amps);
-    if (S->d_wtab) hipFree(S->d_wtab);
-    if (S->d_hinfo) hipFree(S->d_hinfo);
-    if (S->d_tw) hipFree(S->d_tw);
-    if (S->d_tiles) hipFree(S->d_tiles);
-    if (S->d_chunks) hipFree(S->d_chunks);
-    if (S->d_recs) hipFree(S->d_recs);
-    if (S->d_counter) hipFree(S->d_counter);
-    if (S->d_stack) hipFree(S->d_stack);
+    dfree(S, S->d_amps);
+    dfree(S, S->d_wtab);
+    dfree(S, S->d_hinfo);
+    dfree(S, S->d_tiles);
+    dfree(S, S->d_chunks);
+    dfree(S, S->d_recs);
+    dfree(S, S->d_counter);
+    dfree(S, S->d_stack);
     if (S->ev0) hipEventDestroy(S->ev0);
     if (S->ev1) hipEventDestroy(S->ev1);
-    if (S->st) hipStreamDestroy(S->st);
+    if (S->own_stream && S->st) hipStreamDestroy(S->st);
     delete S;
 }
-
 extern "C" CS_API int cs_gpu_available(int device, char *errbuf, size_t errlen)
 {
     int count = 0;
@@ -241,11 +290,12 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
     return 0;
 }
 
-extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const SearchParams *p,
-                                                    const DirectPlanSet *dplans,
-                                                    const long long *cstarts, long long total,
-                                                    int Nprof, long long nchunks,
-                                                    double r_lo, double lodr, int device)
+/* ext: run on the caller's stream with stream-ordered allocations (the
+ * engine's concurrent sessions); null: own stream, plain allocations. */
+__attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFTFile *ft, const SearchParams *p,
+                                 const DirectPlanSet *dplans, const long long *cstarts,
+                                 long long total, int Nprof, long long nchunks, double r_lo,
+                                 double lodr, int device, hipStream_t ext)
 {
     if (!ft || !p || !dplans || (!cstarts && nchunks > 0)) return NULL;
     GpuSession *S = new GpuSession();
@@ -267,13 +317,14 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
     S->d_amps = NULL; S->d_wtab = NULL; S->d_hinfo = NULL; S->d_tw = NULL;
     S->d_tiles = NULL; S->d_chunks = NULL; S->d_recs = NULL; S->d_counter = NULL;
     S->d_stack = NULL; S->st = NULL; S->ev0 = NULL; S->ev1 = NULL;
+    S->async_alloc = false;
+    S->own_stream = false;
     S->last_ms = 0.0;
     S->fast = NULL;
 
     const int nh = p->nharms, m = p->m;
     std::vector<float> wtab;
     std::vector<int2> hinfo;
-    std::vector<float2> tw;
     int ws;
     int rc;
 
@@ -368,14 +419,6 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
         }
         hinfo[(size_t)h - 1] = make_int2(pl->rfloor0, cthr);
     }
-    /* twiddles tw(j, N) = (cos, sin)(2 pi j / N), the project c2r's table */
-    tw.resize(CS_TW_SIZE);
-    for (int Nn = 1; Nn < CS_TW_NMAX; Nn++)
-        for (int j = 0; j < Nn; j++) {
-            const double ang = 2.0 * M_PI * (double)j / (double)Nn;
-            tw[(size_t)cs_tw_off(Nn) + j] = make_float2((float)cos(ang), (float)sin(ang));
-        }
-
     rc = build_chunks(S, dplans);
     if (rc == -2) {
         fprintf(stderr, "Error: boxcar width bank outside the GPU kernel's candidates "
@@ -396,27 +439,36 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
     }
 
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+    if (ext) {
+        S->st = ext;
+        S->async_alloc = true;
+    } else {
+        HIPCHK(hipStreamCreateWithFlags(&S->st, hipStreamNonBlocking));
+        S->own_stream = true;
+    }
     HIPCHK(hipEventCreate(&S->ev0));
     HIPCHK(hipEventCreate(&S->ev1));
-    HIPCHK(hipMalloc(&S->d_amps, (size_t)std::max<long long>(S->namps, 1) * sizeof(float2)));
-    HIPCHK(hipMalloc(&S->d_wtab, wtab.size() * sizeof(float)));
-    HIPCHK(hipMemcpy(S->d_wtab, wtab.data(), wtab.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&S->d_hinfo, hinfo.size() * sizeof(int2)));
-    HIPCHK(hipMemcpy(S->d_hinfo, hinfo.data(), hinfo.size() * sizeof(int2), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&S->d_tw, tw.size() * sizeof(float2)));
-    HIPCHK(hipMemcpy(S->d_tw, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&S->d_chunks, std::max<size_t>(S->chunks.size(), 1) * sizeof(ChunkDesc)));
+    S->h_wtab.swap(wtab);
+    S->h_hinfo.swap(hinfo);
+    S->d_tw = shared_twiddles(device);
+    if (!S->d_tw) {
+        fprintf(stderr, "Error: twiddle table upload failed\n");
+        goto fail;
+    }
+    HIPCHK(dalloc(S, (void **)&S->d_amps, (size_t)std::max<long long>(S->namps, 1) * sizeof(float2)));
+    HIPCHK(dalloc(S, (void **)&S->d_wtab, S->h_wtab.size() * sizeof(float)));
+    HIPCHK(dcopy(S, S->d_wtab, S->h_wtab.data(), S->h_wtab.size() * sizeof(float)));
+    HIPCHK(dalloc(S, (void **)&S->d_hinfo, S->h_hinfo.size() * sizeof(int2)));
+    HIPCHK(dcopy(S, S->d_hinfo, S->h_hinfo.data(), S->h_hinfo.size() * sizeof(int2)));
+    HIPCHK(dalloc(S, (void **)&S->d_chunks, std::max<size_t>(S->chunks.size(), 1) * sizeof(ChunkDesc)));
     if (!S->chunks.empty())
-        HIPCHK(hipMemcpy(S->d_chunks, S->chunks.data(), S->chunks.size() * sizeof(ChunkDesc),
-                         hipMemcpyHostToDevice));
-    HIPCHK(hipMalloc(&S->d_tiles, std::max<size_t>(S->tiles.size(), 4) * sizeof(int)));
+        HIPCHK(dcopy(S, S->d_chunks, S->chunks.data(), S->chunks.size() * sizeof(ChunkDesc)));
+    HIPCHK(dalloc(S, (void **)&S->d_tiles, std::max<size_t>(S->tiles.size(), 4) * sizeof(int)));
     if (!S->tiles.empty())
-        HIPCHK(hipMemcpy(S->d_tiles, S->tiles.data(), S->tiles.size() * sizeof(int),
-                         hipMemcpyHostToDevice));
-    S->cap = 1u << 20;
-    HIPCHK(hipMalloc(&S->d_recs, (size_t)S->cap * sizeof(CandRec)));
-    HIPCHK(hipMalloc(&S->d_counter, sizeof(unsigned int)));
+        HIPCHK(dcopy(S, S->d_tiles, S->tiles.data(), S->tiles.size() * sizeof(int)));
+    S->cap = 1u << 16;
+    HIPCHK(dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)));
+    HIPCHK(dalloc(S, (void **)&S->d_counter, sizeof(unsigned int)));
 
     S->args.amps = S->d_amps;
     S->args.namps = S->namps;
@@ -437,6 +489,16 @@ extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const Sea
 fail:
     session_release(S);
     return NULL;
+}
+
+extern "C" CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const SearchParams *p,
+                                                    const DirectPlanSet *dplans,
+                                                    const long long *cstarts, long long total,
+                                                    int Nprof, long long nchunks,
+                                                    double r_lo, double lodr, int device)
+{
+    return cs_session_create_on(ft, p, dplans, cstarts, total, Nprof, nchunks, r_lo, lodr,
+                                device, NULL);
 }
 
 extern "C" CS_API int cs_gpu_session_matches(const GpuSession *S, const FFTFile *ft,
@@ -542,10 +604,10 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
         S->last_ms = ms;
         if (count <= S->cap) break;
         /* never truncate: grow to the exact count and search again */
-        hipFree(S->d_recs);
+        dfree(S, S->d_recs);
         S->d_recs = NULL;
         S->cap = count;
-        if (hipMalloc(&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
+        if (dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
             fprintf(stderr, "Error: cannot hold %u candidate records\n", count);
             S->cap = 0;
             return out;
@@ -553,8 +615,9 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
     }
     if (count == 0) return out;
     std::vector<CandRec> recs(count);
-    if (hipMemcpy(recs.data(), S->d_recs, (size_t)count * sizeof(CandRec),
-                  hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpyAsync(recs.data(), S->d_recs, (size_t)count * sizeof(CandRec),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess)
         return out;
     std::sort(recs.begin(), recs.end(),
               [](const CandRec &a, const CandRec &b) { return a.key < b.key; });

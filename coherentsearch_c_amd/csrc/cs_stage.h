@@ -17,4 +17,16 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
 
 extern "C" __attribute__((visibility("hidden"))) long long cs_stage_N(const CsStage *g);
 
+/* a stage plan on the caller's stream with stream-ordered allocations */
+extern "C" __attribute__((visibility("hidden"))) CsStage *
+cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double endfreq,
+                   int device, hipStream_t ext);
+
+/* a session on the caller's stream with stream-ordered allocations (ext
+ * non-null), or with its own stream (ext null: cs_gpu_session_create) */
+__attribute__((visibility("hidden"))) GpuSession *
+cs_session_create_on(const FFTFile *ft, const SearchParams *p, const DirectPlanSet *dplans,
+                     const long long *cstarts, long long total, int Nprof, long long nchunks,
+                     double r_lo, double lodr, int device, hipStream_t ext);
+
 #endif

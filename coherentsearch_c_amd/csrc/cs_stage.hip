@@ -392,7 +392,32 @@ struct CsStage {
     int nblocks, nseg;
     hipStream_t st;
     int own_stream;
+    bool async_alloc;             /* stream-ordered allocations on st */
+    /* host tables, kept alive for asynchronous uploads */
+    std::vector<double2> h_tw;
+    std::vector<long long> h_bstart, h_sstart;
+    std::vector<int> h_blen, h_slen;
+    std::vector<int4> h_sinfo;
+    std::vector<double> h_sden;
 };
+
+static hipError_t salloc(CsStage *g, void **p, size_t bytes)
+{
+    return g->async_alloc ? hipMallocAsync(p, bytes, g->st) : hipMalloc(p, bytes);
+}
+
+static hipError_t scopy(CsStage *g, void *dst, const void *src, size_t bytes)
+{
+    return g->async_alloc ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->st)
+                          : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+static void sfree(CsStage *g, void *p)
+{
+    if (!p) return;
+    if (g->async_alloc) hipFreeAsync(p, g->st);
+    else hipFree(p);
+}
 
 extern "C" __attribute__((visibility("hidden"))) long long cs_stage_N(const CsStage *g)
 {
@@ -403,9 +428,9 @@ extern "C" CS_API void cs_gpu_stage_free(CsStage *g)
 {
     if (!g) return;
     hipSetDevice(g->device);
-    hipFree(g->d_z0); hipFree(g->d_z1); hipFree(g->d_tw); hipFree(g->d_x);
-    hipFree(g->d_packed); hipFree(g->d_bstart); hipFree(g->d_sstart); hipFree(g->d_blen);
-    hipFree(g->d_slen); hipFree(g->d_sinfo); hipFree(g->d_sden); hipFree(g->d_mean);
+    sfree(g, g->d_z0); sfree(g, g->d_z1); sfree(g, g->d_tw); sfree(g, g->d_x);
+    sfree(g, g->d_packed); sfree(g, g->d_bstart); sfree(g, g->d_sstart); sfree(g, g->d_blen);
+    sfree(g, g->d_slen); sfree(g, g->d_sinfo); sfree(g, g->d_sden); sfree(g, g->d_mean);
     if (g->own_stream && g->st) hipStreamDestroy(g->st);
     delete g;
 }
@@ -420,8 +445,9 @@ extern "C" CS_API void cs_gpu_stage_free(CsStage *g)
         }                                                                          \
     } while (0)
 
-extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int startwidth,
-                                               int endwidth, double endfreq, int device)
+extern "C" __attribute__((visibility("hidden"))) CsStage *
+cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double endfreq,
+                   int device, hipStream_t ext)
 {
     if (startwidth <= 0) startwidth = 6;
     if (endwidth <= 0) endwidth = 100;
@@ -443,14 +469,21 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
         return nullptr;
     }
     CsStage *g = new CsStage();
-    memset((void *)g, 0, sizeof(*g));
     g->N = N;
     g->n2 = n2;
     g->dt = dt;
     g->device = device;
     g->nlev = (int)Ls.size();
+    if (hipSetDevice(device) != hipSuccess) goto fail;
+    if (ext) {
+        g->st = ext;
+        g->async_alloc = true;
+    } else {
+        SCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+        g->own_stream = 1;
+    }
     {
-        std::vector<double2> tw;
+        std::vector<double2> &tw = g->h_tw;
         long long B = 1;
         for (int l = 0; l < g->nlev; l++) {
             LevelArgs &a = g->lev[l];
@@ -485,14 +518,13 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
             }
             B *= a.L;
         }
-        SCHK(hipSetDevice(device));
-        SCHK(hipMalloc(&g->d_tw, tw.size() * sizeof(double2)));
-        SCHK(hipMemcpy(g->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice));
+        SCHK(salloc(g, (void **)&g->d_tw, tw.size() * sizeof(double2)));
+        SCHK(scopy(g, g->d_tw, tw.data(), tw.size() * sizeof(double2)));
     }
     {
         /* rednoise block layout (datpipe.c:143-226): depends on N, dt, widths only */
-        std::vector<long long> bstart;
-        std::vector<int> blen;
+        std::vector<long long> &bstart = g->h_bstart;
+        std::vector<int> &blen = g->h_blen;
         const long long nc = n2;
         const float T = (float)((double)N * dt);
         int bufflen = startwidth;
@@ -523,10 +555,10 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
             }
         }
         const int nbk = (int)bstart.size();
-        std::vector<long long> sstart;
-        std::vector<int> slen;
-        std::vector<int4> sinfo;
-        std::vector<double> sden;
+        std::vector<long long> &sstart = g->h_sstart;
+        std::vector<int> &slen = g->h_slen;
+        std::vector<int4> &sinfo = g->h_sinfo;
+        std::vector<double> &sden = g->h_sden;
         /* first half block: constant norm */
         sstart.push_back(1);
         slen.push_back(blen[0] / 2);
@@ -553,30 +585,34 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
         }
         g->nblocks = nbk;
         g->nseg = (int)sstart.size();
-        SCHK(hipMalloc(&g->d_bstart, nbk * sizeof(long long)));
-        SCHK(hipMalloc(&g->d_blen, nbk * sizeof(int)));
-        SCHK(hipMalloc(&g->d_mean, nbk * sizeof(float)));
-        SCHK(hipMalloc(&g->d_sstart, g->nseg * sizeof(long long)));
-        SCHK(hipMalloc(&g->d_slen, g->nseg * sizeof(int)));
-        SCHK(hipMalloc(&g->d_sinfo, g->nseg * sizeof(int4)));
-        SCHK(hipMalloc(&g->d_sden, g->nseg * sizeof(double)));
-        SCHK(hipMemcpy(g->d_bstart, bstart.data(), nbk * sizeof(long long), hipMemcpyHostToDevice));
-        SCHK(hipMemcpy(g->d_blen, blen.data(), nbk * sizeof(int), hipMemcpyHostToDevice));
-        SCHK(hipMemcpy(g->d_sstart, sstart.data(), g->nseg * sizeof(long long), hipMemcpyHostToDevice));
-        SCHK(hipMemcpy(g->d_slen, slen.data(), g->nseg * sizeof(int), hipMemcpyHostToDevice));
-        SCHK(hipMemcpy(g->d_sinfo, sinfo.data(), g->nseg * sizeof(int4), hipMemcpyHostToDevice));
-        SCHK(hipMemcpy(g->d_sden, sden.data(), g->nseg * sizeof(double), hipMemcpyHostToDevice));
+        SCHK(salloc(g, (void **)&g->d_bstart, nbk * sizeof(long long)));
+        SCHK(salloc(g, (void **)&g->d_blen, nbk * sizeof(int)));
+        SCHK(salloc(g, (void **)&g->d_mean, nbk * sizeof(float)));
+        SCHK(salloc(g, (void **)&g->d_sstart, g->nseg * sizeof(long long)));
+        SCHK(salloc(g, (void **)&g->d_slen, g->nseg * sizeof(int)));
+        SCHK(salloc(g, (void **)&g->d_sinfo, g->nseg * sizeof(int4)));
+        SCHK(salloc(g, (void **)&g->d_sden, g->nseg * sizeof(double)));
+        SCHK(scopy(g, g->d_bstart, bstart.data(), nbk * sizeof(long long)));
+        SCHK(scopy(g, g->d_blen, blen.data(), nbk * sizeof(int)));
+        SCHK(scopy(g, g->d_sstart, sstart.data(), g->nseg * sizeof(long long)));
+        SCHK(scopy(g, g->d_slen, slen.data(), g->nseg * sizeof(int)));
+        SCHK(scopy(g, g->d_sinfo, sinfo.data(), g->nseg * sizeof(int4)));
+        SCHK(scopy(g, g->d_sden, sden.data(), g->nseg * sizeof(double)));
     }
-    SCHK(hipMalloc(&g->d_z0, n2 * sizeof(double2)));
-    SCHK(hipMalloc(&g->d_z1, n2 * sizeof(double2)));
-    SCHK(hipMalloc(&g->d_x, N * sizeof(float)));
-    SCHK(hipMalloc(&g->d_packed, n2 * sizeof(float2)));
-    SCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
-    g->own_stream = 1;
+    SCHK(salloc(g, (void **)&g->d_z0, n2 * sizeof(double2)));
+    SCHK(salloc(g, (void **)&g->d_z1, n2 * sizeof(double2)));
+    SCHK(salloc(g, (void **)&g->d_x, N * sizeof(float)));
+    SCHK(salloc(g, (void **)&g->d_packed, n2 * sizeof(float2)));
     return g;
 fail:
     cs_gpu_stage_free(g);
     return nullptr;
+}
+
+extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int startwidth,
+                                               int endwidth, double endfreq, int device)
+{
+    return cs_stage_create_on(N, dt, startwidth, endwidth, endfreq, device, nullptr);
 }
 
 /* Enqueue staging of N host samples into d_out (float2[N/2]) on stream st

@@ -253,6 +253,18 @@ def search_file(spec, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
     return take_list(cl)
 
 
+def collapse(cands, dr_tol=1.0, numharm=16, harm_tol=1.0):
+    """search.c:1467-1478: remove_duplicates (runs within dr_tol keep the
+    strongest) then remove_harmonics of a raw candidate list."""
+    L = abi.host()
+    cl = to_list(cands)
+    a = L.remove_duplicates(C.byref(cl), dr_tol)
+    L.candlist_free(C.byref(cl))
+    b = L.remove_harmonics(C.byref(a), numharm, harm_tol)
+    L.candlist_free(C.byref(a))
+    return take_list(b)
+
+
 def finalize(spec, params, cands, ncands=100):
     """cli.c:1059-1061: sort by S/N, truncate, measure duty cycles."""
     L = abi.host()

@@ -127,6 +127,66 @@ CS_API int cs_gpu_stage_run_host(CsStage *g, const float *samples, float *amps, 
 CS_API int cs_gpu_session_stage(GpuSession *s, CsStage *g, const float *samples,
                                 float *amps_host);
 
+/* ------------------------------------------------------------------ */
+/* Search engine: continuous batching of whole searches (no reference
+ * counterpart: the reference searches one file at a time behind prefetch
+ * threads, src/cli.c:590-617).  `nslots` worker streams; a request is
+ * admitted, oldest first, when its HBM footprint fits `hbm_budget` next to the
+ * running ones (a request larger than the budget runs alone).  Each request
+ * stages its raw series on the device (cs_gpu_stage_*) and searches it as
+ * search_file would (src/search.c:1247-1406), returning the raw candidate
+ * list of cs_gpu_session_run. */
+typedef struct CsEngine CsEngine;
+
+typedef struct {
+    const float *samples;   /* host, N floats; must stay valid until the request is done */
+    long long N;            /* even */
+    double dt;
+    const SearchParams *p;  /* copied at submit */
+    double lofreq, hifreq;  /* search band (Hz); lobin != 100 overrides lofreq, as search_file */
+    int lobin;
+    int blocksize;          /* 0: CS_GPU_DEFAULT_BLOCKSIZE */
+    double threshold;
+    float *amps_out;        /* optional host buffer of N floats: the staged spectrum */
+} CsSearchRequest;
+
+typedef struct {
+    int status;             /* 0 ok, -1 failed (message on stderr) */
+    int pad;
+    long long harmonic_trials, trials, footprint;
+    double queue_ms;        /* submit -> admitted */
+    double run_ms;          /* admitted -> done */
+    double latency_ms;      /* submit -> done */
+    double kernel_ms;       /* search kernel, HIP events */
+} CsSearchInfo;
+
+typedef struct {
+    long long budget, in_use, peak_in_use;
+    int nslots, running, peak_running, pad;
+    long long queued, completed;
+} CsEngineStats;
+
+/* hbm_budget <= 0: 80 % of the device's free memory; nslots <= 0: 4 */
+CS_API CsEngine *cs_engine_create(int device, long long hbm_budget, int nslots);
+/* ticket >= 0, or -1 for an invalid request */
+CS_API long long cs_engine_submit(CsEngine *e, const CsSearchRequest *r);
+/* 1 done, 0 pending, -1 unknown ticket */
+CS_API int cs_engine_poll(CsEngine *e, long long ticket);
+/* the ticket of the next request to finish (completion order), -1 on timeout
+ * (timeout_ms < 0 waits indefinitely) */
+CS_API long long cs_engine_next_done(CsEngine *e, double timeout_ms);
+/* waits for the request, hands over its list (caller frees with candlist_free
+ * or free(out->v)) and releases the ticket; returns the request's status */
+CS_API int cs_engine_wait(CsEngine *e, long long ticket, CandidateList *out,
+                          CsSearchInfo *info);
+CS_API void cs_engine_stats(CsEngine *e, CsEngineStats *s);
+/* finishes every submitted request, then releases the engine */
+CS_API void cs_engine_free(CsEngine *e);
+
+/* pinned host memory for series and spectra (hipHostMalloc) */
+CS_API void *cs_host_alloc(long long bytes);
+CS_API void cs_host_free(void *p);
+
 #ifdef __cplusplus
 }
 #endif

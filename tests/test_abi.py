@@ -48,6 +48,10 @@ def test_struct_layouts():
     assert abi.SearchParams.decimations.offset == 60 and abi.SearchParams.sigma.offset == 320
     assert abi.DirectPlan.row.offset == 48 and abi.DirectPlan.gA_im.offset == 136
     assert abi.Candidate.ducy.offset == 32
+    assert C.sizeof(abi.CsSearchRequest) == 72 and abi.CsSearchRequest.amps_out.offset == 64
+    assert abi.CsSearchRequest.lobin.offset == 48 and abi.CsSearchRequest.threshold.offset == 56
+    assert C.sizeof(abi.CsSearchInfo) == 64 and abi.CsSearchInfo.kernel_ms.offset == 56
+    assert C.sizeof(abi.CsEngineStats) == 56 and abi.CsEngineStats.completed.offset == 48
 
 
 def test_no_device_reports_error():
