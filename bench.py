@@ -73,6 +73,100 @@ def cpu_baseline(spec, params_kw, budget_s=12.0):
                        % (c_start, nchunks, done_ht / max(sum(ht), 1), elapsed, done_ht))
 
 
+def _found(final, period, tol=2e-3):
+    f0 = 1.0 / period
+    return bool(final.size and any(abs(fr / f0 - round(fr / f0)) < tol and round(fr / f0) >= 1
+                                   for fr in final["freq"][:3]))
+
+
+def _same_lists(a, b):
+    return a.size == b.size and all(np.array_equal(a[f], b[f])
+                                    for f in ("r", "nharm", "metric", "ducy"))
+
+
+def dm_sweep(nfiles, slots, host_files=3):
+    """The batched DM sweep: the batched/served form (engine, `slots`
+    streams, every file submitted at once, host post-processing overlapped)
+    against one-file-at-a-time forms on the same series: with device staging
+    (a one-slot engine, each file finished before the next starts) and the
+    reference-style flow (host realfft + rednoise, search_file, duty cycles;
+    timed on the first `host_files` files, whose harmonic-trials come from the
+    same searches)."""
+    from coherentsearch_c_amd import engine as eng
+    t = time.perf_counter()
+    jobs = eng.sweep_jobs(nfiles)
+    log("dm sweep: %d series, %.3g samples, generated in %.1f s"
+        % (nfiles, sum(j["samples"].n for j in jobs), time.perf_counter() - t))
+    with eng.Engine(nslots=1) as E:          # code objects and memory pool, untimed
+        eng.run_plain_device(E, jobs[:2])
+    with eng.Engine(nslots=slots) as E:
+        fb, ib, wb = eng.run_batched(E, jobs, post_workers=16)
+        st = E.stats()
+    ht = sum(i["harmonic_trials"] for i in ib)
+    log("  batched: %.3f s, %.4g ht/s" % (wb, ht / wb))
+    with eng.Engine(nslots=1) as E:
+        fp, _, wp = eng.run_plain_device(E, jobs)
+    log("  plain, device staging: %.3f s, %.4g ht/s" % (wp, ht / wp))
+    k = min(host_files, nfiles)
+    fh, wh = eng.run_plain(jobs[:k])
+    hth = sum(ib[i]["harmonic_trials"] for i in range(k))
+    log("  plain, host staging (%d files): %.3f s, %.4g ht/s" % (k, wh, hth / wh))
+    # host staging differs from device staging in the last float bits of a few
+    # bins (both FFTs are double precision): lists agree within the tolerance
+    host_ok = True
+    for i in range(k):
+        a, b = fb[i], fh[i]
+        if a.size != b.size or not np.array_equal(a["nharm"], b["nharm"]) or \
+                np.any(np.abs(a["metric"] - b["metric"]) > 1e-4 * np.maximum(np.abs(b["metric"]), 1)):
+            host_ok = False
+    psr = [i for i, j in enumerate(jobs) if j["snr"] >= 8.0]
+    return {
+        "files": nfiles, "samples": int(sum(j["samples"].n for j in jobs)),
+        "harmonic_trials": int(ht), "slots": slots,
+        "batched_s": wb, "batched_ht_s": ht / wb,
+        "plain_device_s": wp, "plain_device_ht_s": ht / wp,
+        "ratio_vs_plain_device": wp / wb,
+        "plain_host_files": k, "plain_host_s": wh, "plain_host_ht_s": hth / wh,
+        "ratio_vs_plain_host": (ht / wb) / (hth / wh),
+        "lists_identical_batched_vs_plain_device": all(_same_lists(a, b) for a, b in zip(fb, fp)),
+        "lists_match_host_staging_within_1e-4": host_ok,
+        "pulsar_files_snr_ge_8_detected": "%d/%d" % (sum(_found(fb[i], jobs[i]["period"])
+                                                       for i in psr), len(psr)),
+        "peak_concurrent_searches": st["peak_running"],
+        "peak_hbm_in_use_bytes": st["peak_in_use"],
+    }
+
+
+def serving(nreq, slots, rate=20.0):
+    """The served stream: Poisson arrivals at `rate`/s, continuous batching
+    on the engine; latency = arrival -> collapsed list with duty cycles."""
+    from coherentsearch_c_amd import engine as eng
+    t = time.perf_counter()
+    jobs, arrivals = eng.serving_jobs(nreq, rate)
+    log("serving: %d requests over %.1f s, generated in %.1f s"
+        % (nreq, arrivals[-1], time.perf_counter() - t))
+    with eng.Engine(nslots=slots) as E:
+        fin, recs, wall = eng.serve(E, jobs, arrivals, post_workers=16)
+        st = E.stats()
+    lat = np.array([r["latency_s"] for r in recs]) * 1e3
+    ht = sum(r["harmonic_trials"] for r in recs)
+    psr = [i for i, j in enumerate(jobs) if j["snr"] > 0]
+    log("  p50 %.1f ms, p99 %.1f ms, %.4g ht/s over %.1f s" % (np.percentile(lat, 50),
+                                                             np.percentile(lat, 99),
+                                                             ht / wall, wall))
+    return {
+        "requests": nreq, "rate_per_s": rate, "slots": slots,
+        "p50_latency_ms": float(np.percentile(lat, 50)),
+        "p99_latency_ms": float(np.percentile(lat, 99)),
+        "max_latency_ms": float(lat.max()),
+        "mean_queue_ms": float(np.mean([r["queue_ms"] for r in recs])),
+        "harmonic_trials": int(ht), "wall_s": wall, "throughput_ht_s": ht / wall,
+        "peak_concurrent_searches": st["peak_running"],
+        "pulsar_requests_detected": "%d/%d" % (sum(_found(fin[i], jobs[i]["period"])
+                                                 for i in psr), len(psr)),
+    }
+
+
 def calibrate_copy(nbytes=256 << 20):
     import torch
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -92,6 +186,11 @@ def main():
     ap.add_argument("--samples", type=int, default=1 << 24)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the 64-file DM sweep")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served request stream")
+    ap.add_argument("--sweep-files", type=int, default=64)
+    ap.add_argument("--serve-requests", type=int, default=500)
+    ap.add_argument("--slots", type=int, default=4, help="engine worker streams")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -176,6 +275,10 @@ def main():
         "upload_inclusive_ht_s": ht / (wall / args.steps + up_s),
         "upload_s": up_s,
     }
+    if not args.no_sweep:
+        out["dm_sweep"] = dm_sweep(args.sweep_files, args.slots)
+    if not args.no_serve:
+        out["serving"] = serving(args.serve_requests, args.slots)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(spec, pkw, args.cpu_budget)

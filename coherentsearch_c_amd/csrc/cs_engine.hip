@@ -212,6 +212,15 @@ extern "C" CS_API CsEngine *cs_engine_create(int device, long long hbm_budget, i
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return NULL;
         hbm_budget = (long long)(0.8 * (double)fr);
     }
+    {
+        /* keep freed request buffers in the device's pool for the next request
+         * instead of returning them to the driver at every synchronisation */
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t thr = (uint64_t)hbm_budget;
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+    }
     CsEngine *E = new CsEngine();
     E->device = device;
     E->budget = hbm_budget;

@@ -61,15 +61,22 @@ class Engine:
         self._lock = threading.Lock()
 
     def submit(self, samples, dt, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
-               threshold=8.0, spectrum=True):
+               threshold=8.0, spectrum=True, out=None):
         """Queue one search of a raw float32 series; returns its ticket.
         With `spectrum`, the staged spectrum comes back for host
-        post-processing (duty cycles)."""
+        post-processing (duty cycles), into `out` (N float32, e.g. pinned)
+        when given."""
         s = samples.a if isinstance(samples, PinnedSeries) else samples
         s = np.ascontiguousarray(s, dtype=np.float32)
         if s.size % 2:
             raise ValueError("the series needs an even number of samples")
-        amps = np.zeros(s.size, dtype=np.float32) if spectrum else None
+        amps = None
+        if spectrum:
+            amps = out.a if isinstance(out, PinnedSeries) else out
+            if amps is None:
+                amps = np.zeros(s.size, dtype=np.float32)
+            if amps.dtype != np.float32 or amps.size != s.size or not amps.flags.c_contiguous:
+                raise ValueError("spectrum buffer must be N contiguous float32")
         req = abi.CsSearchRequest()
         req.samples = s.ctypes.data
         req.N = s.size
@@ -121,6 +128,78 @@ class Engine:
         self.close()
 
 
+def _submit_job(engine, j):
+    return engine.submit(j["samples"], j["dt"], j["params"], lofreq=j.get("lofreq", 0.1),
+                         hifreq=j.get("hifreq", 125.0), lobin=j.get("lobin", 100),
+                         blocksize=j.get("blocksize", 0), threshold=j.get("threshold", 8.0),
+                         out=j.get("out"))
+
+
+def sweep_jobs(nfiles=64, seed=4, pinned=True, threshold=8.0, lo=22, hi=25):
+    """The batched DM sweep: `nfiles` series, lengths log-uniform in
+    [2^lo, 2^hi] rounded up to even 7-smooth sizes, per-file seeded N(0,1)
+    noise, a 3.7 ms pulsar in 8 files around the middle DM (S/N peaking at
+    18), nharms cycling 16/32/60, m 16, maxdecim 6, blocksize 65536.  With
+    `pinned`, series and spectrum buffers are pinned host memory."""
+    from . import synth
+    jobs = []
+    for m in synth.dm_sweep(nfiles, seed, lo=lo, hi=hi):
+        x = synth.series_with_pulsar(m["N"], m["period"], m["duty"], m["snr"], m["seed"])
+        s, o = x, None
+        if pinned:
+            s = PinnedSeries(m["N"])
+            s.a[:] = x
+            o = PinnedSeries(m["N"])
+        jobs.append(dict(samples=s, out=o, dt=synth.DT, threshold=threshold, blocksize=65536,
+                         params=make_params(nharms=m["nharms"], m=16, maxdecim=6),
+                         snr=m["snr"], period=m["period"], nharms=m["nharms"]))
+    return jobs
+
+
+def serving_jobs(n=500, rate=20.0, seed=5, lo=20, hi=25, pool=8, threshold=8.0):
+    """The served stream: Poisson arrivals at `rate` per second, lengths
+    log-uniform in [2^lo, 2^hi] (even 7-smooth), nharms uniform over
+    16/32/60, 10 % with a pulsar (period log-uniform 1-500 ms, S/N 8-20).
+    Noise-only requests read windows of `pool` pinned noise buffers; pulsar
+    requests get their own series.  Returns (jobs, arrival seconds)."""
+    from . import synth
+    meta = synth.serving_requests(n, rate, seed, lo, hi)
+    nmax = max(m["N"] for m in meta)
+    bufs = []
+    for b in range(pool):
+        ps = PinnedSeries(nmax)
+        ps.a[:] = np.random.default_rng(seed * 7919 + b).standard_normal(nmax, dtype=np.float32)
+        bufs.append(ps)
+    jobs = []
+    for i, m in enumerate(meta):
+        if m["snr"] > 0:
+            x = synth.series_with_pulsar(m["N"], m["period"], m["duty"], m["snr"], m["seed"])
+        else:
+            b = bufs[i % pool]
+            off = (i * 7919) % (nmax - m["N"] + 1)
+            x = b.a[off:off + m["N"]]
+        jobs.append(dict(samples=x, dt=synth.DT, threshold=threshold, blocksize=65536,
+                         params=make_params(nharms=m["nharms"], m=16, maxdecim=6),
+                         snr=m["snr"], period=m["period"], nharms=m["nharms"],
+                         _pool=bufs))
+    return jobs, np.array([m["arrival"] for m in meta])
+
+
+def run_plain_device(engine, jobs, ncands=100):
+    """One file at a time through a one-slot engine: device staging, search,
+    collapse and duty cycles of each file before the next is submitted.
+    Returns (final lists, infos, wall seconds)."""
+    finals, infos = [], []
+    t0 = time.perf_counter()
+    for j in jobs:
+        raw, info, spec = engine.wait(_submit_job(engine, j))
+        if info["status"] != 0:
+            raise RuntimeError("request failed")
+        finals.append(_post(spec, j["params"], raw, ncands))
+        infos.append(info)
+    return finals, infos, time.perf_counter() - t0
+
+
 def _post(spec, params, raw, ncands):
     return finalize(spec, params, collapse(raw), ncands)
 
@@ -149,10 +228,7 @@ def run_batched(engine, jobs, post_workers=8, ncands=100):
     t0 = time.perf_counter()
     tickets = {}
     for i, j in enumerate(jobs):
-        t = engine.submit(j["samples"], j["dt"], j["params"], lofreq=j.get("lofreq", 0.1),
-                          hifreq=j.get("hifreq", 125.0), lobin=j.get("lobin", 100),
-                          blocksize=j.get("blocksize", 0), threshold=j.get("threshold", 8.0))
-        tickets[t] = i
+        tickets[_submit_job(engine, j)] = i
     finals = [None] * len(jobs)
     infos = [None] * len(jobs)
     futs = {}
@@ -188,11 +264,7 @@ def serve(engine, jobs, arrivals, post_workers=8, ncands=100):
             j = jobs[i]
             with lock:
                 t_arr[i] = start + float(arrivals[i])
-                t = engine.submit(j["samples"], j["dt"], j["params"],
-                                  lofreq=j.get("lofreq", 0.1), hifreq=j.get("hifreq", 125.0),
-                                  lobin=j.get("lobin", 100), blocksize=j.get("blocksize", 0),
-                                  threshold=j.get("threshold", 8.0))
-                tickets[t] = i
+                tickets[_submit_job(engine, j)] = i
 
     th = threading.Thread(target=feeder, daemon=True)
     th.start()
@@ -224,4 +296,5 @@ def serve(engine, jobs, arrivals, post_workers=8, ncands=100):
     return finals, recs, time.perf_counter() - start
 
 
-__all__ = ["Engine", "PinnedSeries", "run_plain", "run_batched", "serve", "make_params"]
+__all__ = ["Engine", "PinnedSeries", "run_plain", "run_plain_device", "run_batched", "serve",
+           "sweep_jobs", "serving_jobs", "make_params"]
