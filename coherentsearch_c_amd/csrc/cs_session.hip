@@ -35,9 +35,8 @@ typedef hipError_t (*fast_launch_fn)(const csk::FastArgs *, int, hipStream_t);
 #define CS_DECL_FAST(NH, M, K)                                                    \
     extern "C" HIDDEN hipError_t cs_launch_fast_##NH##_##M##_##K(const csk::FastArgs *, \
                                                                 int, hipStream_t);
-#define CS_FAST_LIST(X) \
-    X(16, 16, 6) X(16, 32, 6) X(32, 16, 6) X(32, 32, 6) X(60, 16, 6) X(60, 32, 6)
-CS_FAST_LIST(CS_DECL_FAST)
+#define CS_DECL_NH(NH) CS_DECL_FAST(NH, 16, 16) CS_DECL_FAST(NH, 32, 16)
+CS_NH_LIST(CS_DECL_NH)
 
 namespace {
 
@@ -47,7 +46,8 @@ struct FastVariant {
 };
 
 #define CS_ENTRY_FAST(NH, M, K) {NH, M, K, cs_launch_fast_##NH##_##M##_##K},
-const FastVariant kFast[] = {CS_FAST_LIST(CS_ENTRY_FAST)};
+#define CS_ENTRY_NH(NH) CS_ENTRY_FAST(NH, 16, 16) CS_ENTRY_FAST(NH, 32, 16)
+const FastVariant kFast[] = {CS_NH_LIST(CS_ENTRY_NH)};
 
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
@@ -327,6 +327,7 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     std::vector<int2> hinfo;
     int ws;
     int rc;
+    int mp = 0;
 
     if (dplans->n != nh || nh < 1) {
         fprintf(stderr, "Error: plan set has %d harmonics, expected %d\n", dplans->n, nh);
@@ -354,8 +355,10 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         S->slot_Hk[S->nslot] = nh / k;
         S->nslot++;
     }
+    /* the kernel of the smallest padded tap count M >= m */
+    mp = m <= 16 ? 16 : (m <= 32 ? 32 : 0);
     for (const FastVariant &v : kFast) {
-        if (v.nh != nh || v.m != m) continue;
+        if (v.nh != nh || v.m != mp) continue;
         bool ok = true;
         for (int s = 1; s < S->nslot; s++) ok = ok && S->slot_k[s] <= v.kmax;
         if (ok) { S->fast = &v; break; }
@@ -382,16 +385,17 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     /* all harmonic windows of a tile must fit the kernel's LDS buffer */
     {
         double need = 0.0;
-        for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + m + 2;
-        if (need > (double)CS_WIN_CAP(nh, m)) {
+        for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + mp + 2;
+        if (need > (double)CS_WIN_CAP(nh, mp)) {
             fprintf(stderr, "Error: --hidr too large for the GPU tile window\n");
             goto fail;
         }
     }
 
-    /* plan arena: row (h, res) = W[0..m-1], A.re, A.im, zero padding to
-     * CS_WSP(m) floats; row index (h-1)*q + res must fit 32-bit offsets */
-    ws = CS_WSP(m);
+    /* plan arena: row (h, res) = M - m zeros, W[0..m-1], A.re, A.im, zero
+     * padding to CS_WSP(M) floats; row index (h-1)*q + res must fit 32-bit
+     * offsets */
+    ws = CS_WSP(mp);
     if ((double)nh * S->q * ws >= 4294967295.0 / 4) {
         fprintf(stderr, "Error: weight table too large\n");
         goto fail;
@@ -405,9 +409,9 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
             if (pl->row[res] == 0) continue;
             const int ph = pl->row[res] - 1;
             float *row = &wtab[((size_t)(h - 1) * S->q + res) * ws];
-            memcpy(row, pl->W + (size_t)ph * m, (size_t)m * sizeof(float));
-            row[m] = pl->A_re[ph];
-            row[m + 1] = pl->A_im[ph];
+            memcpy(row + (mp - m), pl->W + (size_t)ph * m, (size_t)m * sizeof(float));
+            row[mp] = pl->A_re[ph];
+            row[mp + 1] = pl->A_im[ph];
             if (pl->carry[ph] && res < cthr) cthr = res;
         }
         for (int res = 0; res < S->q; res++) {
@@ -481,6 +485,7 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->args.pnum = S->pnum;
     S->args.ws = ws;
     S->args.grp = S->q < CS_TILE ? CS_TILE / S->q : 1;
+    S->args.m2e = mp - m / 2;
     S->args.chunks = S->d_chunks;
     S->args.recs = S->d_recs;
     S->args.counter = S->d_counter;

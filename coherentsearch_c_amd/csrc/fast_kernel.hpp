@@ -301,6 +301,7 @@ struct FastArgs {
     int pnum;
     int ws;                   /* weight row stride (floats) */
     int grp;                  /* trials of the tile per residue class, max(1, CS_TILE/q) */
+    int m2e;                  /* first tap at base + 1 - m2e: M - m/2 (rows front-padded to M taps) */
     const ChunkDesc *chunks;
     CandRec *recs;
     unsigned int *counter;
@@ -372,7 +373,7 @@ void fast_search_kernel(FastArgs a)
     const int lj = T.y + (active ? slot_l : T.z - 1);
     const long long t = i0 + lj;
     const int q = a.q;
-    constexpr int m2 = M / 2;
+    const int m2 = a.m2e;
 
     /* per-harmonic windows of this tile; harmonics that do not fit the chunk
      * get a window too (zero outside the spectrum) so that the interpolation

@@ -1,0 +1,13 @@
+/* Fused search kernel instantiations for nharms = 1, 17, 33, 49 (M = 16, 32 padded taps;
+ * decimations k <= 16).  Harmonic counts are interleaved across the group
+ * files so that their compile times balance. */
+#include "fast_kernel.hpp"
+
+CS_FAST_INSTANTIATE(1, 16, 16)
+CS_FAST_INSTANTIATE(1, 32, 16)
+CS_FAST_INSTANTIATE(17, 16, 16)
+CS_FAST_INSTANTIATE(17, 32, 16)
+CS_FAST_INSTANTIATE(33, 16, 16)
+CS_FAST_INSTANTIATE(33, 32, 16)
+CS_FAST_INSTANTIATE(49, 16, 16)
+CS_FAST_INSTANTIATE(49, 32, 16)

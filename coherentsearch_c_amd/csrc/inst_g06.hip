@@ -1,0 +1,13 @@
+/* Fused search kernel instantiations for nharms = 7, 23, 39, 55 (M = 16, 32 padded taps;
+ * decimations k <= 16).  Harmonic counts are interleaved across the group
+ * files so that their compile times balance. */
+#include "fast_kernel.hpp"
+
+CS_FAST_INSTANTIATE(7, 16, 16)
+CS_FAST_INSTANTIATE(7, 32, 16)
+CS_FAST_INSTANTIATE(23, 16, 16)
+CS_FAST_INSTANTIATE(23, 32, 16)
+CS_FAST_INSTANTIATE(39, 16, 16)
+CS_FAST_INSTANTIATE(39, 32, 16)
+CS_FAST_INSTANTIATE(55, 16, 16)
+CS_FAST_INSTANTIATE(55, 32, 16)

@@ -64,6 +64,30 @@ def test_every_trial_metric_bit_identical(oracle, nh, m, lobin):
     sess.close()
 
 
+# every harmonic count has a kernel (M = 16 or 32 padded taps, m <= M front-padded
+# with zero weights; decimations up to 16): odd and prime counts, small and
+# large m, long decimation lists
+OTHER = [(1, 16, 1), (7, 16, 6), (13, 8, 6), (20, 12, 10), (37, 24, 12), (45, 16, 6),
+         (64, 32, 16), (60, 16, 16), (17, 2, 8)]
+
+
+@pytest.mark.parametrize("nh,m,maxdecim", OTHER)
+def test_other_configurations_bit_identical(oracle, nh, m, maxdecim):
+    spec, p, o, sess = _case(oracle, nh, m, lobin=100, nbins=1 << 14, maxdecim=maxdecim)
+    ci = o.nchunks // 2
+    i0, n, _ = o.chunk(ci)
+    say("config nh", nh, "m", m, "maxdecim", maxdecim, "trials", o.total)
+    want, _ = o.stack(ci)
+    got = sess.dump_stack(i0, n)
+    assert np.array_equal(got.transpose(1, 0, 2).view(np.uint32), want.view(np.uint32))
+    got = sess.run(-np.inf)
+    want = o.search(-np.inf, 1)
+    assert got.size == want.size
+    for f in ("r", "nharm", "metric"):
+        assert np.array_equal(got[f], want[f]), f
+    sess.close()
+
+
 def test_config1_candidate_list_identical(oracle):
     """BASELINE config 1 (2^20-bin .fft, pulsar at r0 = 12345.37, 60 harmonics,
     S/N 15), blocksize 2048, threshold 8: raw lists identical in order and
@@ -140,5 +164,6 @@ def test_session_reuse_and_mismatch(oracle):
 def test_unsupported_configuration_fails_loudly():
     amps = synth.spectrum_with_pulsar(1 << 14, 300.1, 4, 0.05, 5.0, 1)
     spec = cs.Spectrum(amps, 1 << 15, synth.DT)
-    with pytest.raises(RuntimeError):
-        cs.Session(spec, cs.make_params(nharms=17), blocksize=2048)
+    for kw in (dict(nharms=16, m=34), dict(nharms=65), dict(nharms=16, hidr=0.9)):
+        with pytest.raises(RuntimeError):
+            cs.Session(spec, cs.make_params(**kw), blocksize=2048)
