@@ -38,13 +38,13 @@ typedef struct {
     unsigned long long filled;      /* bit h-1: harmonic h fits the chunk */
     int nvalid[CS_MAXSLOT];         /* trials below the decimated Nyquist */
     int valid[CS_MAXSLOT];          /* 1/sigma > 0 */
-    float invsw[CS_MAXSLOT][CS_NSEQ];
+    float invsw[CS_MAXSLOT][CS_NSEQ];     /* f32 path: (float)(invsigma*shape_w) */
+    double invswd[CS_MAXSLOT][CS_NSEQ];   /* f64 path: invsigma*shape_w */
 } ChunkDesc;
 
 typedef struct {
     unsigned long long key;         /* chunk << 40 | slot << 32 | j */
-    float metric;
-    unsigned int pad;
+    double metric;                  /* the f32 metric widened exactly, or the f64 metric */
 } CandRec;
 
 /* harmonic counts with a fused kernel (each for M = 16 and 32 padded taps,

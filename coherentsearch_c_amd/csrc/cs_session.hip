@@ -35,17 +35,23 @@ typedef hipError_t (*fast_launch_fn)(const csk::FastArgs *, int, hipStream_t);
 #define CS_DECL_FAST(NH, M, K)                                                    \
     extern "C" HIDDEN hipError_t cs_launch_fast_##NH##_##M##_##K(const csk::FastArgs *, \
                                                                 int, hipStream_t);
-#define CS_DECL_NH(NH) CS_DECL_FAST(NH, 16, 16) CS_DECL_FAST(NH, 32, 16)
+#define CS_DECL_FAST64(NH, M, K)                                                  \
+    extern "C" HIDDEN hipError_t cs_launch_fast64_##NH##_##M##_##K(const csk::FastArgs *, \
+                                                                  int, hipStream_t);
+#define CS_DECL_NH(NH) CS_DECL_FAST(NH, 16, 16) CS_DECL_FAST(NH, 32, 16) \
+    CS_DECL_FAST64(NH, 16, 16) CS_DECL_FAST64(NH, 32, 16)
 CS_NH_LIST(CS_DECL_NH)
 
 namespace {
 
 struct FastVariant {
     int nh, m, kmax;
-    fast_launch_fn fn;
+    fast_launch_fn fn;        /* f32 profiles */
+    fast_launch_fn fn64;      /* f64 profiles */
 };
 
-#define CS_ENTRY_FAST(NH, M, K) {NH, M, K, cs_launch_fast_##NH##_##M##_##K},
+#define CS_ENTRY_FAST(NH, M, K)                                                    \
+    {NH, M, K, cs_launch_fast_##NH##_##M##_##K, cs_launch_fast64_##NH##_##M##_##K},
 #define CS_ENTRY_NH(NH) CS_ENTRY_FAST(NH, 16, 16) CS_ENTRY_FAST(NH, 32, 16)
 const FastVariant kFast[] = {CS_NH_LIST(CS_ENTRY_NH)};
 
@@ -109,11 +115,18 @@ struct GpuSession {
     unsigned int cap;
     csk::FastArgs args;
     double last_ms;
+    bool f64;                     /* --precision f64: profiles and metrics in double */
     bool async_alloc;             /* stream-ordered allocations on st (engine sessions) */
     bool own_stream;
     std::vector<float> h_wtab;    /* host copies kept alive for asynchronous uploads */
     std::vector<int2> h_hinfo;
 };
+
+/* cos and sin as two separate libm calls, through pointers the compiler cannot
+ * merge into one sincos call (which differs from them in the last bit for some
+ * angles): the c2r twiddle definition is libm cos and sin. */
+static double (*volatile cs_cos_fn)(double) = cos;
+static double (*volatile cs_sin_fn)(double) = sin;
 
 /* The project c2r's twiddles tw(j, N) = (cos, sin)(2 pi j / N), N < CS_TW_NMAX:
  * one table per device, shared by every session and held until the process
@@ -129,11 +142,38 @@ static float2 *shared_twiddles(int device)
         for (int Nn = 1; Nn < CS_TW_NMAX; Nn++)
             for (int j = 0; j < Nn; j++) {
                 const double ang = 2.0 * M_PI * (double)j / (double)Nn;
-                tw[(size_t)cs_tw_off(Nn) + j] = make_float2((float)cos(ang), (float)sin(ang));
+                tw[(size_t)cs_tw_off(Nn) + j] =
+                    make_float2((float)cs_cos_fn(ang), (float)cs_sin_fn(ang));
             }
         float2 *d = NULL;
         if (hipMalloc(&d, tw.size() * sizeof(float2)) != hipSuccess) return NULL;
         if (hipMemcpy(d, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+            hipFree(d);
+            return NULL;
+        }
+        tab[device] = d;
+    }
+    return tab[device];
+}
+
+/* the same table in double precision for the f64 path */
+static double2 *shared_twiddles64(int device)
+{
+    static std::mutex mu;
+    static double2 *tab[64] = {};
+    if (device < 0 || device >= 64) return NULL;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!tab[device]) {
+        std::vector<double2> tw(CS_TW_SIZE);
+        for (int Nn = 1; Nn < CS_TW_NMAX; Nn++)
+            for (int j = 0; j < Nn; j++) {
+                const double ang = 2.0 * M_PI * (double)j / (double)Nn;
+                tw[(size_t)cs_tw_off(Nn) + j] = make_double2(cs_cos_fn(ang), cs_sin_fn(ang));
+            }
+        double2 *d = NULL;
+        if (hipMalloc(&d, tw.size() * sizeof(double2)) != hipSuccess) return NULL;
+        if (hipMemcpy(d, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice) !=
             hipSuccess) {
             hipFree(d);
             return NULL;
@@ -274,11 +314,17 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
                 csx_boxcar_shape(shape[s], nb, widths[s], nw[s], filled.data(), k, Hk);
                 shapeH[s] = nf;
             }
-            const float sig = (float)csx_analytic_sigma(filled.data(), k, Hk);
+            /* f32: sigma and 1/sigma in float, invsw = f32(invsigma*shape_w);
+             * f64: everything in double (src/search.c:1001-1002, boxcar.c) */
+            const double sigd = csx_analytic_sigma(filled.data(), k, Hk);
+            const float sig = (float)sigd;
             const float invsigma = sig > 0.0f ? 1.0f / sig : 0.0f;
-            C.valid[s] = invsigma > 0.0f;
-            for (int i = 0; i < nw[s]; i++)
+            const double invsd = sigd > 0.0 ? 1.0 / sigd : 0.0;
+            C.valid[s] = S->f64 ? invsd > 0.0 : invsigma > 0.0f;
+            for (int i = 0; i < nw[s]; i++) {
                 C.invsw[s][wsel[s][i]] = (float)((double)invsigma * shape[s][i]);
+                C.invswd[s][wsel[s][i]] = invsd * shape[s][i];
+            }
         }
         for (int j0 = 0; j0 < n; j0 += CS_TILE) {
             S->tiles.push_back((int)c);
@@ -319,6 +365,7 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->d_stack = NULL; S->st = NULL; S->ev0 = NULL; S->ev1 = NULL;
     S->async_alloc = false;
     S->own_stream = false;
+    S->f64 = p->precision == CS_PREC_F64;
     S->last_ms = 0.0;
     S->fast = NULL;
 
@@ -333,8 +380,9 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         fprintf(stderr, "Error: plan set has %d harmonics, expected %d\n", dplans->n, nh);
         goto fail;
     }
-    if (p->precision != CS_PREC_F32 || p->sigma != CS_SIGMA_ANALYTIC) {
-        fprintf(stderr, "Error: this build's GPU search implements --precision f32 with "
+    if ((p->precision != CS_PREC_F32 && p->precision != CS_PREC_F64) ||
+        p->sigma != CS_SIGMA_ANALYTIC) {
+        fprintf(stderr, "Error: this build's GPU search implements --precision f32/f64 with "
                         "--sigma analytic\n");
         goto fail;
     }
@@ -455,7 +503,8 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->h_wtab.swap(wtab);
     S->h_hinfo.swap(hinfo);
     S->d_tw = shared_twiddles(device);
-    if (!S->d_tw) {
+    S->args.twd = S->f64 ? shared_twiddles64(device) : NULL;
+    if (!S->d_tw || (S->f64 && !S->args.twd)) {
         fprintf(stderr, "Error: twiddle table upload failed\n");
         goto fail;
     }
@@ -530,7 +579,9 @@ static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
     hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
     if (hipGetLastError() != hipSuccess) return -1;
     hipEventRecord(S->ev0, S->st);
-    if (ntiles > 0 && S->fast->fn(&S->args, nblk, S->st) != hipSuccess) return -1;
+    if (ntiles > 0 &&
+        (S->f64 ? S->fast->fn64 : S->fast->fn)(&S->args, nblk, S->st) != hipSuccess)
+        return -1;
     hipEventRecord(S->ev1, S->st);
     return 0;
 }

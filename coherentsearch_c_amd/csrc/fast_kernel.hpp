@@ -30,9 +30,11 @@
 
 namespace csk {
 
-struct cf {
-    float x, y;
+template <typename T>
+struct cx {
+    T x, y;
 };
+using cf = cx<float>;
 
 template <int... Is, class F>
 __device__ __forceinline__ void sfor_(std::integer_sequence<int, Is...>, F &&f)
@@ -95,33 +97,43 @@ constexpr int ncand(int nb)
 constexpr int wcmax(int nb) { return ncand(nb) > 0 ? wseq(ncand(nb) - 1) : 0; }
 
 /* Twiddles are read through the constant address space: every index is a
- * compile-time constant, so they become scalar loads and SGPR operands. */
+ * compile-time constant, so they become scalar loads and SGPR operands.  The
+ * f32 path uses (float)cos/sin, the f64 path the double values. */
 typedef const __attribute__((address_space(4))) float twc_t;
+typedef const __attribute__((address_space(4))) double twd_t;
 
-__device__ __forceinline__ float2 ldtw(twc_t *__restrict__ tw, int i)
+__device__ __forceinline__ cx<float> ldtw(twc_t *__restrict__ tw, int i)
 {
-    return make_float2(tw[2 * i], tw[2 * i + 1]);
+    return {tw[2 * i], tw[2 * i + 1]};
 }
+__device__ __forceinline__ cx<double> ldtw(twd_t *__restrict__ tw, int i)
+{
+    return {tw[2 * i], tw[2 * i + 1]};
+}
+
+__device__ __forceinline__ float fm(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fm(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 /* y * w with the project c2r's fused form:
  * (fma(y.re, w.re, -(y.im*w.im)), fma(y.re, w.im, y.im*w.re)) */
-__device__ __forceinline__ cf cmul(cf a, float2 w)
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> w)
 {
-    const float p2 = a.y * w.y, p4 = a.y * w.x;
-    return {__builtin_fmaf(a.x, w.x, -p2), __builtin_fmaf(a.x, w.y, p4)};
+    const T p2 = a.y * w.y, p4 = a.y * w.x;
+    return {fm(a.x, w.x, -p2), fm(a.x, w.y, p4)};
 }
 
-template <int R>
-__device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], twc_t *__restrict__ tw)
+template <int R, typename T, typename TP>
+__device__ __forceinline__ void bfly(const cx<T> (&u)[R], cx<T> (&v)[R], TP tw)
 {
     if constexpr (R == 2) {
         v[0] = {u[0].x + u[1].x, u[0].y + u[1].y};
         v[1] = {u[0].x - u[1].x, u[0].y - u[1].y};
     } else if constexpr (R == 4) {
-        const cf t0 = {u[0].x + u[2].x, u[0].y + u[2].y};
-        const cf t1 = {u[0].x - u[2].x, u[0].y - u[2].y};
-        const cf t2 = {u[1].x + u[3].x, u[1].y + u[3].y};
-        const cf t3 = {u[1].x - u[3].x, u[1].y - u[3].y};
+        const cx<T> t0 = {u[0].x + u[2].x, u[0].y + u[2].y};
+        const cx<T> t1 = {u[0].x - u[2].x, u[0].y - u[2].y};
+        const cx<T> t2 = {u[1].x + u[3].x, u[1].y + u[3].y};
+        const cx<T> t3 = {u[1].x - u[3].x, u[1].y - u[3].y};
         v[0] = {t0.x + t2.x, t0.y + t2.y};
         v[2] = {t0.x - t2.x, t0.y - t2.y};
         v[1] = {t1.x - t3.y, t1.y + t3.x};
@@ -131,13 +143,13 @@ __device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], twc_t *__rest
          * output pair A = u0 + sum sp_r c (fused, left to right) and
          * B = sum sm_r sn (first term a product, the rest fused) */
         constexpr int hh = (R - 1) / 2;
-        cf sp[hh + 1], sm[hh + 1];
+        cx<T> sp[hh + 1], sm[hh + 1];
         sfor<hh>([&](auto I) {
             constexpr int r = decltype(I)::value + 1;
             sp[r] = {u[r].x + u[R - r].x, u[r].y + u[R - r].y};
             sm[r] = {u[r].x - u[R - r].x, u[r].y - u[R - r].y};
         });
-        cf x0 = u[0];
+        cx<T> x0 = u[0];
         sfor<hh>([&](auto I) {
             constexpr int r = decltype(I)::value + 1;
             x0 = {x0.x + sp[r].x, x0.y + sp[r].y};
@@ -145,14 +157,14 @@ __device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], twc_t *__rest
         v[0] = x0;
         sfor<hh>([&](auto S) {
             constexpr int s = decltype(S)::value + 1;
-            cf A = u[0], B = {0.0f, 0.0f};
+            cx<T> A = u[0], B = {T(0), T(0)};
             sfor<hh>([&](auto I) {
                 constexpr int r = decltype(I)::value + 1;
                 constexpr int j = (r * s) % R;
-                const float2 w = ldtw(tw, twoff(R) + j);
-                A = {__builtin_fmaf(sp[r].x, w.x, A.x), __builtin_fmaf(sp[r].y, w.x, A.y)};
+                const cx<T> w = ldtw(tw, twoff(R) + j);
+                A = {fm(sp[r].x, w.x, A.x), fm(sp[r].y, w.x, A.y)};
                 if constexpr (r == 1) B = {sm[r].x * w.y, sm[r].y * w.y};
-                else B = {__builtin_fmaf(sm[r].x, w.y, B.x), __builtin_fmaf(sm[r].y, w.y, B.y)};
+                else B = {fm(sm[r].x, w.y, B.x), fm(sm[r].y, w.y, B.y)};
             });
             v[s] = {A.x - B.y, A.y + B.x};
             v[R - s] = {A.x + B.y, A.y - B.x};
@@ -165,30 +177,30 @@ __device__ __forceinline__ void bfly(const cf (&u)[R], cf (&v)[R], twc_t *__rest
  * DFT_N1 over x[(N2*n1 + N1*n2) mod N] for every n2, then DFT_N2 over n2 for
  * every k1, written to the CRT index -- no twiddles, only renaming.  A prime
  * power: decimation in time with radix 4, 2 or the prime, twiddles cmul'd. */
-template <int N>
-__device__ __forceinline__ void idft(const cf (&x)[N], cf (&X)[N], twc_t *__restrict__ tw)
+template <int N, typename T, typename TP>
+__device__ __forceinline__ void idft(const cx<T> (&x)[N], cx<T> (&X)[N], TP tw)
 {
     if constexpr (N == 1) {
         X[0] = x[0];
     } else if constexpr (pfa_n1(N) < N) {
         constexpr int N1 = pfa_n1(N), N2 = N / N1;
         constexpr int E1 = N2 * modinv(N2, N1), E2 = N1 * modinv(N1, N2);
-        cf T[N2][N1];
+        cx<T> Tm[N2][N1];
         sfor<N2>([&](auto I2) {
             constexpr int n2 = decltype(I2)::value;
-            cf g[N1];
+            cx<T> g[N1];
             sfor<N1>([&](auto I1) {
                 constexpr int n1 = decltype(I1)::value;
                 g[n1] = x[(N2 * n1 + N1 * n2) % N];
             });
-            idft<N1>(g, T[n2], tw);
+            idft<N1>(g, Tm[n2], tw);
         });
         sfor<N1>([&](auto K1) {
             constexpr int k1 = decltype(K1)::value;
-            cf c[N2], o[N2];
+            cx<T> c[N2], o[N2];
             sfor<N2>([&](auto I2) {
                 constexpr int n2 = decltype(I2)::value;
-                c[n2] = T[n2][k1];
+                c[n2] = Tm[n2][k1];
             });
             idft<N2>(c, o, tw);
             sfor<N2>([&](auto K2) {
@@ -198,10 +210,10 @@ __device__ __forceinline__ void idft(const cf (&x)[N], cf (&X)[N], twc_t *__rest
         });
     } else {
         constexpr int R = radix_of(N), M = N / R;
-        cf Y[R][M];
+        cx<T> Y[R][M];
         sfor<R>([&](auto I) {
             constexpr int r = decltype(I)::value;
-            cf g[M];
+            cx<T> g[M];
             sfor<M>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 g[j] = x[r + j * R];
@@ -210,7 +222,7 @@ __device__ __forceinline__ void idft(const cf (&x)[N], cf (&X)[N], twc_t *__rest
         });
         sfor<M>([&](auto K) {
             constexpr int k = decltype(K)::value;
-            cf u[R], v[R];
+            cx<T> u[R], v[R];
             u[0] = Y[0][k];
             sfor<R - 1>([&](auto I) {
                 constexpr int r = decltype(I)::value + 1;
@@ -226,24 +238,25 @@ __device__ __forceinline__ void idft(const cf (&x)[N], cf (&X)[N], twc_t *__rest
     }
 }
 
-/* Project c2r of X[0], X[XS], ..., X[H*XS] into x[0..2H-1] (unnormalised). */
-template <int H, int XS, int LEN>
-__device__ __forceinline__ void c2r(const cf (&X)[LEN], float (&x)[2 * H],
-                                    twc_t *__restrict__ tw)
+/* Project c2r of X[0], X[XS], ..., X[H*XS] (the f32 stack, widened to T) into
+ * x[0..2H-1] (unnormalised), in precision T. */
+template <int H, int XS, int LEN, typename T, typename TP>
+__device__ __forceinline__ void c2r(const cf (&X)[LEN], T (&x)[2 * H], TP tw)
 {
-    cf Z[H];
+    cx<T> Z[H];
     {
-        const float x0 = X[0].x, xh = X[H * XS].x;
+        const T x0 = (T)X[0].x, xh = (T)X[H * XS].x;
         Z[0] = {x0 + xh, x0 - xh};
     }
     sfor<H - 1>([&](auto I) {
         constexpr int k = decltype(I)::value + 1;
-        const cf a = X[k * XS], b = X[(H - k) * XS];
-        const float er = a.x + b.x, ei = a.y - b.y;
-        const cf o = cmul({a.x - b.x, a.y + b.y}, ldtw(tw, twoff(2 * H) + k));
+        const cx<T> a = {(T)X[k * XS].x, (T)X[k * XS].y};
+        const cx<T> b = {(T)X[(H - k) * XS].x, (T)X[(H - k) * XS].y};
+        const T er = a.x + b.x, ei = a.y - b.y;
+        const cx<T> o = cmul(cx<T>{a.x - b.x, a.y + b.y}, ldtw(tw, twoff(2 * H) + k));
         Z[k] = {er - o.y, ei + o.x};
     });
-    cf z[H];
+    cx<T> z[H];
     idft<H>(Z, z, tw);
     sfor<H>([&](auto I) {
         constexpr int s = decltype(I)::value;
@@ -252,41 +265,58 @@ __device__ __forceinline__ void c2r(const cf (&X)[LEN], float (&x)[2 * H],
     });
 }
 
+__device__ __forceinline__ float tmax(float m, float d) { return fmaxf(m, d); }
+__device__ __forceinline__ double tmax(double m, double d) { return d > m ? d : m; }
+
 /* Boxcar S/N of one profile over the enabled widths (mask over the fsp=1.5
- * candidates), src/boxcar.c:203-231: wrapped prefix sums, max window sum per
- * width, (m - duty*total) * f32(invsigma*shape_w). */
-template <int NB>
-__device__ __forceinline__ float boxcar(const float (&x)[NB], unsigned wmask,
-                                        const float *__restrict__ invsw, int valid)
+ * candidates), src/boxcar.c:203-252 in precision T: wrapped prefix sums, max
+ * window sum per width, (m - duty*total) * invsw_w with invsw_w =
+ * invsigma*shape_w rounded to T. */
+template <int NB, typename T>
+__device__ __forceinline__ T boxcar(const T (&x)[NB], unsigned wmask,
+                                    const T *__restrict__ invsw, int valid)
 {
     constexpr int NC = ncand(NB);
     constexpr int WC = wcmax(NB);
-    float P[NB + WC + 1];
-    P[0] = 0.0f;
+    T P[NB + WC + 1];
+    P[0] = T(0);
     sfor<NB + WC>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
         constexpr int idx = (i > NB ? i - NB : i) - 1;
         P[i] = P[i - 1] + x[idx];
     });
-    const float stot = P[NB] - P[0];
-    float best = -INFINITY;
+    const T stot = P[NB] - P[0];
+    T best = -(T)INFINITY;
     sfor<NC>([&](auto I) {
         constexpr int wi = decltype(I)::value;
         constexpr int W = wseq(wi);
         if (wmask & (1u << wi)) {
-            float m = P[W] - P[0];
+            T m = P[W] - P[0];
             sfor<NB - 1>([&](auto J) {
                 constexpr int p = decltype(J)::value + 2;
-                const float d = P[p + W - 1] - P[p - 1];
-                m = fmaxf(m, d);
+                const T d = P[p + W - 1] - P[p - 1];
+                m = tmax(m, d);
             });
-            constexpr float duty = (float)W / (float)NB;
-            const float dt = duty * stot;
-            const float c = (m - dt) * invsw[wi];
+            constexpr T duty = (T)W / (T)NB;
+            const T dt = duty * stot;
+            const T c = (m - dt) * invsw[wi];
             best = c > best ? c : best;
         }
     });
-    return valid ? best : 0.0f;
+    return valid ? best : T(0);
+}
+
+template <typename T>
+__device__ __forceinline__ const T *invsw_of(const ChunkDesc *C, int slot);
+template <>
+__device__ __forceinline__ const float *invsw_of<float>(const ChunkDesc *C, int slot)
+{
+    return C->invsw[slot];
+}
+template <>
+__device__ __forceinline__ const double *invsw_of<double>(const ChunkDesc *C, int slot)
+{
+    return C->invswd[slot];
 }
 
 struct FastArgs {
@@ -294,7 +324,8 @@ struct FastArgs {
     long long namps;
     const float *wtab;
     const int2 *hinfo;        /* (rfloor0, carry residue threshold) per h */
-    const float2 *tw;
+    const float2 *tw;         /* (float)cos/sin table */
+    const double2 *twd;       /* double cos/sin table (f64 path) */
     const int4 *tiles;
     int ntiles;
     int q;
@@ -314,7 +345,7 @@ struct FastArgs {
     long long dump_n;
 };
 
-__device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, float metric,
+__device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, double metric,
                                      const FastArgs &a)
 {
     const unsigned long long mask = __ballot(pred);
@@ -331,17 +362,21 @@ __device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, floa
             r.key = ((unsigned long long)chunk << 40) | ((unsigned long long)slot << 32) |
                     (unsigned int)j;
             r.metric = metric;
-            r.pad = 0u;
             a.recs[pos] = r;
         }
     }
 }
 
-/* Occupancy target: 2 waves/SIMD for the largest stacks, 3 below. */
-constexpr int waves_for(int nh) { return nh > 32 ? 2 : 3; }
+/* Occupancy target: f32 profiles 2 waves/SIMD for the largest stacks, 3
+ * below; f64 profiles (twice the registers) 1 above 16 harmonics, 2 below. */
+template <typename T>
+constexpr int waves_for(int nh)
+{
+    return sizeof(T) == 4 ? (nh > 32 ? 2 : 3) : (nh > 16 ? 1 : 2);
+}
 
-template <int NH, int M, int KMAX>
-__global__ __launch_bounds__(CS_TILE) __attribute__((amdgpu_waves_per_eu(waves_for(NH))))
+template <int NH, int M, int KMAX, typename T>
+__global__ __launch_bounds__(CS_TILE) __attribute__((amdgpu_waves_per_eu(waves_for<T>(NH))))
 void fast_search_kernel(FastArgs a)
 {
     static_assert(NH >= 1 && NH <= 64, "fast path holds up to 64 harmonics");
@@ -357,8 +392,8 @@ void fast_search_kernel(FastArgs a)
     const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     if (tile >= a.ntiles) return;
 
-    const int4 T = a.tiles[tile];
-    const ChunkDesc *C = a.chunks + T.x;
+    const int4 Tl = a.tiles[tile];
+    const ChunkDesc *C = a.chunks + Tl.x;
     const long long i0 = C->i0;
     const unsigned long long filled = C->filled;
     const int tid = threadIdx.x;
@@ -369,8 +404,8 @@ void fast_search_kernel(FastArgs a)
      * whole tile either way. */
     const int G = a.grp;
     const int slot_l = tid < G * a.q ? (tid % G) * a.q + tid / G : tid;
-    const bool active = slot_l < T.z;
-    const int lj = T.y + (active ? slot_l : T.z - 1);
+    const bool active = slot_l < Tl.z;
+    const int lj = Tl.y + (active ? slot_l : Tl.z - 1);
     const long long t = i0 + lj;
     const int q = a.q;
     const int m2 = a.m2e;
@@ -381,7 +416,7 @@ void fast_search_kernel(FastArgs a)
     if (tid < NH) {
         const int h = tid + 1;
         const int2 hi = a.hinfo[tid];
-        const long long tf = i0 + T.y, tl = i0 + T.y + T.z - 1;
+        const long long tf = i0 + Tl.y, tl = i0 + Tl.y + Tl.z - 1;
         const long long af = tf * a.pnum, al = tl * a.pnum;
         const int rf = (int)(af % q), rl = (int)(al % q);
         const long long qf = af / q, ql = al / q;
@@ -484,7 +519,12 @@ void fast_search_kernel(FastArgs a)
     }
 
     const double thr = a.threshold;
-    const int chunk = T.x;
+    const int chunk = Tl.x;
+    /* the c2r twiddles of this precision */
+    auto twp = [&]() {
+        if constexpr (sizeof(T) == 4) return (twc_t *)a.tw;
+        else return (twd_t *)a.twd;
+    }();
 
     /* decimations k not dividing nharms: strided c2r of the stack */
     sfor<KMAX - 1>([&](auto I) {
@@ -493,17 +533,18 @@ void fast_search_kernel(FastArgs a)
             const int slot = a.kslot[k];
             if (slot >= 0) {
                 constexpr int Hk = NH / k;
-                float xk[2 * Hk];
-                c2r<Hk, k, NH + 1>(X, xk, (twc_t *)a.tw);
-                const float mv = boxcar<2 * Hk>(xk, a.wmask[slot], C->invsw[slot], C->valid[slot]);
+                T xk[2 * Hk];
+                c2r<Hk, k, NH + 1>(X, xk, twp);
+                const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
+                                            C->valid[slot]);
                 emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
             }
         }
     });
 
     /* base profile */
-    float x[2 * NH];
-    c2r<NH, 1, NH + 1>(X, x, (twc_t *)a.tw);
+    T x[2 * NH];
+    c2r<NH, 1, NH + 1>(X, x, twp);
 
     /* divisor decimations fold the base profile (src/search.c:298-331) */
     sfor<KMAX - 1>([&](auto I) {
@@ -512,24 +553,24 @@ void fast_search_kernel(FastArgs a)
             const int slot = a.kslot[k];
             if (slot >= 0) {
                 constexpr int nk = 2 * (NH / k);
-                constexpr float invk = 1.0f / (float)k;
-                float xk[nk];
+                constexpr T invk = T(1) / (T)k;
+                T xk[nk];
                 sfor<nk>([&](auto J) {
                     constexpr int j = decltype(J)::value;
-                    float s = x[j];
+                    T s = x[j];
                     sfor<k - 1>([&](auto Q) {
                         constexpr int qq = decltype(Q)::value + 1;
                         s = s + x[j + qq * nk];
                     });
                     xk[j] = s * invk;
                 });
-                const float mv = boxcar<nk>(xk, a.wmask[slot], C->invsw[slot], C->valid[slot]);
+                const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot), C->valid[slot]);
                 emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
             }
         }
     });
 
-    const float mb = boxcar<2 * NH>(x, a.wmask[0], C->invsw[0], C->valid[0]);
+    const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0]);
     emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
 }
 
@@ -541,7 +582,17 @@ void fast_search_kernel(FastArgs a)
     cs_launch_fast_##NH##_##M##_##KMAX(const csk::FastArgs *a, int nblocks,      \
                                        hipStream_t st)                           \
     {                                                                            \
-        hipLaunchKernelGGL((csk::fast_search_kernel<NH, M, KMAX>), dim3(nblocks), \
-                           dim3(CS_TILE), 0, st, *a);                            \
+        hipLaunchKernelGGL((csk::fast_search_kernel<NH, M, KMAX, float>),        \
+                           dim3(nblocks), dim3(CS_TILE), 0, st, *a);             \
+        return hipGetLastError();                                                \
+    }
+
+#define CS_FAST64_INSTANTIATE(NH, M, KMAX)                                       \
+    extern "C" __attribute__((visibility("hidden"))) hipError_t                   \
+    cs_launch_fast64_##NH##_##M##_##KMAX(const csk::FastArgs *a, int nblocks,    \
+                                         hipStream_t st)                         \
+    {                                                                            \
+        hipLaunchKernelGGL((csk::fast_search_kernel<NH, M, KMAX, double>),       \
+                           dim3(nblocks), dim3(CS_TILE), 0, st, *a);             \
         return hipGetLastError();                                                \
     }

@@ -548,7 +548,8 @@ int orc_harmonic_fits(const OrcCtx *c, int h, long long t)
 /* op in IEEE arithmetic, with fused multiply-adds exactly where it    */
 /* writes fma and no other contraction, so their profiles are         */
 /* bit-identical; the reference uses FFTW/cuFFT instead):             */
-/*  tw(j,N) = ((T)cos(a), (T)sin(a)), a = 2*pi*(double)j/N            */
+/*  tw(j,N) = ((T)cos(a), (T)sin(a)), a = 2*pi*(double)j/N, cos and   */
+/*            sin separate libm calls (not a merged sincos)           */
 /*  cmul(y,w) = (fma(y.re, w.re, -(y.im*w.im)),                       */
 /*               fma(y.re, w.im,   y.im*w.re))                        */
 /*  c2r(X[0..H]) -> x[0..2H-1]:                                        */
@@ -581,11 +582,16 @@ int orc_harmonic_fits(const OrcCtx *c, int h, long long t)
 /*         B = sm_1*sn; B = fma(sm_r, sn, B) for r = 2, 3, ...;       */
 /*         X_s = (A.re-B.im, A.im+B.re), X_{p-s} = (A.re+B.im, A.im-B.re) */
 /* ------------------------------------------------------------------ */
+/* cos and sin as two separate libm calls, through pointers the compiler
+ * cannot merge into one sincos call (which differs from them in the last
+ * bit for some angles): the twiddle definition is libm cos and sin. */
+static double (*volatile orc_cos_fn)(double) = cos;
+static double (*volatile orc_sin_fn)(double) = sin;
 static inline void otw(int j, int N, double *c, double *s)
 {
     double a = 2.0 * M_PI * (double)j / (double)N;
-    *c = cos(a);
-    *s = sin(a);
+    *c = orc_cos_fn(a);
+    *s = orc_sin_fn(a);
 }
 
 static int ospf(int N)
@@ -821,6 +827,28 @@ void orc_c2r_f32(int H, const float *X, float *x)
     }
     free(xr); free(xi); free(wk);
 }
+/* IDFT_N of N interleaved complex values (the project definition above). */
+void orc_idft_f64(int N, const double *in, double *out)
+{
+    double *ir = malloc(sizeof(double) * N), *ii = malloc(sizeof(double) * N);
+    double *or_ = malloc(sizeof(double) * N), *oi = malloc(sizeof(double) * N);
+    long *io = calloc((size_t)N, sizeof(long));
+    for (int k = 0; k < N; k++) { ir[k] = in[2 * k]; ii[k] = in[2 * k + 1]; io[k] = k; }
+    dft_f64(N, ir, ii, io, or_, oi, io, 1);
+    for (int k = 0; k < N; k++) { out[2 * k] = or_[k]; out[2 * k + 1] = oi[k]; }
+    free(ir); free(ii); free(or_); free(oi); free(io);
+}
+void orc_idft_f32(int N, const float *in, float *out)
+{
+    float *ir = malloc(sizeof(float) * N), *ii = malloc(sizeof(float) * N);
+    float *or_ = malloc(sizeof(float) * N), *oi = malloc(sizeof(float) * N);
+    long *io = calloc((size_t)N, sizeof(long));
+    for (int k = 0; k < N; k++) { ir[k] = in[2 * k]; ii[k] = in[2 * k + 1]; io[k] = k; }
+    dft_f32(N, ir, ii, io, or_, oi, io, 1);
+    for (int k = 0; k < N; k++) { out[2 * k] = or_[k]; out[2 * k + 1] = oi[k]; }
+    free(ir); free(ii); free(or_); free(oi); free(io);
+}
+
 void orc_c2r_f64(int H, const double *X, double *x)
 {
     double *xr = malloc(sizeof(double) * (H + 1)), *xi = malloc(sizeof(double) * (H + 1));

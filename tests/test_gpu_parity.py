@@ -24,13 +24,14 @@ def say(*a):
 
 
 def _case(oracle, nh, m, lobin=100, nbins=1 << 15, blocksize=2048, r0=271.3, seed=1,
-          maxdecim=6, hifreq=125.0):
+          maxdecim=6, hifreq=125.0, precision="f32", sigma="analytic"):
     amps = synth.spectrum_with_pulsar(nbins, r0, nh, 0.05, 12.0, seed)
     spec = cs.Spectrum(amps, 2 * nbins, synth.DT)
-    p = cs.make_params(nharms=nh, m=m, maxdecim=maxdecim)
+    p = cs.make_params(nharms=nh, m=m, maxdecim=maxdecim, precision=precision, sigma=sigma)
     o = oracle.Oracle(amps.view(np.float32), spec.N, spec.T,
                       oracle.config(nharms=nh, m=m, lobin=lobin, blocksize=blocksize,
-                                    maxdecim=maxdecim, hifreq=hifreq))
+                                    maxdecim=maxdecim, hifreq=hifreq, precision=precision,
+                                    sigma=sigma))
     sess = cs.Session(spec, p, lobin=lobin, blocksize=blocksize, hifreq=hifreq)
     assert sess.total == o.total and np.array_equal(sess.cstarts, o.chunk_starts())
     return spec, p, o, sess
@@ -80,6 +81,21 @@ def test_other_configurations_bit_identical(oracle, nh, m, maxdecim):
     want, _ = o.stack(ci)
     got = sess.dump_stack(i0, n)
     assert np.array_equal(got.transpose(1, 0, 2).view(np.uint32), want.view(np.uint32))
+    got = sess.run(-np.inf)
+    want = o.search(-np.inf, 1)
+    assert got.size == want.size
+    for f in ("r", "nharm", "metric"):
+        assert np.array_equal(got[f], want[f]), f
+    sess.close()
+
+
+@pytest.mark.parametrize("nh,m,maxdecim", [(16, 16, 6), (32, 32, 6), (60, 16, 6), (7, 8, 3),
+                                            (45, 16, 12)])
+def test_f64_every_trial_metric_bit_identical(oracle, nh, m, maxdecim):
+    """--precision f64: the f32 stack widened to double, c2r, fold and boxcar
+    in double -- bit-identical to the oracle's f64 path."""
+    spec, p, o, sess = _case(oracle, nh, m, nbins=1 << 14, maxdecim=maxdecim, precision="f64")
+    say("f64 nh", nh, "m", m, "trials", o.total)
     got = sess.run(-np.inf)
     want = o.search(-np.inf, 1)
     assert got.size == want.size
