@@ -22,6 +22,7 @@
 #define CS_MAXSLOT 17          /* base + up to 16 decimations */
 #define CS_NSEQ 12             /* width candidates 1,2,3,4,6,9,13,19,28,42,63,94 */
 #define CS_TW_NMAX 257
+#define CS_SIGCAP 8192         /* measured sigma: samples per (chunk, decimation slot) */
 #define CS_TW_SIZE (CS_TW_NMAX * (CS_TW_NMAX - 1) / 2)
 /* LDS window slots (float2) per workgroup for nh harmonics of m taps: covers
  * sum_h (ceil(255*h*lodr) + m + 2) whenever hidr <= 0.5, rounded to 256. */

@@ -86,6 +86,72 @@ int wcap_of(int nb)
 
 }  // namespace
 
+/* Measured sigma of one (chunk, slot) per workgroup (src/boxcar.c:18-73,
+ * 138-167; src/search.c:1005-1050): the sample (|profile| values of the
+ * sampled rows, written by the search kernel's sample pass) is sorted in LDS
+ * by a bitonic network, sigma = 1.4826 * median (the mean of the two middle
+ * values for an even count), and the chunk's per-width factors
+ * invsigma*shape_w and valid flag are written for the search pass. */
+struct SigmaArgs {
+    int nslot;
+    int nb[CS_MAXSLOT];
+    unsigned int wmask[CS_MAXSLOT];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void sigma_kernel(const T *__restrict__ samp,
+                                                    ChunkDesc *__restrict__ chunks,
+                                                    const double *__restrict__ shp, SigmaArgs sa)
+{
+    __shared__ T v[CS_SIGCAP];
+    const int c = blockIdx.x / sa.nslot, s = blockIdx.x % sa.nslot;
+    ChunkDesc *C = chunks + c;
+    const int nb = sa.nb[s], nv = C->nvalid[s];
+    int ns;
+    if ((long long)nb * nv <= CS_SIGCAP) {
+        ns = nb * nv;
+    } else {
+        int ncol = CS_SIGCAP / nb;
+        if (ncol < 1) ncol = 1;
+        if (ncol > nv) ncol = nv;
+        ns = ncol * nb;
+    }
+    int P = 1;
+    while (P < ns) P <<= 1;
+    const T *src = samp + ((long long)c * sa.nslot + s) * CS_SIGCAP;
+    for (int i = threadIdx.x; i < P; i += blockDim.x) v[i] = i < ns ? src[i] : (T)INFINITY;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const T x = v[i], y = v[ixj];
+                    if ((x > y) == ((i & k) == 0)) {
+                        v[i] = y;
+                        v[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    if (threadIdx.x != 0) return;
+    T sig = 0;
+    if (ns > 0) {
+        const int half = ns >> 1;
+        const T med = (ns & 1) ? v[half] : (T)0.5 * (v[half - 1] + v[half]);
+        sig = (T)1.4826 * med;
+    }
+    const T invsigma = sig > (T)0 ? (T)1 / sig : (T)0;
+    C->valid[s] = invsigma > (T)0;
+    const double *sh = shp + ((long long)c * sa.nslot + s) * CS_NSEQ;
+    for (int wi = 0; wi < CS_NSEQ; wi++) {
+        if (!(sa.wmask[s] & (1u << wi))) continue;
+        if (sizeof(T) == 4) C->invsw[s][wi] = (float)((double)invsigma * sh[wi]);
+        else C->invswd[s][wi] = (double)invsigma * sh[wi];
+    }
+}
+
 struct GpuSession {
     const SearchParams *p;
     int device;
@@ -116,6 +182,10 @@ struct GpuSession {
     csk::FastArgs args;
     double last_ms;
     bool f64;                     /* --precision f64: profiles and metrics in double */
+    bool measured;                /* --sigma measured: sample pass + per-chunk sigma */
+    std::vector<double> h_shp;    /* shape_w per (chunk, slot, width candidate) */
+    double *d_shp;
+    void *d_samp;
     bool async_alloc;             /* stream-ordered allocations on st (engine sessions) */
     bool own_stream;
     std::vector<float> h_wtab;    /* host copies kept alive for asynchronous uploads */
@@ -212,6 +282,8 @@ static void session_release(GpuSession *S)
     dfree(S, S->d_recs);
     dfree(S, S->d_counter);
     dfree(S, S->d_stack);
+    dfree(S, S->d_shp);
+    dfree(S, S->d_samp);
     if (S->ev0) hipEventDestroy(S->ev0);
     if (S->ev1) hipEventDestroy(S->ev1);
     if (S->own_stream && S->st) hipStreamDestroy(S->st);
@@ -275,6 +347,7 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
     static thread_local double shape[CS_MAXSLOT][256];
     for (int s = 0; s < CS_MAXSLOT; s++) shapeH[s] = -1;
     S->chunks.resize((size_t)S->nchunks);
+    if (S->measured) S->h_shp.assign((size_t)S->nchunks * S->nslot * CS_NSEQ, 0.0);
     S->tiles.clear();
     S->ht = 0;
     for (long long c = 0; c < S->nchunks; c++) {
@@ -325,6 +398,9 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
                 C.invsw[s][wsel[s][i]] = (float)((double)invsigma * shape[s][i]);
                 C.invswd[s][wsel[s][i]] = invsd * shape[s][i];
             }
+            if (S->measured)
+                for (int i = 0; i < nw[s]; i++)
+                    S->h_shp[((size_t)c * S->nslot + s) * CS_NSEQ + wsel[s][i]] = shape[s][i];
         }
         for (int j0 = 0; j0 < n; j0 += CS_TILE) {
             S->tiles.push_back((int)c);
@@ -366,6 +442,9 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->async_alloc = false;
     S->own_stream = false;
     S->f64 = p->precision == CS_PREC_F64;
+    S->measured = p->sigma == CS_SIGMA_MEASURED;
+    S->d_shp = NULL;
+    S->d_samp = NULL;
     S->last_ms = 0.0;
     S->fast = NULL;
 
@@ -381,9 +460,8 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         goto fail;
     }
     if ((p->precision != CS_PREC_F32 && p->precision != CS_PREC_F64) ||
-        p->sigma != CS_SIGMA_ANALYTIC) {
-        fprintf(stderr, "Error: this build's GPU search implements --precision f32/f64 with "
-                        "--sigma analytic\n");
+        (p->sigma != CS_SIGMA_ANALYTIC && p->sigma != CS_SIGMA_MEASURED)) {
+        fprintf(stderr, "Error: unknown --precision or --sigma\n");
         goto fail;
     }
     /* slots: base, then decimations k > 1 in SearchParams order */
@@ -519,6 +597,12 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     HIPCHK(dalloc(S, (void **)&S->d_tiles, std::max<size_t>(S->tiles.size(), 4) * sizeof(int)));
     if (!S->tiles.empty())
         HIPCHK(dcopy(S, S->d_tiles, S->tiles.data(), S->tiles.size() * sizeof(int)));
+    if (S->measured && S->nchunks > 0) {
+        const size_t nsamp = (size_t)S->nchunks * S->nslot * CS_SIGCAP;
+        HIPCHK(dalloc(S, (void **)&S->d_samp, nsamp * (S->f64 ? sizeof(double) : sizeof(float))));
+        HIPCHK(dalloc(S, (void **)&S->d_shp, S->h_shp.size() * sizeof(double)));
+        HIPCHK(dcopy(S, S->d_shp, S->h_shp.data(), S->h_shp.size() * sizeof(double)));
+    }
     S->cap = 1u << 16;
     HIPCHK(dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)));
     HIPCHK(dalloc(S, (void **)&S->d_counter, sizeof(unsigned int)));
@@ -535,6 +619,8 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->args.ws = ws;
     S->args.grp = S->q < CS_TILE ? CS_TILE / S->q : 1;
     S->args.m2e = mp - m / 2;
+    S->args.nslot = S->nslot;
+    S->args.samp = NULL;
     S->args.chunks = S->d_chunks;
     S->args.recs = S->d_recs;
     S->args.counter = S->d_counter;
@@ -576,12 +662,36 @@ static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
     S->args.cap = S->cap;
     const int ntiles = S->args.ntiles;
     const int nblk = ((ntiles + 7) / 8) * 8;
+    fast_launch_fn fn = S->f64 ? S->fast->fn64 : S->fast->fn;
     hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
     if (hipGetLastError() != hipSuccess) return -1;
     hipEventRecord(S->ev0, S->st);
-    if (ntiles > 0 &&
-        (S->f64 ? S->fast->fn64 : S->fast->fn)(&S->args, nblk, S->st) != hipSuccess)
-        return -1;
+    if (S->measured && ntiles > 0) {
+        /* sample pass, then sigma and the per-width factors of every chunk */
+        S->args.samp = S->d_samp;
+        float2 *keep = S->args.stack_out;
+        S->args.stack_out = NULL;
+        const hipError_t e = fn(&S->args, nblk, S->st);
+        S->args.samp = NULL;
+        S->args.stack_out = keep;
+        if (e != hipSuccess) return -1;
+        SigmaArgs sa;
+        memset(&sa, 0, sizeof(sa));
+        sa.nslot = S->nslot;
+        for (int s = 0; s < S->nslot; s++) {
+            sa.nb[s] = 2 * S->slot_Hk[s];
+            sa.wmask[s] = S->args.wmask[s];
+        }
+        const unsigned grid = (unsigned)(S->nchunks * S->nslot);
+        if (S->f64)
+            hipLaunchKernelGGL(sigma_kernel<double>, dim3(grid), dim3(256), 0, S->st,
+                               (const double *)S->d_samp, S->d_chunks, S->d_shp, sa);
+        else
+            hipLaunchKernelGGL(sigma_kernel<float>, dim3(grid), dim3(256), 0, S->st,
+                               (const float *)S->d_samp, S->d_chunks, S->d_shp, sa);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
+    if (ntiles > 0 && fn(&S->args, nblk, S->st) != hipSuccess) return -1;
     hipEventRecord(S->ev1, S->st);
     return 0;
 }

@@ -340,6 +340,9 @@ struct FastArgs {
     double threshold;
     int kslot[CS_MAXSLOT];    /* slot index of decimation k (k<=16), -1 off */
     unsigned int wmask[CS_MAXSLOT];   /* per slot */
+    void *samp;               /* measured sigma sample pass: |profile| samples per
+                                 (chunk, slot), CS_SIGCAP of T each; null otherwise */
+    int nslot;
     float2 *stack_out;        /* optional stack dump of trials [dump_t0, +dump_n) */
     long long dump_t0;
     long long dump_n;
@@ -365,6 +368,34 @@ __device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, doub
             a.recs[pos] = r;
         }
     }
+}
+
+/* Measured sigma (src/boxcar.c:18-73, 138-167): the sample of (chunk, slot)
+ * is the |value| of every profile bin of the valid trials when that is at
+ * most CS_SIGCAP values, otherwise the whole profiles of the trials
+ * j0 = c*nv/ncol, c < ncol = CS_SIGCAP/nb, in that order.  A lane writes its
+ * profile at its row's place in the sample when its trial is one of them. */
+template <int NB, typename T>
+__device__ __forceinline__ void sample_out(const T (&x)[NB], bool active, int lj, int nv,
+                                           int chunk, int slot, const FastArgs &a)
+{
+    if (!active || lj >= nv) return;
+    long long pos;
+    if ((long long)NB * nv <= CS_SIGCAP) {
+        pos = (long long)lj * NB;
+    } else {
+        long long ncol = CS_SIGCAP / NB;
+        if (ncol < 1) ncol = 1;
+        if (ncol > nv) ncol = nv;
+        const long long c = ((long long)lj * ncol + nv - 1) / nv;   /* ceil(lj*ncol/nv) */
+        if (c >= ncol || (c * nv) / ncol != lj) return;
+        pos = c * NB;
+    }
+    T *out = (T *)a.samp + ((long long)chunk * a.nslot + slot) * CS_SIGCAP + pos;
+    sfor<NB>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        out[i] = x[i] < T(0) ? -x[i] : x[i];
+    });
 }
 
 /* Occupancy target: f32 profiles 2 waves/SIMD for the largest stacks, 3
@@ -535,9 +566,15 @@ void fast_search_kernel(FastArgs a)
                 constexpr int Hk = NH / k;
                 T xk[2 * Hk];
                 c2r<Hk, k, NH + 1>(X, xk, twp);
-                const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
-                                            C->valid[slot]);
-                emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
+                if (a.samp) {
+                    sample_out<2 * Hk>(xk, active, lj, C->nvalid[slot], chunk,
+                                       slot, a);
+                } else {
+                    const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
+                                                C->valid[slot]);
+                    emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj,
+                         mv, a);
+                }
             }
         }
     });
@@ -564,12 +601,22 @@ void fast_search_kernel(FastArgs a)
                     });
                     xk[j] = s * invk;
                 });
-                const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot), C->valid[slot]);
-                emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj, mv, a);
+                if (a.samp) {
+                    sample_out<nk>(xk, active, lj, C->nvalid[slot], chunk, slot, a);
+                } else {
+                    const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
+                                            C->valid[slot]);
+                    emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj,
+                         mv, a);
+                }
             }
         }
     });
 
+    if (a.samp) {
+        sample_out<2 * NH>(x, active, lj, C->nvalid[0], chunk, 0, a);
+        return;
+    }
     const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0]);
     emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
 }

@@ -104,6 +104,26 @@ def test_f64_every_trial_metric_bit_identical(oracle, nh, m, maxdecim):
     sess.close()
 
 
+@pytest.mark.parametrize("nh,m,blocksize,precision", [
+    (16, 16, 128, "f32"),      # small chunks: every row of every chunk sampled
+    (32, 32, 2048, "f32"),     # large chunks: rows c*nv/ncol
+    (60, 16, 2048, "f32"),
+    (60, 16, 8192, "f64"),
+    (21, 8, 100, "f64")])
+def test_measured_sigma_every_trial_metric_bit_identical(oracle, nh, m, blocksize, precision):
+    """--sigma measured: per chunk and decimation, 1.4826 * median |profile|
+    over the sampled rows (exact order statistic), then the same boxcar."""
+    spec, p, o, sess = _case(oracle, nh, m, nbins=1 << 14, blocksize=blocksize,
+                             precision=precision, sigma="measured")
+    say("measured sigma nh", nh, "m", m, "blocksize", blocksize, precision, "chunks", o.nchunks)
+    got = sess.run(-np.inf)
+    want = o.search(-np.inf, 1)
+    assert got.size == want.size
+    for f in ("r", "nharm", "metric"):
+        assert np.array_equal(got[f], want[f]), f
+    sess.close()
+
+
 def test_config1_candidate_list_identical(oracle):
     """BASELINE config 1 (2^20-bin .fft, pulsar at r0 = 12345.37, 60 harmonics,
     S/N 15), blocksize 2048, threshold 8: raw lists identical in order and
