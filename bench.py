@@ -160,6 +160,11 @@ def serving(nreq, slots, rate=20.0):
         "p99_latency_ms": float(np.percentile(lat, 99)),
         "max_latency_ms": float(lat.max()),
         "mean_queue_ms": float(np.mean([r["queue_ms"] for r in recs])),
+        "mean_plan_ms": float(np.mean([r["plan_ms"] for r in recs])),
+        "mean_stage_ms": float(np.mean([r["stage_ms"] for r in recs])),
+        "mean_search_ms": float(np.mean([r["search_ms"] for r in recs])),
+        "mean_kernel_ms": float(np.mean([r["kernel_ms"] for r in recs])),
+        "mean_post_ms": float(np.mean([1e3 * r["latency_s"] - r["latency_ms"] for r in recs])),
         "harmonic_trials": int(ht), "wall_s": wall, "throughput_ht_s": ht / wall,
         "peak_concurrent_searches": st["peak_running"],
         "pulsar_requests_detected": "%d/%d" % (sum(_found(fin[i], jobs[i]["period"])

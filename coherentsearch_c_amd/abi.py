@@ -83,7 +83,8 @@ class CsSearchInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("pad", C.c_int), ("harmonic_trials", C.c_longlong),
                 ("trials", C.c_longlong), ("footprint", C.c_longlong),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
-                ("kernel_ms", C.c_double)]
+                ("kernel_ms", C.c_double), ("plan_ms", C.c_double), ("stage_ms", C.c_double),
+                ("search_ms", C.c_double)]
 
 
 class CsEngineStats(C.Structure):
@@ -103,7 +104,8 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_dump_stack", "cs_gpu_session_last_kernel_ms",
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
                "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
-               "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_engine_create",
+               "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_gpu_session_set_graphs",
+               "cs_engine_create",
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
                "cs_engine_stats", "cs_engine_free", "cs_host_alloc", "cs_host_free"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
@@ -211,6 +213,8 @@ def _declare(L):
     L.cs_gpu_stage_run_host.restype = I
     L.cs_gpu_session_stage.argtypes = [P, P, P, P]
     L.cs_gpu_session_stage.restype = I
+    L.cs_gpu_session_set_graphs.argtypes = [P, I]
+    L.cs_gpu_session_set_graphs.restype = I
     L.cs_engine_create.argtypes = [I, LL, I]
     L.cs_engine_create.restype = P
     L.cs_engine_submit.argtypes = [P, Ps(CsSearchRequest)]

@@ -37,7 +37,7 @@
 #include "cs_stage.h"
 
 static_assert(sizeof(CsSearchRequest) == 72, "CsSearchRequest layout");
-static_assert(sizeof(CsSearchInfo) == 64, "CsSearchInfo layout");
+static_assert(sizeof(CsSearchInfo) == 88, "CsSearchInfo layout");
 static_assert(sizeof(CsEngineStats) == 56, "CsEngineStats layout");
 
 namespace {
@@ -126,6 +126,8 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     GpuSession *S = NULL;
     CsStage *g = NULL;
     int rc = -1;
+    const clk::time_point t0 = clk::now();
+    clk::time_point t1, t2;
     if (csx_plans_build(&set, p, r_lo) != 0) goto done;
     if (csx_chunk_starts(&set, lodr, ft.namps, N, total, Nprof, &cstarts, &nchunks) != 0)
         goto done;
@@ -134,8 +136,13 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     if (!S) goto done;
     g = cs_stage_create_on(N, dt, 6, 100, 6.0, E->device, st);
     if (!g) goto done;
+    t1 = clk::now();
+    r->info.plan_ms = ms_between(t0, t1);
     if (cs_gpu_session_stage(S, g, r->in.samples, r->in.amps_out) != 0) goto done;
+    t2 = clk::now();
+    r->info.stage_ms = ms_between(t1, t2);
     r->out = cs_gpu_session_search_resident(S, r->in.threshold);
+    r->info.search_ms = ms_between(t2, clk::now());
     r->info.kernel_ms = cs_gpu_session_last_kernel_ms(S);
     {
         long long ht = 0, tr = 0;

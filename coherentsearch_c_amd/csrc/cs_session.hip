@@ -182,6 +182,14 @@ struct GpuSession {
     csk::FastArgs args;
     double last_ms;
     bool f64;                     /* --precision f64: profiles and metrics in double */
+    /* the search's launch sequence captured once and replayed while the
+     * threshold and the record buffer stay the same */
+    bool use_graph;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    double g_thr;
+    CandRec *g_recs;
+    unsigned int g_cap;
     bool measured;                /* --sigma measured: sample pass + per-chunk sigma */
     std::vector<double> h_shp;    /* shape_w per (chunk, slot, width candidate) */
     double *d_shp;
@@ -284,6 +292,8 @@ static void session_release(GpuSession *S)
     dfree(S, S->d_stack);
     dfree(S, S->d_shp);
     dfree(S, S->d_samp);
+    if (S->gexec) hipGraphExecDestroy(S->gexec);
+    if (S->graph) hipGraphDestroy(S->graph);
     if (S->ev0) hipEventDestroy(S->ev0);
     if (S->ev1) hipEventDestroy(S->ev1);
     if (S->own_stream && S->st) hipStreamDestroy(S->st);
@@ -442,6 +452,12 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->async_alloc = false;
     S->own_stream = false;
     S->f64 = p->precision == CS_PREC_F64;
+    S->use_graph = ext == NULL;
+    S->graph = NULL;
+    S->gexec = NULL;
+    S->g_recs = NULL;
+    S->g_cap = 0;
+    S->g_thr = 0.0;
     S->measured = p->sigma == CS_SIGMA_MEASURED;
     S->d_shp = NULL;
     S->d_samp = NULL;
@@ -654,7 +670,7 @@ extern "C" CS_API int cs_gpu_session_matches(const GpuSession *S, const FFTFile 
 
 __global__ void cs_zero_u32(unsigned int *p) { p[0] = 0u; }
 
-static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
+static int enqueue_search(GpuSession *S, double threshold, float2 *stack_out)
 {
     S->args.stack_out = stack_out;
     S->args.threshold = threshold;
@@ -665,7 +681,6 @@ static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
     fast_launch_fn fn = S->f64 ? S->fast->fn64 : S->fast->fn;
     hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
     if (hipGetLastError() != hipSuccess) return -1;
-    hipEventRecord(S->ev0, S->st);
     if (S->measured && ntiles > 0) {
         /* sample pass, then sigma and the per-width factors of every chunk */
         S->args.samp = S->d_samp;
@@ -692,6 +707,55 @@ static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
         if (hipGetLastError() != hipSuccess) return -1;
     }
     if (ntiles > 0 && fn(&S->args, nblk, S->st) != hipSuccess) return -1;
+    return 0;
+}
+
+/* One search: the counter reset, the measured-sigma passes and the search
+ * kernel, between the timing events.  Without a stack dump, sessions on their
+ * own stream replay a captured graph of the sequence (kernels only). */
+static int launch_search(GpuSession *S, double threshold, float2 *stack_out)
+{
+    hipEventRecord(S->ev0, S->st);
+    if (!S->use_graph || stack_out) {
+        if (enqueue_search(S, threshold, stack_out) != 0) return -1;
+        hipEventRecord(S->ev1, S->st);
+        return 0;
+    }
+    /* the graph holds the record buffer and its capacity by value: a regrown
+     * buffer (possibly at the same address) needs a new capture */
+    if (!S->gexec || S->g_thr != threshold || S->g_recs != S->d_recs || S->g_cap != S->cap) {
+        if (S->gexec) hipGraphExecDestroy(S->gexec);
+        if (S->graph) hipGraphDestroy(S->graph);
+        S->gexec = NULL;
+        S->graph = NULL;
+        hipError_t e = hipStreamBeginCapture(S->st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            fprintf(stderr, "Error: search graph capture: %s\n", hipGetErrorString(e));
+            return -1;
+        }
+        const int rc = enqueue_search(S, threshold, NULL);
+        e = hipStreamEndCapture(S->st, &S->graph);
+        if (e != hipSuccess || rc != 0) {
+            fprintf(stderr, "Error: search graph capture (%d): %s\n", rc, hipGetErrorString(e));
+            if (e == hipSuccess && S->graph) hipGraphDestroy(S->graph);
+            S->graph = NULL;
+            return -1;
+        }
+        e = hipGraphInstantiate(&S->gexec, S->graph, NULL, NULL, 0);
+        if (e != hipSuccess) {
+            fprintf(stderr, "Error: search graph instantiate: %s\n", hipGetErrorString(e));
+            S->gexec = NULL;
+            return -1;
+        }
+        S->g_thr = threshold;
+        S->g_recs = S->d_recs;
+        S->g_cap = S->cap;
+    }
+    const hipError_t el = hipGraphLaunch(S->gexec, S->st);
+    if (el != hipSuccess) {
+        fprintf(stderr, "Error: search graph launch: %s\n", hipGetErrorString(el));
+        return -1;
+    }
     hipEventRecord(S->ev1, S->st);
     return 0;
 }
@@ -794,6 +858,14 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
         const long long c = (long long)(r.key >> 40);
         const int slot = (int)((r.key >> 32) & 0xff);
         const long long j = (long long)(r.key & 0xffffffffull);
+        if (c >= S->nchunks || slot >= S->nslot || j >= (long long)S->chunks[(size_t)c].n) {
+            fprintf(stderr, "Error: GPU search returned an invalid record (chunk %lld, "
+                            "slot %d, trial %lld)\n", c, slot, j);
+            free(out.v);
+            out.v = NULL;
+            out.n = out.cap = 0;
+            return out;
+        }
         const double metric = (double)r.metric;
         if (!(metric > threshold)) continue;
         const double rstart = S->r_lo + (double)S->cstarts[(size_t)c] * S->lodr;
@@ -819,6 +891,14 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
 }
 
 extern "C" CS_API void cs_gpu_session_free(GpuSession *S) { session_release(S); }
+
+extern "C" CS_API int cs_gpu_session_set_graphs(GpuSession *S, int enable)
+{
+    if (!S) return -1;
+    if (!S->own_stream) return enable ? -1 : 0;
+    S->use_graph = enable != 0;
+    return 0;
+}
 
 extern "C" CS_API CandidateList cs_gpu_search_region(const FFTFile *ft, const SearchParams *p,
                                                      const DirectPlanSet *dplans,

@@ -209,6 +209,11 @@ class Session:
             self.spec.ft.DC = float(self.spec.amps[0])
             self.spec.ft.Nyquist = float(self.spec.amps[1])
 
+    def set_graphs(self, enable):
+        """replay the search from a captured hipGraph (default) or launch eagerly"""
+        if self.L.cs_gpu_session_set_graphs(self.h, int(bool(enable))) != 0:
+            raise RuntimeError("graph mode not available for this session")
+
     def search_resident(self, threshold):
         return take_list(self.L.cs_gpu_session_search_resident(self.h, threshold))
 

@@ -88,6 +88,13 @@ CS_API int cs_gpu_session_upload(GpuSession *s, const FFTFile *ft);
  * the same candidate list without the host-to-device copy. */
 CS_API CandidateList cs_gpu_session_search_resident(GpuSession *s, double threshold);
 
+/* Replay the search's launch sequence (counter reset, measured-sigma passes,
+ * search kernel) from a captured hipGraph (enable = 1, the default for
+ * sessions with their own stream) or launch it eagerly (0).  The graph is
+ * re-captured when the threshold or the record buffer changes; results are
+ * identical either way.  0 on success. */
+CS_API int cs_gpu_session_set_graphs(GpuSession *s, int enable);
+
 /* Device time of the last cs_gpu_session_run's search kernel(s), in ms,
  * measured with HIP events on the session's stream. */
 CS_API double cs_gpu_session_last_kernel_ms(const GpuSession *s);
@@ -158,6 +165,9 @@ typedef struct {
     double run_ms;          /* admitted -> done */
     double latency_ms;      /* submit -> done */
     double kernel_ms;       /* search kernel, HIP events */
+    double plan_ms;         /* host: plans, chunk starts, session and stage tables */
+    double stage_ms;        /* upload + device realfft/rednoise (+ spectrum copy back) */
+    double search_ms;       /* search launch to candidate list on the host */
 } CsSearchInfo;
 
 typedef struct {

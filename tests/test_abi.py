@@ -50,7 +50,8 @@ def test_struct_layouts():
     assert abi.Candidate.ducy.offset == 32
     assert C.sizeof(abi.CsSearchRequest) == 72 and abi.CsSearchRequest.amps_out.offset == 64
     assert abi.CsSearchRequest.lobin.offset == 48 and abi.CsSearchRequest.threshold.offset == 56
-    assert C.sizeof(abi.CsSearchInfo) == 64 and abi.CsSearchInfo.kernel_ms.offset == 56
+    assert C.sizeof(abi.CsSearchInfo) == 88 and abi.CsSearchInfo.kernel_ms.offset == 56
+    assert abi.CsSearchInfo.search_ms.offset == 80
     assert C.sizeof(abi.CsEngineStats) == 56 and abi.CsEngineStats.completed.offset == 48
 
 

@@ -124,6 +124,31 @@ def test_measured_sigma_every_trial_metric_bit_identical(oracle, nh, m, blocksiz
     sess.close()
 
 
+@pytest.mark.parametrize("sigma", ["analytic", "measured"])
+def test_graph_replays_match_eager(oracle, sigma):
+    """The captured search graph, replayed several times with eager launches,
+    host copies and synchronisations in between, returns the eager result
+    every time (the graph holds kernels only: the counter reset is a fill
+    kernel)."""
+    spec, p, o, sess = _case(oracle, 32, 16, nbins=1 << 14, sigma=sigma)
+    sess.set_graphs(False)
+    eager = sess.run(5.0)
+    sess.set_graphs(True)
+    for i in range(4):
+        got = sess.search_resident(5.0)
+        say("graph replay", i, sigma, got.size)
+        for f in ("r", "nharm", "metric"):
+            assert np.array_equal(got[f], eager[f]), (i, f)
+        i0, n, _ = o.chunk(i)
+        sess.dump_stack(i0, min(n, 64))            # an eager launch in between
+        sess.upload()                               # and a host-to-device copy
+    # a threshold change re-captures
+    lo = sess.search_resident(4.0)
+    sess.set_graphs(False)
+    assert np.array_equal(lo["metric"], sess.run(4.0)["metric"])
+    sess.close()
+
+
 def test_config1_candidate_list_identical(oracle):
     """BASELINE config 1 (2^20-bin .fft, pulsar at r0 = 12345.37, 60 harmonics,
     S/N 15), blocksize 2048, threshold 8: raw lists identical in order and
