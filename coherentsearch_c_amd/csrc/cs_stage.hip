@@ -38,7 +38,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLmax = 512;        /* longest sub-FFT of one level */
-constexpr int kCL = 4096;         /* columns x length held in LDS (64 KB of double2) */
+constexpr int kCL = 2048;         /* columns x length held in LDS (32 KB of double2) */
 constexpr int kMaxLevels = 4;
 constexpr int kMaxStages = 10;
 
@@ -78,6 +78,24 @@ __device__ __forceinline__ void dft_r(double2 (&v)[R], const double2 *__restrict
         v[2] = csub(t0, t2);
         v[1] = make_double2(t1.x + t3.y, t1.y - t3.x);   /* t1 - i t3 */
         v[3] = make_double2(t1.x - t3.y, t1.y + t3.x);   /* t1 + i t3 */
+    } else if constexpr (R == 8) {
+        /* two 4-point DFTs of the even and odd inputs, combined with
+         * w8^k = exp(-2 pi i k/8) */
+        double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        dft_r<4>(e, tw, tstep);
+        dft_r<4>(o, tw, tstep);
+        const double h = 0.70710678118654752440;
+        const double2 o1 = make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+        const double2 o2 = make_double2(o[2].y, -o[2].x);
+        const double2 o3 = make_double2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], o1);
+        v[5] = csub(e[1], o1);
+        v[2] = cadd(e[2], o2);
+        v[6] = csub(e[2], o2);
+        v[3] = cadd(e[3], o3);
+        v[7] = csub(e[3], o3);
     } else {
         double2 y[R];
 #pragma unroll
@@ -140,6 +158,28 @@ __device__ __forceinline__ double2 phase(long long m, long long n)
     return make_double2(c, s);
 }
 
+/* stages of one level in LDS; POW2 levels instantiate the radix-8/4/2
+ * butterflies only (fewer registers, higher occupancy) */
+template <bool POW2>
+__device__ __forceinline__ void level_stages(double2 *lds, const LevelArgs &a,
+                                             const double2 *__restrict__ tw)
+{
+    int Ns = 1;
+    for (int s = 0; s < a.nrad; s++) {
+        const int R = a.rad[s];
+        if (R == 8) stage<8>(lds, a.L, a.C, Ns, tw);
+        else if (R == 4) stage<4>(lds, a.L, a.C, Ns, tw);
+        else if (R == 2) stage<2>(lds, a.L, a.C, Ns, tw);
+        else if constexpr (!POW2) {
+            if (R == 3) stage<3>(lds, a.L, a.C, Ns, tw);
+            else if (R == 5) stage<5>(lds, a.L, a.C, Ns, tw);
+            else stage<7>(lds, a.L, a.C, Ns, tw);
+        }
+        Ns *= R;
+    }
+}
+
+template <bool POW2>
 __global__ __launch_bounds__(kThreads) void fft_level_kernel(const double2 *__restrict__ in,
                                                              double2 *__restrict__ out,
                                                              const double2 *__restrict__ twall,
@@ -161,17 +201,7 @@ __global__ __launch_bounds__(kThreads) void fft_level_kernel(const double2 *__re
             lds[ai * C + c] = i < a.S ? in[base + ai * a.S + i] : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        int Ns = 1;
-        for (int s = 0; s < a.nrad; s++) {
-            const int R = a.rad[s];
-            if (R == 8) stage<8>(lds, L, C, Ns, tw);
-            else if (R == 4) stage<4>(lds, L, C, Ns, tw);
-            else if (R == 2) stage<2>(lds, L, C, Ns, tw);
-            else if (R == 3) stage<3>(lds, L, C, Ns, tw);
-            else if (R == 5) stage<5>(lds, L, C, Ns, tw);
-            else stage<7>(lds, L, C, Ns, tw);
-            Ns *= R;
-        }
+        level_stages<POW2>(lds, a, tw);
         const long long LS = (long long)L * a.S;
         for (int e = tid; e < L * C; e += kThreads) {
             const int k = e / C, c = e % C;
@@ -202,17 +232,7 @@ __global__ __launch_bounds__(kThreads) void fft_level_kernel(const double2 *__re
             lds[ai * C + c] = b >= 0 ? in[b * L + ai] : make_double2(0.0, 0.0);
         }
         __syncthreads();
-        int Ns = 1;
-        for (int s = 0; s < a.nrad; s++) {
-            const int R = a.rad[s];
-            if (R == 8) stage<8>(lds, L, C, Ns, tw);
-            else if (R == 4) stage<4>(lds, L, C, Ns, tw);
-            else if (R == 2) stage<2>(lds, L, C, Ns, tw);
-            else if (R == 3) stage<3>(lds, L, C, Ns, tw);
-            else if (R == 5) stage<5>(lds, L, C, Ns, tw);
-            else stage<7>(lds, L, C, Ns, tw);
-            Ns *= R;
-        }
+        level_stages<POW2>(lds, a, tw);
         for (int e = tid; e < L * C; e += kThreads) {
             const int k = e / C, c = e % C;
             if (r0 + c < a.B) out[r0 + c + a.B * k] = lds[k * C + c];
@@ -629,7 +649,13 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
         double2 *src = g->d_z0, *dst = g->d_z1;
         for (int l = 0; l < g->nlev; l++) {
             const int last = l == g->nlev - 1;
-            fft_level_kernel<<<(unsigned)g->grid[l], kThreads, 0, st>>>(src, dst, g->d_tw, g->lev[l], last);
+            const int L = g->lev[l].L;
+            if ((L & (L - 1)) == 0)
+                fft_level_kernel<true><<<(unsigned)g->grid[l], kThreads, 0, st>>>(
+                    src, dst, g->d_tw, g->lev[l], last);
+            else
+                fft_level_kernel<false><<<(unsigned)g->grid[l], kThreads, 0, st>>>(
+                    src, dst, g->d_tw, g->lev[l], last);
             SCHK(hipGetLastError());
             std::swap(src, dst);
         }
