@@ -830,6 +830,7 @@ void orc_c2r_f32(int H, const float *X, float *x)
 /* IDFT_N of N interleaved complex values (the project definition above). */
 void orc_idft_f64(int N, const double *in, double *out)
 {
+    if (N <= 0) return;
     double *ir = malloc(sizeof(double) * N), *ii = malloc(sizeof(double) * N);
     double *or_ = malloc(sizeof(double) * N), *oi = malloc(sizeof(double) * N);
     long *io = calloc((size_t)N, sizeof(long));
@@ -840,6 +841,7 @@ void orc_idft_f64(int N, const double *in, double *out)
 }
 void orc_idft_f32(int N, const float *in, float *out)
 {
+    if (N <= 0) return;
     float *ir = malloc(sizeof(float) * N), *ii = malloc(sizeof(float) * N);
     float *or_ = malloc(sizeof(float) * N), *oi = malloc(sizeof(float) * N);
     long *io = calloc((size_t)N, sizeof(long));
@@ -1614,13 +1616,33 @@ static void dft_direct_d(int H, const double *Xr, const double *Xi, int hs,
     }
 }
 
-int orc_chunk_metrics_fp64(const OrcCtx *c, long long ci, double *out)
+/* FP64 metrics of rows [j0, j0+nr) of chunk ci (NaN elsewhere).  Measured
+ * sigma still sees the whole chunk: the rows its sample reads are marked by a
+ * dry run of the sampler and computed in the first pass. */
+static int chunk_metrics_fp64_rows(const OrcCtx *c, long long ci, int j0, int nr,
+                                   double *out)
 {
     long long i0; int n; double rstart;
     if (orc_ctx_chunk(c, ci, &i0, &n, &rstart) != 0) return -1;
+    if (j0 < 0 || nr < 0 || j0 + nr > n) return -1;
     const int nh = c->cfg.nharms;
     uint8_t *f = malloc((size_t)nh);
     chunk_filled(c, i0, n, f);
+    uint8_t *need = calloc((size_t)n, 1);
+    for (int j = j0; j < j0 + nr; j++) need[j] = 1;
+    if (c->cfg.sigma == 1) {
+        double *dry = malloc(sizeof(double) * ORC_SIGCAP);
+        for (int s = 0; s <= c->ndec; s++) {
+            const int k = slot_k(c, s), nb = slot_nb(c, s);
+            int nv = s == 0 ? n : nvalid_of(c, k, rstart, n);
+            int ns;
+#define GETMARK(j, i) (need[(j)] = 1, 0.0)
+            if ((long long)nb * nv > 0) SIGMA_SAMPLE(double, GETMARK, nb, nv, ORC_SIGCAP, dry, ns);
+#undef GETMARK
+            (void)ns;
+        }
+        free(dry);
+    }
     double *Xr = calloc((size_t)nh + 1, sizeof(double));
     double *Xi = calloc((size_t)nh + 1, sizeof(double));
     double *x = malloc(sizeof(double) * 2 * (nh + 1));
@@ -1628,7 +1650,7 @@ int orc_chunk_metrics_fp64(const OrcCtx *c, long long ci, double *out)
     /* measured sigma needs the whole chunk's double profiles */
     double **profs = NULL;
     if (c->cfg.sigma == 1) {
-        profs = calloc((size_t)c->ndec + 1, sizeof(double *));
+        profs = calloc(c->ndec >= 0 ? (size_t)c->ndec + 1 : 1, sizeof(double *));
         for (int s = 0; s <= c->ndec; s++)
             profs[s] = malloc(sizeof(double) * (size_t)slot_nb(c, s) * n);
     }
@@ -1656,6 +1678,11 @@ int orc_chunk_metrics_fp64(const OrcCtx *c, long long ci, double *out)
             }
         }
         for (int j = 0; j < n; j++) {
+            if (!need[j]) {
+                for (int s = 0; s <= c->ndec; s++) mv[(size_t)s * n + j] = NAN;
+                continue;
+            }
+            if (pass == 1 && (j < j0 || j >= j0 + nr)) continue;
             long long t = i0 + j;
             for (int h = 1; h <= nh; h++) {
                 Xr[h] = Xi[h] = 0.0;
@@ -1683,13 +1710,28 @@ int orc_chunk_metrics_fp64(const OrcCtx *c, long long ci, double *out)
             }
         }
     }
+    for (int s = 0; s <= c->ndec; s++)
+        for (int j = 0; j < n; j++)
+            if (j < j0 || j >= j0 + nr) mv[(size_t)s * n + j] = NAN;
     memcpy(out, mv, sizeof(double) * (size_t)(c->ndec + 1) * n);
     if (profs) {
         for (int s = 0; s <= c->ndec; s++) free(profs[s]);
         free(profs);
     }
-    free(mv); free(f); free(Xr); free(Xi); free(x);
+    free(mv); free(f); free(Xr); free(Xi); free(x); free(need);
     return n;
+}
+
+int orc_chunk_metrics_fp64(const OrcCtx *c, long long ci, double *out)
+{
+    long long i0; int n; double rstart;
+    if (orc_ctx_chunk(c, ci, &i0, &n, &rstart) != 0) return -1;
+    return chunk_metrics_fp64_rows(c, ci, 0, n, out);
+}
+
+int orc_chunk_metrics_fp64_rows(const OrcCtx *c, long long ci, int j0, int nr, double *out)
+{
+    return chunk_metrics_fp64_rows(c, ci, j0, nr, out);
 }
 
 /* ------------------------------------------------------------------ */

@@ -116,6 +116,9 @@ int  orc_chunk_metrics(const OrcCtx *c, long long ci, double *out);
 /* FP64 oracle metrics of chunk ci (direct sinc interpolation, direct real
  * DFT, double boxcar with exact analytic sigma and shapes). */
 int  orc_chunk_metrics_fp64(const OrcCtx *c, long long ci, double *out);
+/* the same for rows [j0, j0+nr) of the chunk only (NaN elsewhere); measured
+ * sigma is still taken over the whole chunk's sample */
+int  orc_chunk_metrics_fp64_rows(const OrcCtx *c, long long ci, int j0, int nr, double *out);
 
 /* Full raw search (candidates above threshold, reference CPU order for
  * nthreads round-robin chunks, before collapsing).  chunk_lo/chunk_hi

@@ -69,6 +69,8 @@ def lib():
         L.orc_chunk_profiles.argtypes = [P, LL, I, P]; L.orc_chunk_profiles.restype = I
         L.orc_chunk_metrics.argtypes = [P, LL, P]; L.orc_chunk_metrics.restype = I
         L.orc_chunk_metrics_fp64.argtypes = [P, LL, P]; L.orc_chunk_metrics_fp64.restype = I
+        L.orc_chunk_metrics_fp64_rows.argtypes = [P, LL, I, I, P]
+        L.orc_chunk_metrics_fp64_rows.restype = I
         L.orc_search.argtypes = [P, D, I, LL, LL, C.POINTER(C.POINTER(OrcCand))]
         L.orc_search.restype = LL
         L.orc_c2r_f32.argtypes = [I, P, P]
@@ -223,6 +225,14 @@ class Oracle:
         i0, n, _ = self.chunk(ci)
         out = np.zeros((len(self.ks), n), dtype=np.float64)
         self.L.orc_chunk_metrics_fp64(self.h, ci, ptr(out))
+        return out
+
+    def metrics_fp64_rows(self, ci, j0, nr):
+        """FP64 metrics of rows [j0, j0+nr) of chunk ci, NaN at other rows."""
+        i0, n, _ = self.chunk(ci)
+        out = np.zeros((len(self.ks), n), dtype=np.float64)
+        if self.L.orc_chunk_metrics_fp64_rows(self.h, ci, j0, nr, ptr(out)) < 0:
+            raise ValueError("row window outside the chunk")
         return out
 
     def search(self, threshold, nthreads=1, chunk_lo=0, chunk_hi=-1):
