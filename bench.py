@@ -86,9 +86,10 @@ def _same_lists(a, b):
 
 def dm_sweep(nfiles, slots, host_files=3):
     """The batched DM sweep: the batched/served form (engine, `slots`
-    streams, every file submitted at once, host post-processing overlapped)
-    against one-file-at-a-time forms on the same series: with device staging
-    (a one-slot engine, each file finished before the next starts) and the
+    streams, every file submitted at once, each request finishing on the
+    device: collapse, duty cycles from the resident spectrum) against
+    one-file-at-a-time forms on the same series: with device staging (a
+    one-slot engine, each file finished before the next starts) and the
     reference-style flow (host realfft + rednoise, search_file, duty cycles;
     timed on the first `host_files` files, whose harmonic-trials come from the
     same searches)."""
@@ -139,7 +140,8 @@ def dm_sweep(nfiles, slots, host_files=3):
 
 def serving(nreq, slots, rate=20.0):
     """The served stream: Poisson arrivals at `rate`/s, continuous batching
-    on the engine; latency = arrival -> collapsed list with duty cycles."""
+    on the engine; latency = arrival -> collapsed list with duty cycles (each
+    request finishes on the device)."""
     from coherentsearch_c_amd import engine as eng
     t = time.perf_counter()
     jobs, arrivals = eng.serving_jobs(nreq, rate)
@@ -164,7 +166,9 @@ def serving(nreq, slots, rate=20.0):
         "mean_stage_ms": float(np.mean([r["stage_ms"] for r in recs])),
         "mean_search_ms": float(np.mean([r["search_ms"] for r in recs])),
         "mean_kernel_ms": float(np.mean([r["kernel_ms"] for r in recs])),
-        "mean_post_ms": float(np.mean([1e3 * r["latency_s"] - r["latency_ms"] for r in recs])),
+        "mean_post_ms": float(np.mean([r["post_ms"] for r in recs])),
+        "mean_host_return_ms": float(np.mean([1e3 * r["latency_s"] - r["latency_ms"]
+                                              for r in recs])),
         "harmonic_trials": int(ht), "wall_s": wall, "throughput_ht_s": ht / wall,
         "peak_concurrent_searches": st["peak_running"],
         "pulsar_requests_detected": "%d/%d" % (sum(_found(fin[i], jobs[i]["period"])

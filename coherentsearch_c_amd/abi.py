@@ -76,7 +76,9 @@ class CsSearchRequest(C.Structure):
     _fields_ = [("samples", C.c_void_p), ("N", C.c_longlong), ("dt", C.c_double),
                 ("p", C.POINTER(SearchParams)), ("lofreq", C.c_double), ("hifreq", C.c_double),
                 ("lobin", C.c_int), ("blocksize", C.c_int), ("threshold", C.c_double),
-                ("amps_out", C.c_void_p)]
+                ("amps_out", C.c_void_p), ("ncands", C.c_int), ("numharm", C.c_int),
+                ("remove", C.c_int), ("harm_remove", C.c_int), ("dr_tol", C.c_double),
+                ("harm_tol", C.c_double)]
 
 
 class CsSearchInfo(C.Structure):
@@ -84,7 +86,7 @@ class CsSearchInfo(C.Structure):
                 ("trials", C.c_longlong), ("footprint", C.c_longlong),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("plan_ms", C.c_double), ("stage_ms", C.c_double),
-                ("search_ms", C.c_double)]
+                ("search_ms", C.c_double), ("post_ms", C.c_double)]
 
 
 class CsEngineStats(C.Structure):
@@ -105,7 +107,7 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
                "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
                "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_gpu_session_set_graphs",
-               "cs_engine_create",
+               "cs_gpu_session_finalize", "cs_engine_create",
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
                "cs_engine_stats", "cs_engine_free", "cs_host_alloc", "cs_host_free"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
@@ -215,6 +217,8 @@ def _declare(L):
     L.cs_gpu_session_stage.restype = I
     L.cs_gpu_session_set_graphs.argtypes = [P, I]
     L.cs_gpu_session_set_graphs.restype = I
+    L.cs_gpu_session_finalize.argtypes = [P, Ps(CandidateList), I]
+    L.cs_gpu_session_finalize.restype = CandidateList
     L.cs_engine_create.argtypes = [I, LL, I]
     L.cs_engine_create.restype = P
     L.cs_engine_submit.argtypes = [P, Ps(CsSearchRequest)]

@@ -32,12 +32,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "cands.h"
 #include "core.h"
 #include "cs_gpu.h"
 #include "cs_stage.h"
 
-static_assert(sizeof(CsSearchRequest) == 72, "CsSearchRequest layout");
-static_assert(sizeof(CsSearchInfo) == 88, "CsSearchInfo layout");
+static_assert(sizeof(CsSearchRequest) == 104, "CsSearchRequest layout");
+static_assert(sizeof(CsSearchInfo) == 96, "CsSearchInfo layout");
 static_assert(sizeof(CsEngineStats) == 56, "CsEngineStats layout");
 
 namespace {
@@ -127,7 +128,7 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     CsStage *g = NULL;
     int rc = -1;
     const clk::time_point t0 = clk::now();
-    clk::time_point t1, t2;
+    clk::time_point t1, t2, t3;
     if (csx_plans_build(&set, p, r_lo) != 0) goto done;
     if (csx_chunk_starts(&set, lodr, ft.namps, N, total, Nprof, &cstarts, &nchunks) != 0)
         goto done;
@@ -142,8 +143,30 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     t2 = clk::now();
     r->info.stage_ms = ms_between(t1, t2);
     r->out = cs_gpu_session_search_resident(S, r->in.threshold);
-    r->info.search_ms = ms_between(t2, clk::now());
+    t3 = clk::now();
+    r->info.search_ms = ms_between(t2, t3);
     r->info.kernel_ms = cs_gpu_session_last_kernel_ms(S);
+    if (r->in.ncands > 0) {
+        /* src/search.c:1467-1478, then src/cli.c:1059-1061 on the device */
+        CandidateList c = r->out;
+        if (r->in.remove) {
+            CandidateList kept = csx_remove_duplicates(&c, r->in.dr_tol);
+            csx_candlist_free(&c);
+            c = kept;
+        } else {
+            csx_candlist_sort_freq(&c);
+        }
+        if (r->in.harm_remove) {
+            CandidateList kept = csx_remove_harmonics(&c, r->in.numharm, r->in.harm_tol);
+            csx_candlist_free(&c);
+            c = kept;
+        }
+        const size_t nin = c.n;
+        r->out = cs_session_finalize(S, &c, r->in.ncands);
+        csx_candlist_free(&c);
+        if (nin > 0 && r->out.n == 0) goto done;
+        r->info.post_ms = ms_between(t3, clk::now());
+    }
     {
         long long ht = 0, tr = 0;
         double ab = 0.0;

@@ -22,6 +22,7 @@
 #include <mutex>
 #include <vector>
 
+#include "cands.h"
 #include "core.h"
 #include "cs_device.h"
 #include "cs_gpu.h"
@@ -960,4 +961,237 @@ extern "C" CS_API int cs_gpu_session_work(const GpuSession *S, long long *ht,
     if (trials) *trials = S->total;
     if (alg_bytes) *alg_bytes = S->alg_bytes;
     return 0;
+}
+
+/* ---- duty cycles of the reported candidates, on the device ----------------
+ * src/cli.c:1059-1061 (sort by S/N, truncate) and src/candidate.c:305-329
+ * (measure_ducy) with candidate_profile (src/candidate.c:208-271) and
+ * boxcar_best_width (src/boxcar.c:736-785), read from the resident spectrum so
+ * it never has to come back to the host. */
+namespace {
+
+constexpr int kDucyM = 64;        /* exact interpolation taps */
+constexpr int kDucyHmax = 64;     /* harmonics per candidate */
+constexpr int kDucyWcap = 256;    /* boxcar widths per profile */
+
+struct DucyCand {
+    double r;
+    int H;                        /* leading harmonics whose 64-bin window fits */
+    int n;                        /* profile bins, 2H */
+    int nw;                       /* boxcar widths of an n-bin profile */
+    int woff;                     /* their offset in the width/shape tables */
+};
+
+/* One wave per candidate.  Lane h-1 interpolates harmonic h exactly (m = 64
+ * taps of sinc(off) e^{-i pi off}, double); lanes over bins form the profile by
+ * a direct real inverse transform of the H harmonics (harmonic H taken as the
+ * real Nyquist term) scaled by 1/2H; lane 0 takes the prefix sums of the
+ * mean-removed profile over n + wmax wrapped bins; lanes over widths take each
+ * width's best window, (max - duty*total) * shape_w; lane 0 keeps the first
+ * width with the largest value. */
+__global__ __launch_bounds__(64) void ducy_kernel(const float2 *__restrict__ amps,
+                                                  long long namps,
+                                                  const DucyCand *__restrict__ cands,
+                                                  const int *__restrict__ widths,
+                                                  const double *__restrict__ shapes,
+                                                  int *__restrict__ bestw)
+{
+    __shared__ double Xr[kDucyHmax + 1], Xi[kDucyHmax + 1];
+    __shared__ double prof[2 * kDucyHmax];
+    __shared__ double ps[4 * kDucyHmax + 1];
+    __shared__ double cval[kDucyWcap];
+    const DucyCand c = cands[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int H = c.H, n = c.n;
+    if (lane < H) {
+        const int h = lane + 1;
+        const double rh = c.r * (double)h;
+        double dr = fmod(rh, 1.0);
+        if (dr < 0) dr += 1.0;
+        const long long first = (long long)floor(rh + 1e-15) + 1 - kDucyM / 2;
+        double sre = 0.0, sim = 0.0;
+        for (int j = 0; j < kDucyM; j++) {
+            const double off = dr - (double)(-kDucyM / 2 + 1 + j);
+            const double sinc = off == 0.0 ? 1.0 : sin(M_PI * off) / (M_PI * off);
+            const double cr = sinc * cos(M_PI * off), ci = -sinc * sin(M_PI * off);
+            const long long idx = first + j;
+            double bre = 0.0, bim = 0.0;
+            if (idx >= 0 && idx < namps) {
+                const float2 a = amps[idx];
+                bre = (double)a.x;
+                bim = (double)a.y;
+            }
+            sre += cr * bre - ci * bim;
+            sim += cr * bim + ci * bre;
+        }
+        Xr[h] = sre;
+        Xi[h] = sim;
+    }
+    __syncthreads();
+    const double inv = 1.0 / (double)n;
+    for (int i = lane; i < n; i += 64) {
+        double acc = 0.0;
+        for (int h = 1; h < H; h++) {
+            double sn, cs;
+            sincospi(2.0 * (double)(((long long)h * i) % n) / (double)n, &sn, &cs);
+            acc += 2.0 * (Xr[h] * cs - Xi[h] * sn);
+        }
+        acc += (i & 1) ? -Xr[H] : Xr[H];
+        prof[i] = acc * inv;
+    }
+    __syncthreads();
+    const int wmax = widths[c.woff + c.nw - 1];
+    if (lane == 0) {
+        double sum = 0.0;
+        for (int i = 0; i < n; i++) sum += prof[i];
+        const double mean = sum / (double)n;
+        ps[0] = 0.0;
+        for (int i = 1; i <= n + wmax; i++) {
+            const int idx = i > n ? i - n : i;
+            ps[i] = ps[i - 1] + (prof[idx - 1] - mean);
+        }
+    }
+    __syncthreads();
+    const double stot = ps[n] - ps[0];
+    for (int k = lane; k < c.nw; k += 64) {
+        const int w = widths[c.woff + k];
+        const double duty = (double)w / (double)n;
+        double mx = ps[w] - ps[0];
+        for (int q = 2; q <= n; q++) {
+            const double d = ps[q + w - 1] - ps[q - 1];
+            if (d > mx) mx = d;
+        }
+        cval[k] = (mx - duty * stot) * shapes[c.woff + k];
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double best = -INFINITY;
+        int bw = widths[c.woff];
+        for (int k = 0; k < c.nw; k++)
+            if (cval[k] > best) {
+                best = cval[k];
+                bw = widths[c.woff + k];
+            }
+        bestw[blockIdx.x] = bw;
+    }
+}
+
+}  // namespace
+
+__attribute__((visibility("hidden"))) CandidateList
+cs_session_finalize(GpuSession *S, const CandidateList *in, int ncands)
+{
+    CandidateList out;
+    csx_candlist_init(&out);
+    if (!S || !in) return out;
+    for (size_t i = 0; i < in->n; i++) csx_candlist_push(&out, in->v[i]);
+    if (out.n != in->n) {
+        fprintf(stderr, "Error: out of host memory for %zu candidates\n", in->n);
+        csx_candlist_free(&out);
+        return out;
+    }
+    csx_candlist_sort_metric_desc(&out);
+    if (ncands >= 0 && out.n > (size_t)ncands) out.n = (size_t)ncands;
+    const double fsp = S->p->boxcar_fsp, maxfrac = S->p->boxcar_maxfrac;
+    const long long Nhalf = S->N / 2;
+    std::vector<DucyCand> dc;
+    std::vector<size_t> dc_of;           /* list index of each device candidate */
+    std::vector<int> wt;
+    std::vector<double> st;
+    std::vector<int> woff_of_n(2 * kDucyHmax + 1, -1), nw_of_n(2 * kDucyHmax + 1, 0);
+    for (size_t i = 0; i < out.n; i++) {
+        Candidate *c = &out.v[i];
+        c->ducy = NAN;
+        if (!(c->nharm >= 1 && c->r > 0.0)) continue;
+        if (c->nharm > kDucyHmax) {
+            fprintf(stderr, "Error: duty cycle of a %d-harmonic candidate (limit %d)\n",
+                    c->nharm, kDucyHmax);
+            csx_candlist_free(&out);
+            return out;
+        }
+        int H = 0;
+        for (int h = 1; h <= c->nharm; h++) {
+            const double rh = c->r * (double)h;
+            const long long rint0 = (long long)floor(rh + 1e-15) + 1;
+            if (!(rint0 - kDucyM / 2 >= 0 && rint0 + kDucyM / 2 <= S->namps &&
+                  rh < (double)Nhalf))
+                break;
+            H++;
+        }
+        const int n = H < 1 ? 2 : 2 * H;
+        if (woff_of_n[n] < 0) {
+            int w[kDucyWcap];
+            const int nw = csx_boxcar_widths(n, fsp, maxfrac, w, kDucyWcap);
+            woff_of_n[n] = (int)wt.size();
+            nw_of_n[n] = nw;
+            if (nw > 0) {
+                std::vector<uint8_t> filled((size_t)n / 2, 1);
+                std::vector<double> sh((size_t)nw);
+                csx_boxcar_shape(sh.data(), n, w, nw, filled.data(), 1, n / 2);
+                wt.insert(wt.end(), w, w + nw);
+                st.insert(st.end(), sh.begin(), sh.end());
+            }
+        }
+        if (nw_of_n[n] <= 0) {
+            c->ducy = 1.0 / (double)n;   /* no widths: one bin */
+            continue;
+        }
+        if (H < 1) {
+            /* an all-zero profile: every width scores 0, the first one wins */
+            c->ducy = (double)wt[(size_t)woff_of_n[n]] / (double)n;
+            continue;
+        }
+        DucyCand d;
+        d.r = c->r;
+        d.H = H;
+        d.n = n;
+        d.nw = nw_of_n[n];
+        d.woff = woff_of_n[n];
+        dc.push_back(d);
+        dc_of.push_back(i);
+    }
+    if (dc.empty()) return out;
+    if (hipSetDevice(S->device) != hipSuccess) {
+        csx_candlist_free(&out);
+        return out;
+    }
+    const size_t bc = dc.size() * sizeof(DucyCand), bw = wt.size() * sizeof(int),
+                 bs = st.size() * sizeof(double), bo = dc.size() * sizeof(int);
+    const size_t o_w = (bc + 255) / 256 * 256, o_s = o_w + (bw + 255) / 256 * 256,
+                 o_o = o_s + (bs + 255) / 256 * 256, total = o_o + bo;
+    std::vector<unsigned char> h_in(o_o);
+    memcpy(h_in.data(), dc.data(), bc);
+    memcpy(h_in.data() + o_w, wt.data(), bw);
+    memcpy(h_in.data() + o_s, st.data(), bs);
+    std::vector<int> best(dc.size(), 0);
+    unsigned char *d = NULL;
+    int ok = 0;
+    if (dalloc(S, (void **)&d, total) == hipSuccess) {
+        if (hipMemcpyAsync(d, h_in.data(), o_o, hipMemcpyHostToDevice, S->st) == hipSuccess) {
+            hipLaunchKernelGGL(ducy_kernel, dim3((unsigned)dc.size()), dim3(64), 0, S->st,
+                               (const float2 *)S->d_amps, S->namps, (const DucyCand *)d,
+                               (const int *)(d + o_w), (const double *)(d + o_s),
+                               (int *)(d + o_o));
+            if (hipGetLastError() == hipSuccess &&
+                hipMemcpyAsync(best.data(), d + o_o, bo, hipMemcpyDeviceToHost, S->st) ==
+                    hipSuccess &&
+                hipStreamSynchronize(S->st) == hipSuccess)
+                ok = 1;
+        }
+        dfree(S, d);
+    }
+    if (!ok) {
+        fprintf(stderr, "Error: device duty-cycle pass failed\n");
+        csx_candlist_free(&out);
+        return out;
+    }
+    for (size_t k = 0; k < dc.size(); k++)
+        out.v[dc_of[k]].ducy = (double)best[k] / (double)dc[k].n;
+    return out;
+}
+
+extern "C" CS_API CandidateList cs_gpu_session_finalize(GpuSession *S, const CandidateList *cands,
+                                                        int ncands)
+{
+    return cs_session_finalize(S, cands, ncands);
 }

@@ -29,4 +29,10 @@ cs_session_create_on(const FFTFile *ft, const SearchParams *p, const DirectPlanS
                      const long long *cstarts, long long total, int Nprof, long long nchunks,
                      double r_lo, double lodr, int device, hipStream_t ext);
 
+/* sort by S/N, keep the first ncands (all when negative) and measure their
+ * duty cycles on the device from the session's resident spectrum; a new list,
+ * empty with a message on stderr on failure */
+__attribute__((visibility("hidden"))) CandidateList
+cs_session_finalize(GpuSession *s, const CandidateList *in, int ncands);
+
 #endif

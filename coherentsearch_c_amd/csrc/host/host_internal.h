@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "cands.h"
 #include "core.h"
 #include "cs_host.h"
 

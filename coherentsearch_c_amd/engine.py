@@ -8,8 +8,9 @@ queued requests by HBM footprint, device staging of each raw series
   run_plain    the one-file-at-a-time GPU form the reference runs
                (src/cli.c:590-700): host realfft + rednoise, search_file
                (session per file, search, collapse), duty cycles, in turn;
-  run_batched  every series submitted at once; results collapsed and finished
-               on host threads as they complete, overlapping the GPU work;
+  run_batched  every series submitted at once; each request collapses its
+               list and measures duty cycles on the device from the resident
+               spectrum (or on host threads from a copied-back spectrum);
   serve        requests submitted at their arrival times (a served stream):
                whatever arrives while others run is admitted as slots and HBM
                free up; per-request latency is arrival -> final list ready.
@@ -61,11 +62,17 @@ class Engine:
         self._lock = threading.Lock()
 
     def submit(self, samples, dt, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
-               threshold=8.0, spectrum=True, out=None):
+               threshold=8.0, spectrum=None, out=None, ncands=0, numharm=16, dr_tol=1.0,
+               harm_tol=1.0, remove=True, harm_remove=True):
         """Queue one search of a raw float32 series; returns its ticket.
-        With `spectrum`, the staged spectrum comes back for host
-        post-processing (duty cycles), into `out` (N float32, e.g. pinned)
-        when given."""
+
+        ncands > 0: the request returns the final list (collapsed, sorted by
+        S/N, cut to ncands, duty cycles measured on the device) and the
+        spectrum stays on the device.  ncands == 0: the raw list.
+        With `spectrum` (default: only for raw lists) the staged spectrum
+        comes back too, into `out` (N float32, e.g. pinned) when given."""
+        if spectrum is None:
+            spectrum = ncands <= 0
         s = samples.a if isinstance(samples, PinnedSeries) else samples
         s = np.ascontiguousarray(s, dtype=np.float32)
         if s.size % 2:
@@ -85,6 +92,9 @@ class Engine:
         req.lofreq, req.hifreq, req.lobin = float(lofreq), float(hifreq), int(lobin)
         req.blocksize, req.threshold = int(blocksize), float(threshold)
         req.amps_out = amps.ctypes.data if spectrum else None
+        req.ncands, req.numharm = int(ncands), int(numharm)
+        req.remove, req.harm_remove = int(bool(remove)), int(bool(harm_remove))
+        req.dr_tol, req.harm_tol = float(dr_tol), float(harm_tol)
         # the ticket is recorded before any waiter can see it finish
         with self._lock:
             t = self.L.cs_engine_submit(self.h, C.byref(req))
@@ -128,11 +138,13 @@ class Engine:
         self.close()
 
 
-def _submit_job(engine, j):
+def _submit_job(engine, j, ncands=0):
+    """ncands > 0: the request finishes on the device (final list with duty
+    cycles); 0: the raw list and the staged spectrum come back."""
     return engine.submit(j["samples"], j["dt"], j["params"], lofreq=j.get("lofreq", 0.1),
                          hifreq=j.get("hifreq", 125.0), lobin=j.get("lobin", 100),
                          blocksize=j.get("blocksize", 0), threshold=j.get("threshold", 8.0),
-                         out=j.get("out"))
+                         out=j.get("out") if ncands <= 0 else None, ncands=ncands)
 
 
 def sweep_jobs(nfiles=64, seed=4, pinned=True, threshold=8.0, lo=22, hi=25):
@@ -185,17 +197,18 @@ def serving_jobs(n=500, rate=20.0, seed=5, lo=20, hi=25, pool=8, threshold=8.0):
     return jobs, np.array([m["arrival"] for m in meta])
 
 
-def run_plain_device(engine, jobs, ncands=100):
+def run_plain_device(engine, jobs, ncands=100, device_post=True):
     """One file at a time through a one-slot engine: device staging, search,
-    collapse and duty cycles of each file before the next is submitted.
-    Returns (final lists, infos, wall seconds)."""
+    collapse and duty cycles of each file before the next is submitted
+    (duty cycles on the device, or on the host from the copied-back spectrum
+    with device_post=False).  Returns (final lists, infos, wall seconds)."""
     finals, infos = [], []
     t0 = time.perf_counter()
     for j in jobs:
-        raw, info, spec = engine.wait(_submit_job(engine, j))
+        lst, info, spec = engine.wait(_submit_job(engine, j, ncands if device_post else 0))
         if info["status"] != 0:
             raise RuntimeError("request failed")
-        finals.append(_post(spec, j["params"], raw, ncands))
+        finals.append(lst if device_post else _post(spec, j["params"], lst, ncands))
         infos.append(info)
     return finals, infos, time.perf_counter() - t0
 
@@ -221,35 +234,42 @@ def run_plain(jobs, device=0, ncands=100):
     return out, time.perf_counter() - t0
 
 
-def run_batched(engine, jobs, post_workers=8, ncands=100):
-    """Every job submitted at once; each result collapsed and finished on a
-    host thread as soon as its search completes.  Returns (final lists in job
-    order, infos, wall seconds)."""
+def run_batched(engine, jobs, post_workers=8, ncands=100, device_post=True):
+    """Every job submitted at once.  With device_post each request finishes
+    on the device (collapse, then duty cycles from the resident spectrum);
+    otherwise each raw result is collapsed and finished on a host thread as
+    soon as its search completes.  Returns (final lists in job order, infos,
+    wall seconds)."""
     t0 = time.perf_counter()
     tickets = {}
     for i, j in enumerate(jobs):
-        tickets[_submit_job(engine, j)] = i
+        tickets[_submit_job(engine, j, ncands if device_post else 0)] = i
     finals = [None] * len(jobs)
     infos = [None] * len(jobs)
     futs = {}
     with cf.ThreadPoolExecutor(post_workers) as pool:
         for _ in range(len(jobs)):
             t = engine.next_done()
-            raw, info, spec = engine.wait(t)
+            lst, info, spec = engine.wait(t)
             i = tickets[t]
             infos[i] = info
             if info["status"] != 0:
                 raise RuntimeError("request %d failed" % i)
-            futs[pool.submit(_post, spec, jobs[i]["params"], raw, ncands)] = i
+            if device_post:
+                finals[i] = lst
+            else:
+                futs[pool.submit(_post, spec, jobs[i]["params"], lst, ncands)] = i
         for f in cf.as_completed(futs):
             finals[futs[f]] = f.result()
     return finals, infos, time.perf_counter() - t0
 
 
-def serve(engine, jobs, arrivals, post_workers=8, ncands=100):
+def serve(engine, jobs, arrivals, post_workers=8, ncands=100, device_post=True):
     """Submit job i at arrivals[i] seconds after the start; returns (final
     lists, per-request dicts with latency_s = arrival -> final list ready,
-    wall seconds)."""
+    wall seconds).  With device_post requests finish on the device (final
+    list with duty cycles); otherwise on host threads from the copied-back
+    spectrum."""
     n = len(jobs)
     order = np.argsort(arrivals, kind="stable")
     start = time.perf_counter()
@@ -264,7 +284,7 @@ def serve(engine, jobs, arrivals, post_workers=8, ncands=100):
             j = jobs[i]
             with lock:
                 t_arr[i] = start + float(arrivals[i])
-                tickets[_submit_job(engine, j)] = i
+                tickets[_submit_job(engine, j, ncands if device_post else 0)] = i
 
     th = threading.Thread(target=feeder, daemon=True)
     th.start()
@@ -276,22 +296,28 @@ def serve(engine, jobs, arrivals, post_workers=8, ncands=100):
         done = time.perf_counter()
         return i, f, done, info
 
+    def record(i, fin, done, info):
+        finals[i] = fin
+        info = dict(info)
+        info["latency_s"] = done - t_arr[i]
+        recs[i] = info
+
     futs = []
     with cf.ThreadPoolExecutor(post_workers) as pool:
         for _ in range(n):
             t = engine.next_done()
-            raw, info, spec = engine.wait(t)
+            lst, info, spec = engine.wait(t)
+            done = time.perf_counter()
             with lock:
                 i = tickets[t]
             if info["status"] != 0:
                 raise RuntimeError("request %d failed" % i)
-            futs.append(pool.submit(post, i, spec, raw, info))
+            if device_post:
+                record(i, lst, done, info)
+            else:
+                futs.append(pool.submit(post, i, spec, lst, info))
         for f in cf.as_completed(futs):
-            i, fin, done, info = f.result()
-            finals[i] = fin
-            info = dict(info)
-            info["latency_s"] = done - t_arr[i]
-            recs[i] = info
+            record(*f.result())
     th.join()
     return finals, recs, time.perf_counter() - start
 

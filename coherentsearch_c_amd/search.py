@@ -217,6 +217,18 @@ class Session:
     def search_resident(self, threshold):
         return take_list(self.L.cs_gpu_session_search_resident(self.h, threshold))
 
+    def finalize(self, cands, ncands=100):
+        """cli.c:1059-1061 on the device: sort by S/N, keep ncands, duty cycles
+        from the resident spectrum (cs_gpu_session_finalize)."""
+        cl = to_list(cands)
+        out = self.L.cs_gpu_session_finalize(self.h, C.byref(cl), int(ncands))
+        n_in = int(cl.n)
+        self.L.candlist_free(C.byref(cl))
+        res = take_list(out)
+        if n_in and ncands != 0 and res.size == 0:
+            raise RuntimeError("device duty-cycle pass failed (see stderr)")
+        return res
+
     def dump_stack(self, t0, n):
         out = np.zeros((n, self.params.nharms + 1, 2), dtype=np.float32)
         rc = self.L.cs_gpu_session_dump_stack(self.h, C.byref(self.spec.ft), t0, n,

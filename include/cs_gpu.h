@@ -95,6 +95,17 @@ CS_API CandidateList cs_gpu_session_search_resident(GpuSession *s, double thresh
  * identical either way.  0 on success. */
 CS_API int cs_gpu_session_set_graphs(GpuSession *s, int enable);
 
+/* The final candidate list of a search, from the session's resident spectrum:
+ * cands (a collapsed list, e.g. search_file's) sorted by S/N (stable), cut to
+ * the first ncands (all when ncands < 0), with duty cycles measured on the
+ * device -- the steps src/cli.c:1059-1061 run on the host (sort, truncate,
+ * measure_ducy of src/candidate.c:305-329: exact 64-tap interpolation of each
+ * harmonic that fits, a 2H-bin profile, boxcar_best_width).  The spectrum does
+ * not come back to the host.  cands is left as it is; the result is a new list
+ * (empty, with a message on stderr, on failure or for nharm > 64). */
+CS_API CandidateList cs_gpu_session_finalize(GpuSession *s, const CandidateList *cands,
+                                             int ncands);
+
 /* Device time of the last cs_gpu_session_run's search kernel(s), in ms,
  * measured with HIP events on the session's stream. */
 CS_API double cs_gpu_session_last_kernel_ms(const GpuSession *s);
@@ -142,7 +153,8 @@ CS_API int cs_gpu_session_stage(GpuSession *s, CsStage *g, const float *samples,
  * running ones (a request larger than the budget runs alone).  Each request
  * stages its raw series on the device (cs_gpu_stage_*) and searches it as
  * search_file would (src/search.c:1247-1406), returning the raw candidate
- * list of cs_gpu_session_run. */
+ * list of cs_gpu_session_run, or the final list with duty cycles when the
+ * request asks for it (the spectrum then never leaves the device). */
 typedef struct CsEngine CsEngine;
 
 typedef struct {
@@ -155,6 +167,15 @@ typedef struct {
     int blocksize;          /* 0: CS_GPU_DEFAULT_BLOCKSIZE */
     double threshold;
     float *amps_out;        /* optional host buffer of N floats: the staged spectrum */
+    /* ncands > 0: the final list instead of the raw one -- collapsed as
+     * search_file does (remove_duplicates within dr_tol, or a stable frequency
+     * sort when remove is 0; then remove_harmonics up to numharm within
+     * harm_tol when harm_remove), then cs_gpu_session_finalize (sort by S/N,
+     * first ncands, duty cycles on the device) */
+    int ncands;
+    int numharm;
+    int remove, harm_remove;
+    double dr_tol, harm_tol;
 } CsSearchRequest;
 
 typedef struct {
@@ -168,6 +189,7 @@ typedef struct {
     double plan_ms;         /* host: plans, chunk starts, session and stage tables */
     double stage_ms;        /* upload + device realfft/rednoise (+ spectrum copy back) */
     double search_ms;       /* search launch to candidate list on the host */
+    double post_ms;         /* collapse and device duty cycles (ncands > 0) */
 } CsSearchInfo;
 
 typedef struct {

@@ -48,10 +48,12 @@ def test_struct_layouts():
     assert abi.SearchParams.decimations.offset == 60 and abi.SearchParams.sigma.offset == 320
     assert abi.DirectPlan.row.offset == 48 and abi.DirectPlan.gA_im.offset == 136
     assert abi.Candidate.ducy.offset == 32
-    assert C.sizeof(abi.CsSearchRequest) == 72 and abi.CsSearchRequest.amps_out.offset == 64
+    assert C.sizeof(abi.CsSearchRequest) == 104 and abi.CsSearchRequest.amps_out.offset == 64
     assert abi.CsSearchRequest.lobin.offset == 48 and abi.CsSearchRequest.threshold.offset == 56
-    assert C.sizeof(abi.CsSearchInfo) == 88 and abi.CsSearchInfo.kernel_ms.offset == 56
-    assert abi.CsSearchInfo.search_ms.offset == 80
+    assert abi.CsSearchRequest.ncands.offset == 72 and abi.CsSearchRequest.harm_remove.offset == 84
+    assert abi.CsSearchRequest.dr_tol.offset == 88 and abi.CsSearchRequest.harm_tol.offset == 96
+    assert C.sizeof(abi.CsSearchInfo) == 96 and abi.CsSearchInfo.kernel_ms.offset == 56
+    assert abi.CsSearchInfo.search_ms.offset == 80 and abi.CsSearchInfo.post_ms.offset == 88
     assert C.sizeof(abi.CsEngineStats) == 56 and abi.CsEngineStats.completed.offset == 48
 
 

@@ -111,3 +111,61 @@ def test_serve_small_stream():
     lat = [r["latency_s"] for r in recs]
     say("latencies", ["%.3f" % v for v in lat], "wall %.2f" % wall)
     assert all(f is not None for f in finals) and all(v > 0 for v in lat)
+
+
+def _final_vs_host(spec, p, final, raw, ncands):
+    """The device-finalised list against the host's finalize of the same
+    collapsed list and spectrum (sort by S/N, truncate, measure_ducy), field by
+    field; duty cycles are w/n of the chosen boxcar width, so they compare
+    exactly."""
+    want = cs.finalize(spec, p, cs.collapse(raw), ncands)
+    assert final.size == want.size
+    for f in ("r", "nharm", "metric", "freq"):
+        assert np.array_equal(final[f], want[f]), f
+    assert np.array_equal(final["ducy"], want["ducy"], equal_nan=True)
+    return want
+
+
+@pytest.mark.parametrize("nh,r0,nbins,hifreq", [(60, 271.3, 1 << 15, 125.0),
+                                                 (16, 3333.7, 1 << 14, 2000.0),
+                                                 (64, 1021.9, 1 << 15, 300.0),
+                                                 (8, 16340.2, 1 << 14, 7800.0)])
+def test_session_finalize_duty_cycles(oracle, nh, r0, nbins, hifreq):
+    """cs_gpu_session_finalize: the S/N order and cut of the host's finalize
+    and its duty cycles, measured on the device from the resident spectrum.
+    In the middle cases only the first harmonics' 64-bin windows fit below
+    the top of the spectrum (profiles of fewer harmonics than reported); in
+    the last the pulsar sits at the top of the band, where no window fits."""
+    amps = synth.spectrum_with_pulsar(nbins, r0, nh, 0.05, 14.0, 3)
+    spec = cs.Spectrum(amps, 2 * nbins, synth.DT)
+    p = cs.make_params(nharms=nh, m=16, maxdecim=6)
+    sess = cs.Session(spec, p, blocksize=4096, hifreq=hifreq)
+    raw = sess.run(4.0)
+    col = cs.collapse(raw)
+    for ncands in (100, 7):
+        final = sess.finalize(col, ncands)
+        say("finalize nh", nh, "r0", r0, "collapsed", col.size, "final", final.size,
+            "ducy", final["ducy"][:4])
+        _final_vs_host(spec, p, final, raw, ncands)
+        # the oracle's duty cycle (direct transform of the same harmonics)
+        od = oracle.measure_ducy(amps.view(np.float32), spec.N, final.astype(oracle.CAND_DTYPE))
+        assert np.array_equal(od["ducy"], final["ducy"], equal_nan=True)
+    assert sess.finalize(col[:0], 100).size == 0
+    sess.close()
+
+
+def test_engine_final_lists_match_host_post():
+    """Requests that ask for the final list (ncands > 0) return the host's
+    collapse + finalize of the same request's raw list and spectrum."""
+    jobs = _jobs(4, seed=15)
+    with eng.Engine(nslots=4) as E:
+        raws = [E.submit(j["samples"], j["dt"], j["params"], threshold=j["threshold"],
+                         blocksize=j["blocksize"]) for j in jobs]
+        fins = [E.submit(j["samples"], j["dt"], j["params"], threshold=j["threshold"],
+                         blocksize=j["blocksize"], ncands=50) for j in jobs]
+        for tr, tf, j in zip(raws, fins, jobs):
+            raw, info_r, spec = E.wait(tr)
+            final, info_f, none = E.wait(tf)
+            assert info_r["status"] == 0 and info_f["status"] == 0 and none is None
+            say("final list", final.size, "post_ms %.3f" % info_f["post_ms"])
+            _final_vs_host(spec, j["params"], final, raw, 50)
