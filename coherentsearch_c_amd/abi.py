@@ -61,6 +61,14 @@ class CandidateList(C.Structure):
     _fields_ = [("v", C.POINTER(Candidate)), ("n", C.c_size_t), ("cap", C.c_size_t)]
 
 
+class ZapBird(C.Structure):
+    _fields_ = [("freq", C.c_double), ("width", C.c_double), ("bary", C.c_int)]
+
+
+class ZapList(C.Structure):
+    _fields_ = [("v", C.POINTER(ZapBird)), ("n", C.c_int)]
+
+
 class SearchCache(C.Structure):
     _fields_ = [("Nprof", C.c_int), ("plan_valid", C.c_int),
                 ("params", C.POINTER(SearchParams)), ("plan_T", C.c_double),
@@ -78,7 +86,8 @@ class CsSearchRequest(C.Structure):
                 ("lobin", C.c_int), ("blocksize", C.c_int), ("threshold", C.c_double),
                 ("amps_out", C.c_void_p), ("ncands", C.c_int), ("numharm", C.c_int),
                 ("remove", C.c_int), ("harm_remove", C.c_int), ("dr_tol", C.c_double),
-                ("harm_tol", C.c_double)]
+                ("harm_tol", C.c_double), ("zaplist", C.POINTER(ZapList)),
+                ("baryv", C.c_double)]
 
 
 class CsSearchInfo(C.Structure):
@@ -107,7 +116,7 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
                "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
                "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_gpu_session_set_graphs",
-               "cs_gpu_session_finalize", "cs_engine_create",
+               "cs_gpu_session_finalize", "cs_gpu_session_zap", "cs_engine_create",
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
                "cs_engine_stats", "cs_engine_free", "cs_host_alloc", "cs_host_free"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
@@ -121,7 +130,8 @@ HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "candlist_sort_freq_stable", "candlist_sort_r_stable", "remove_duplicates",
                 "remove_harmonics", "harmonically_related", "candidate_profile",
                 "measure_ducy", "write_candidates", "cs_format_double",
-                "read_candidates_file", "inf_read", "inf_free", "fftfile_open",
+                "read_candidates_file", "inf_read", "inf_free", "zaplist_read",
+                "zaplist_free", "zaplist_apply", "fftfile_open",
                 "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_redfft",
                 "datpipe_redfft_mem", "cs_open_input", "cs_write_inf", "cs_write_floats",
                 "search_file", "search_cache_free", "cs_main", "cs_main_list"]
@@ -164,6 +174,12 @@ def _declare(L):
     L.write_candidates.argtypes = [Ps(CandidateList), C.c_char_p, D]
     L.write_candidates.restype = I
     L.fftfile_open.argtypes = [Ps(FFTFile), C.c_char_p]
+    L.zaplist_read.argtypes = [C.c_char_p, Ps(ZapList)]
+    L.zaplist_read.restype = I
+    L.zaplist_free.argtypes = [Ps(ZapList)]
+    L.zaplist_free.restype = None
+    L.zaplist_apply.argtypes = [Ps(ZapList), Ps(FFTFile), D]
+    L.zaplist_apply.restype = I
     L.fftfile_open.restype = I
     L.fftfile_close.argtypes = [Ps(FFTFile)]
     L.datpipe_create.argtypes = [I, I, D]
@@ -217,6 +233,8 @@ def _declare(L):
     L.cs_gpu_session_stage.restype = I
     L.cs_gpu_session_set_graphs.argtypes = [P, I]
     L.cs_gpu_session_set_graphs.restype = I
+    L.cs_gpu_session_zap.argtypes = [P, Ps(ZapList), D, P]
+    L.cs_gpu_session_zap.restype = I
     L.cs_gpu_session_finalize.argtypes = [P, Ps(CandidateList), I]
     L.cs_gpu_session_finalize.restype = CandidateList
     L.cs_engine_create.argtypes = [I, LL, I]

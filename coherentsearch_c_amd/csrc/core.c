@@ -417,3 +417,86 @@ int csx_nslots(const SearchParams *p)
     for (int i = 0; i < p->ndecim; i++) n += p->decimations[i] > 1;
     return n;
 }
+
+/* ------------------------------------------------------------------ */
+/* Zaplist intervals (src/zap.c:94-187)                                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    double lo, hi;
+} ZapIv;
+
+/* the median window of count bins from start, clipped to [0, namps) */
+static void zap_window(long long start, int count, long long namps, long long *s, int *n)
+{
+    if (start < 0) {
+        count += (int)start;
+        start = 0;
+    }
+    if (start + count > namps) count = (int)(namps - start);
+    *s = start;
+    *n = count;
+}
+
+int csx_zap_ops(const ZapList *z, double T, long long N, long long namps, double baryv,
+                CsZapOp **out)
+{
+    *out = NULL;
+    if (!z || z->n <= 0) return 0;
+    ZapIv *iv = (ZapIv *)malloc((size_t)z->n * sizeof(ZapIv));
+    CsZapOp *ops = (CsZapOp *)malloc((size_t)z->n * sizeof(CsZapOp));
+    if (!iv || !ops) {
+        free(iv);
+        free(ops);
+        return -1;
+    }
+    for (int i = 0; i < z->n; i++) {
+        const double scale = z->v[i].bary ? 1.0 : 1.0 + baryv;
+        double lo = (z->v[i].freq - 0.5 * z->v[i].width) * T * scale;
+        double hi = (z->v[i].freq + 0.5 * z->v[i].width) * T * scale;
+        if (hi - lo < 1.0) {
+            const double mid = 0.5 * (hi + lo);
+            lo = mid - 0.5;
+            hi = mid + 0.5;
+        }
+        iv[i].lo = lo;
+        iv[i].hi = hi;
+    }
+    /* insertion sort by the low edge: stable, and lists are short */
+    for (int i = 1; i < z->n; i++) {
+        const ZapIv x = iv[i];
+        int j = i - 1;
+        while (j >= 0 && iv[j].lo > x.lo) {
+            iv[j + 1] = iv[j];
+            j--;
+        }
+        iv[j + 1] = x;
+    }
+    long long lim = N / 2;
+    if (lim > namps) lim = namps;
+    int n = 0;
+    for (int i = 0; i < z->n; i++) {
+        const double lo = iv[i].lo;
+        double hi = iv[i].hi;
+        if (lo >= (double)lim) break;
+        if (hi >= (double)lim) hi = (double)(lim - 1);
+        long long ilo = (long long)floor(lo), ihi = (long long)ceil(hi);
+        if (ilo < 1) ilo = 1;
+        if (ihi > namps) ihi = namps;
+        if (ihi - ilo <= 0) continue;
+        if (!(lo - 1.5 * CS_ZAP_MEDIANBINS > 1.0)) continue;
+        CsZapOp *o = &ops[n++];
+        o->ilo = ilo;
+        o->ihi = ihi;
+        zap_window((long long)(lo - 3 * CS_ZAP_MEDIANBINS / 2), CS_ZAP_MEDIANBINS, namps,
+                   &o->s_lo, &o->n_lo);
+        zap_window((long long)(hi - CS_ZAP_MEDIANBINS / 2), CS_ZAP_MEDIANBINS, namps,
+                   &o->s_hi, &o->n_hi);
+    }
+    free(iv);
+    if (n == 0) {
+        free(ops);
+        return 0;
+    }
+    *out = ops;
+    return n;
+}

@@ -53,6 +53,28 @@ void   csx_boxcar_shape(double *shape, int nbins, const int *widths, int nw,
 /* Decimation count with k > 1 (slots after the base). */
 int csx_nslots(const SearchParams *p);
 
+/* One zaplist interval to apply: bins [ilo, ihi) become (avgamp, 0), where
+ * avgamp = f32(sqrt(0.5 * f32(med_lo + med_hi) / ln 2)) and med_lo / med_hi
+ * are the lower medians of the powers re^2 + im^2 (float) of n_lo bins from
+ * s_lo and n_hi bins from s_hi (0 when the count is not positive), read after
+ * the earlier intervals have been applied. */
+typedef struct {
+    long long ilo, ihi;
+    long long s_lo, s_hi;
+    int n_lo, n_hi;
+} CsZapOp;
+
+#define CS_ZAP_MEDIANBINS 200
+
+/* The zaplist's intervals for a spectrum of namps bins over T seconds (N
+ * samples), in application order: bins (f -/+ w/2) T scale with scale 1 or
+ * 1 + baryv, widened to one bin, sorted by their low edge (stable), cut at
+ * min(N/2, namps), dropping those that zap nothing or lie within 301 bins of
+ * DC (src/zap.c:94-187).  *out (malloc'd, NULL when none) receives the ops;
+ * returns their count, -1 on allocation failure. */
+int csx_zap_ops(const ZapList *z, double T, long long N, long long namps, double baryv,
+                CsZapOp **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,7 +37,7 @@
 #include "cs_gpu.h"
 #include "cs_stage.h"
 
-static_assert(sizeof(CsSearchRequest) == 104, "CsSearchRequest layout");
+static_assert(sizeof(CsSearchRequest) == 120, "CsSearchRequest layout");
 static_assert(sizeof(CsSearchInfo) == 96, "CsSearchInfo layout");
 static_assert(sizeof(CsEngineStats) == 56, "CsEngineStats layout");
 
@@ -55,6 +55,8 @@ struct EngReq {
     CsSearchRequest in{};
     SearchParams p{};
     long long footprint = 0;
+    std::vector<ZapBird> zap;     /* the request's zaplist, copied at submit */
+    ZapList zl{};
     int state = 0;                /* 0 queued, 1 running, 2 done */
     CandidateList out{};
     CsSearchInfo info{};
@@ -139,7 +141,13 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     if (!g) goto done;
     t1 = clk::now();
     r->info.plan_ms = ms_between(t0, t1);
-    if (cs_gpu_session_stage(S, g, r->in.samples, r->in.amps_out) != 0) goto done;
+    if (r->zl.n > 0) {
+        if (cs_gpu_session_stage(S, g, r->in.samples, NULL) != 0 ||
+            cs_gpu_session_zap(S, &r->zl, r->in.baryv, r->in.amps_out) < 0)
+            goto done;
+    } else if (cs_gpu_session_stage(S, g, r->in.samples, r->in.amps_out) != 0) {
+        goto done;
+    }
     t2 = clk::now();
     r->info.stage_ms = ms_between(t1, t2);
     r->out = cs_gpu_session_search_resident(S, r->in.threshold);
@@ -267,6 +275,12 @@ extern "C" CS_API long long cs_engine_submit(CsEngine *E, const CsSearchRequest 
     q->in = *r;
     q->p = *r->p;
     q->in.p = &q->p;
+    if (r->zaplist && r->zaplist->n > 0 && r->zaplist->v) {
+        q->zap.assign(r->zaplist->v, r->zaplist->v + r->zaplist->n);
+        q->zl.v = q->zap.data();
+        q->zl.n = (int)q->zap.size();
+    }
+    q->in.zaplist = NULL;
     q->footprint = footprint_of(r, &q->p);
     q->t_submit = clk::now();
     {

@@ -1195,3 +1195,91 @@ extern "C" CS_API CandidateList cs_gpu_session_finalize(GpuSession *S, const Can
 {
     return cs_session_finalize(S, cands, ncands);
 }
+
+/* ---- zaplist on the resident spectrum (src/zap.c:94-187) ------------------
+ * The intervals are applied in order by one workgroup: the median windows of
+ * an interval are read after every earlier interval's bins are written. */
+namespace {
+
+__global__ __launch_bounds__(256) void zap_kernel(float2 *amps, const CsZapOp *__restrict__ ops,
+                                                  int nops, double ln2)
+{
+    __shared__ float pw[2][CS_ZAP_MEDIANBINS];
+    __shared__ float med[2];
+    const int tid = threadIdx.x;
+    for (int k = 0; k < nops; k++) {
+        const CsZapOp o = ops[k];
+        if (tid < 2) med[tid] = 0.0f;
+        for (int i = tid; i < CS_ZAP_MEDIANBINS; i += blockDim.x) {
+            if (i < o.n_lo) {
+                const float2 a = amps[o.s_lo + i];
+                pw[0][i] = a.x * a.x + a.y * a.y;
+            }
+            if (i < o.n_hi) {
+                const float2 a = amps[o.s_hi + i];
+                pw[1][i] = a.x * a.x + a.y * a.y;
+            }
+        }
+        __syncthreads();
+        /* lower median by rank: the value whose stable rank is (n-1)/2 */
+        for (int idx = tid; idx < 2 * CS_ZAP_MEDIANBINS; idx += blockDim.x) {
+            const int w = idx >= CS_ZAP_MEDIANBINS;
+            const int i = idx - w * CS_ZAP_MEDIANBINS;
+            const int n = w ? o.n_hi : o.n_lo;
+            if (i < n) {
+                const float x = pw[w][i];
+                int r = 0;
+                for (int j = 0; j < n; j++) {
+                    const float y = pw[w][j];
+                    r += (y < x) || (y == x && j < i);
+                }
+                if (r == (n - 1) / 2) med[w] = x;
+            }
+        }
+        __syncthreads();
+        const float avg = (float)sqrt(0.5 * (double)(med[0] + med[1]) / ln2);
+        for (long long i = o.ilo + tid; i < o.ihi; i += blockDim.x) amps[i] = make_float2(avg, 0.0f);
+        /* the next interval's windows read what this one wrote */
+        __threadfence();
+        __syncthreads();
+        __threadfence();
+    }
+}
+
+}  // namespace
+
+extern "C" CS_API int cs_gpu_session_zap(GpuSession *S, const ZapList *z, double baryv,
+                                         float *amps_host)
+{
+    if (!S || hipSetDevice(S->device) != hipSuccess) return -1;
+    CsZapOp *ops = NULL;
+    const int n = csx_zap_ops(z, S->T, S->N, S->namps, baryv, &ops);
+    if (n < 0) return -1;
+    int rc = 0;
+    if (n > 0) {
+        CsZapOp *d = NULL;
+        const size_t bytes = (size_t)n * sizeof(CsZapOp);
+        rc = -1;
+        if (dalloc(S, (void **)&d, bytes) == hipSuccess) {
+            if (hipMemcpyAsync(d, ops, bytes, hipMemcpyHostToDevice, S->st) == hipSuccess) {
+                hipLaunchKernelGGL(zap_kernel, dim3(1), dim3(256), 0, S->st, S->d_amps,
+                                   (const CsZapOp *)d, n, -log(0.5));
+                if (hipGetLastError() == hipSuccess) rc = 0;
+            }
+            /* ops must outlive the copy */
+            if (hipStreamSynchronize(S->st) != hipSuccess) rc = -1;
+            dfree(S, d);
+        }
+    }
+    free(ops);
+    if (rc == 0 && amps_host &&
+        hipMemcpyAsync(amps_host, S->d_amps, (size_t)S->namps * sizeof(float2),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess)
+        rc = -1;
+    if (hipStreamSynchronize(S->st) != hipSuccess) rc = -1;
+    if (rc != 0) {
+        fprintf(stderr, "Error: zaplist on the device failed\n");
+        return -1;
+    }
+    return n;
+}

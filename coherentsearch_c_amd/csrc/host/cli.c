@@ -22,9 +22,9 @@ typedef struct {
     char **files;
     int nfiles, files_owned;
     double threshold;
-    const char *outfilenm, *outdir;
+    const char *outfilenm, *outdir, *zaplist;
     int nharms, ncands, lobin, m, blocksize, maxdecim, numharm;
-    double lofreq, hifreq, hidr, drtol;
+    double lofreq, hifreq, hidr, drtol, baryv;
     cs_precision precision;
     cs_sigma sigma;
     int remove, harm_remove, verbose, gpu_device;
@@ -46,6 +46,8 @@ static void usage(FILE *f, int list)
 "      --blocksize N   trials per chunk (65536)    --maxdecim N      (6)\n"
 "      --drtol F       (1.0)                       --numharm N       (16)\n"
 "      --noremove      --noharmremove              --gpu [N]         device (0)\n"
+"      --zaplist F     zap the listed birds (Hz, width Hz; 'B' lines barycentric)\n"
+"      --baryv V       v/c applied to non-'B' zaplist lines (0.0)\n"
 "  -v, --verbose       --threads N  --nowisdom  --wisdomfile F  --progressbar\n"
 "      --noprogress    --noplot     --help\n");
 }
@@ -183,7 +185,8 @@ static int run(int argc, char **argv, int list)
 {
     Opts o;
     memset(&o, 0, sizeof(o));
-    o.threshold = 8.0; o.outfilenm = ""; o.outdir = ""; o.nharms = 60; o.ncands = 100;
+    o.threshold = 8.0; o.outfilenm = ""; o.outdir = ""; o.zaplist = ""; o.nharms = 60;
+    o.ncands = 100;
     o.lobin = 100; o.lofreq = 0.1; o.hifreq = 125.0; o.hidr = 0.5; o.m = 16;
     o.precision = CS_PREC_F32; o.sigma = CS_SIGMA_ANALYTIC; o.maxdecim = 6; o.drtol = 1.0;
     o.numharm = 16; o.remove = 1; o.harm_remove = 1;
@@ -223,6 +226,8 @@ static int run(int argc, char **argv, int list)
         else if (!strcmp(name, "--maxdecim")) { NEED(); if (parse_i(val, &o.maxdecim)) BAD("maxdecim"); }
         else if (!strcmp(name, "--drtol")) { NEED(); if (parse_d(val, &o.drtol)) BAD("drtol"); }
         else if (!strcmp(name, "--numharm")) { NEED(); if (parse_i(val, &o.numharm)) BAD("numharm"); }
+        else if (!strcmp(name, "--zaplist")) { NEED(); o.zaplist = val; }
+        else if (!strcmp(name, "--baryv")) { NEED(); if (parse_d(val, &o.baryv)) BAD("baryv"); }
         else if (!strcmp(name, "--threads")) {
             NEED();
             int t = 0;
@@ -247,8 +252,7 @@ static int run(int argc, char **argv, int list)
         } else if (!strcmp(name, "--metricstats") || !strcmp(name, "--normalize") ||
                    !strcmp(name, "--plot") || !strcmp(name, "--plotstem") ||
                    !strcmp(name, "--plotcols") || !strcmp(name, "--plotrows") ||
-                   !strcmp(name, "--plot-candidates") || !strcmp(name, "--zaplist") ||
-                   !strcmp(name, "--baryv") || !strcmp(name, "--prime-wisdom")) {
+                   !strcmp(name, "--plot-candidates") || !strcmp(name, "--prime-wisdom")) {
             fprintf(stderr, "Error: option '%s' is not supported by this GPU build\n", name);
             return 2;
         } else {
@@ -300,6 +304,15 @@ static int run(int argc, char **argv, int list)
     params.sigma = o.sigma;
     params.ndecim = decimation_set(o.nharms, o.maxdecim, params.decimations, 64);
 
+    ZapList zap;
+    memset(&zap, 0, sizeof(zap));
+    if (o.zaplist[0] && zaplist_read(o.zaplist, &zap) != 0) {
+        fprintf(stderr, "Error: cannot read zaplist '%s'\n", o.zaplist);
+        if (o.files_owned)
+            for (int i = 0; i < o.nfiles; i++) free(o.files[i]);
+        free(o.files);
+        return 2;
+    }
     SearchCache cache;
     memset(&cache, 0, sizeof(cache));
     DatPipe *pipe = datpipe_create(6, 100, 6.0);
@@ -311,6 +324,13 @@ static int run(int argc, char **argv, int list)
         if (cs_open_input(path, pipe, &ft, &is_dat) != 0) {
             fprintf(stderr, "Error: failed to open '%s'\n", path);
             status = 1;
+            continue;
+        }
+        /* src/cli.c:1015-1024: the zaplist edits the in-memory spectrum */
+        if (zap.n > 0 && zaplist_apply(&zap, &ft, o.baryv) < 0) {
+            fprintf(stderr, "Error: failed to apply zaplist to '%s'\n", path);
+            status = 1;
+            fftfile_close(&ft);
             continue;
         }
         CandidateList c = search_file(&ft, &params, o.lofreq, o.hifreq, o.lobin, o.blocksize,
@@ -330,6 +350,7 @@ static int run(int argc, char **argv, int list)
     }
     search_cache_free(&cache);
     datpipe_free(pipe);
+    zaplist_free(&zap);
     if (o.files_owned)
         for (int i = 0; i < o.nfiles; i++) free(o.files[i]);
     free(o.files);

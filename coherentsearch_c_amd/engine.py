@@ -23,7 +23,7 @@ import time
 import numpy as np
 
 from . import abi
-from .search import (Spectrum, collapse, finalize, make_params, redfft, search_file,
+from .search import (Spectrum, Zaplist, collapse, finalize, make_params, redfft, search_file,
                      take_list)
 
 
@@ -63,14 +63,16 @@ class Engine:
 
     def submit(self, samples, dt, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
                threshold=8.0, spectrum=None, out=None, ncands=0, numharm=16, dr_tol=1.0,
-               harm_tol=1.0, remove=True, harm_remove=True):
+               harm_tol=1.0, remove=True, harm_remove=True, zaplist=None, baryv=0.0):
         """Queue one search of a raw float32 series; returns its ticket.
 
         ncands > 0: the request returns the final list (collapsed, sorted by
         S/N, cut to ncands, duty cycles measured on the device) and the
         spectrum stays on the device.  ncands == 0: the raw list.
         With `spectrum` (default: only for raw lists) the staged spectrum
-        comes back too, into `out` (N float32, e.g. pinned) when given."""
+        comes back too, into `out` (N float32, e.g. pinned) when given.
+        `zaplist` (Zaplist or (freq, width, bary) tuples) is applied to the
+        staged spectrum before the search."""
         if spectrum is None:
             spectrum = ncands <= 0
         s = samples.a if isinstance(samples, PinnedSeries) else samples
@@ -95,6 +97,10 @@ class Engine:
         req.ncands, req.numharm = int(ncands), int(numharm)
         req.remove, req.harm_remove = int(bool(remove)), int(bool(harm_remove))
         req.dr_tol, req.harm_tol = float(dr_tol), float(harm_tol)
+        if zaplist is not None and len(zaplist):
+            z = zaplist if isinstance(zaplist, Zaplist) else Zaplist(zaplist)
+            req.zaplist = C.pointer(z.zl)     # copied by the engine at submit
+        req.baryv = float(baryv)
         # the ticket is recorded before any waiter can see it finish
         with self._lock:
             t = self.L.cs_engine_submit(self.h, C.byref(req))

@@ -124,6 +124,49 @@ def redfft(samples, dt, path="memory"):
     return Spectrum(amps, s.size, dt, path)
 
 
+class Zaplist:
+    """A zaplist (ZapList) built from (freq Hz, width Hz, bary) tuples or
+    read from a file (zaplist_read); keeps its birds alive for the C calls."""
+
+    def __init__(self, birds=()):
+        birds = list(birds)
+        self.birds = (abi.ZapBird * max(len(birds), 1))()
+        for i, (f, w, b) in enumerate(birds):
+            self.birds[i].freq, self.birds[i].width, self.birds[i].bary = float(f), float(w), int(b)
+        self.zl = abi.ZapList()
+        self.zl.v = C.cast(self.birds, C.POINTER(abi.ZapBird))
+        self.zl.n = len(birds)
+
+    @classmethod
+    def read(cls, path):
+        L = abi.host()
+        z = abi.ZapList()
+        if L.zaplist_read(str(path).encode(), C.byref(z)) != 0:
+            raise OSError("cannot read zaplist '%s'" % path)
+        birds = [(z.v[i].freq, z.v[i].width, z.v[i].bary) for i in range(z.n)]
+        L.zaplist_free(C.byref(z))
+        return cls(birds)
+
+    def tuples(self):
+        return [(self.birds[i].freq, self.birds[i].width, self.birds[i].bary)
+                for i in range(self.zl.n)]
+
+    def __len__(self):
+        return self.zl.n
+
+
+def zap(spec, zaplist, baryv=0.0):
+    """zaplist_apply on the host spectrum in place (zap.c:155-187); returns
+    the number of intervals zapped."""
+    z = zaplist if isinstance(zaplist, Zaplist) else Zaplist(zaplist)
+    n = abi.host().zaplist_apply(C.byref(z.zl), C.byref(spec.ft), float(baryv))
+    if n < 0:
+        raise RuntimeError("zaplist_apply failed")
+    spec.ft.DC = float(spec.amps[0])
+    spec.ft.Nyquist = float(spec.amps[1])
+    return n
+
+
 def take_list(cl):
     """Copy a CandidateList into a numpy array and free it."""
     L = abi.host()
@@ -216,6 +259,17 @@ class Session:
 
     def search_resident(self, threshold):
         return take_list(self.L.cs_gpu_session_search_resident(self.h, threshold))
+
+    def zap(self, zaplist, baryv=0.0, host_copy=True):
+        """Apply a zaplist to the resident spectrum (cs_gpu_session_zap); with
+        host_copy the session's Spectrum receives the zapped amplitudes.
+        Returns the number of intervals zapped."""
+        z = zaplist if isinstance(zaplist, Zaplist) else Zaplist(zaplist)
+        n = self.L.cs_gpu_session_zap(self.h, C.byref(z.zl), float(baryv),
+                                      self.spec.amps.ctypes.data if host_copy else None)
+        if n < 0:
+            raise RuntimeError("device zaplist failed (see stderr)")
+        return n
 
     def finalize(self, cands, ncands=100):
         """cli.c:1059-1061 on the device: sort by S/N, keep ncands, duty cycles

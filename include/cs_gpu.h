@@ -95,6 +95,14 @@ CS_API CandidateList cs_gpu_session_search_resident(GpuSession *s, double thresh
  * identical either way.  0 on success. */
 CS_API int cs_gpu_session_set_graphs(GpuSession *s, int enable);
 
+/* Apply a zaplist to the session's resident spectrum, as zaplist_apply
+ * (src/zap.c:155-187) does to a host spectrum: each interval's bins become
+ * (avgamp, 0), avgamp from the lower medians of the powers of 200 bins on
+ * either side, intervals in order of their low edge.  amps_host (optional,
+ * 2*namps floats) receives the spectrum afterwards.  Returns the number of
+ * intervals applied, -1 on failure. */
+CS_API int cs_gpu_session_zap(GpuSession *s, const ZapList *z, double baryv, float *amps_host);
+
 /* The final candidate list of a search, from the session's resident spectrum:
  * cands (a collapsed list, e.g. search_file's) sorted by S/N (stable), cut to
  * the first ncands (all when ncands < 0), with duty cycles measured on the
@@ -176,6 +184,11 @@ typedef struct {
     int numharm;
     int remove, harm_remove;
     double dr_tol, harm_tol;
+    /* optional zaplist applied to the staged spectrum before the search
+     * (cs_gpu_session_zap; copied at submit), non-'B' lines scaled by
+     * 1 + baryv; amps_out then receives the zapped spectrum */
+    const ZapList *zaplist;
+    double baryv;
 } CsSearchRequest;
 
 typedef struct {

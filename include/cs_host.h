@@ -76,6 +76,9 @@ CS_API CandidateList read_candidates_file(const char *path, double T);
 /* ---- inputs: src/fileio.c, src/datpipe.c ------------------------------ */
 CS_API int  inf_read(const char *path, SimpleInf *inf);
 CS_API void inf_free(SimpleInf *inf);
+CS_API int  zaplist_read(const char *path, ZapList *z);                   /* zap.c:120-147 */
+CS_API void zaplist_free(ZapList *z);                                      /* zap.c:149-153 */
+CS_API int  zaplist_apply(const ZapList *z, FFTFile *ft, double baryv);    /* zap.c:155-187, 94-118 */
 CS_API int  fftfile_open(FFTFile *ft, const char *path);
 CS_API void fftfile_close(FFTFile *ft);
 typedef struct DatPipe DatPipe;

@@ -11,6 +11,7 @@
  *   tunables            :34-43      SimpleInf / FFTFile  :48-76
  *   cs_precision/sigma  :115-116    SearchParams         :118-131
  *   Candidate(List)     :139-151    DirectPlan(Set)      :170-195
+ *   ZapBird / ZapList   :83-93
  */
 #ifndef CS_TYPES_H
 #define CS_TYPES_H
@@ -126,6 +127,19 @@ typedef struct {
     int n;
 } DirectPlanSet;
 
+/* One zaplist line: a frequency and width (Hz); bary = 1 for a 'B' line,
+ * used as given, otherwise scaled by (1 + baryv). */
+typedef struct {
+    double freq;
+    double width;
+    int bary;
+} ZapBird;
+
+typedef struct {
+    ZapBird *v;
+    int n;
+} ZapList;
+
 #ifdef __cplusplus
 }
 #endif
@@ -158,5 +172,7 @@ CS_STATIC_ASSERT(sizeof(Candidate) == 40, "Candidate layout");
 CS_STATIC_ASSERT(offsetof(Candidate, nharm) == 24, "Candidate.nharm");
 CS_STATIC_ASSERT(offsetof(Candidate, ducy) == 32, "Candidate.ducy");
 CS_STATIC_ASSERT(sizeof(CandidateList) == 24, "CandidateList layout");
+CS_STATIC_ASSERT(sizeof(ZapBird) == 24, "ZapBird layout");
+CS_STATIC_ASSERT(sizeof(ZapList) == 16, "ZapList layout");
 
 #endif /* CS_TYPES_H */

@@ -2190,3 +2190,80 @@ int orc_realfft_rednoise(const float *samples, long long N, double dt, float *ou
     free(tmp);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Zaplist: src/zap.c:77-118 (median_power, zap_one), 155-187 (apply) */
+/* ------------------------------------------------------------------ */
+typedef struct { double lo, hi; int idx; } OrcZapBin;
+
+static int orc_zapbin_cmp(const void *a, const void *b)
+{
+    const OrcZapBin *x = a, *y = b;
+    if (x->lo != y->lo) return (x->lo > y->lo) - (x->lo < y->lo);
+    return (x->idx > y->idx) - (x->idx < y->idx);   /* input order among equal edges */
+}
+
+static int orc_float_cmp(const void *a, const void *b)
+{
+    const float x = *(const float *)a, y = *(const float *)b;
+    return (x > y) - (x < y);
+}
+
+/* lower median of the float powers of `count` bins from `start`, the window
+ * clipped to the spectrum (zap.c:77-92) */
+static float orc_zap_median(const float *amps, long long namps, long long start, int count)
+{
+    if (start < 0) { count += (int)start; start = 0; }
+    if (start + count > namps) count = (int)(namps - start);
+    if (count <= 0) return 0.0f;
+    float buf[200];
+    for (int i = 0; i < count; i++) {
+        const float re = amps[2 * (start + i)], im = amps[2 * (start + i) + 1];
+        buf[i] = re * re + im * im;
+    }
+    qsort(buf, (size_t)count, sizeof(float), orc_float_cmp);
+    return buf[(count - 1) / 2];
+}
+
+int orc_zap(float *amps, long long namps, long long N, double T, const double *freq_hz,
+            const double *width_hz, const int *bary, int nbirds, double baryv)
+{
+    if (nbirds <= 0) return 0;
+    OrcZapBin *b = malloc(sizeof(OrcZapBin) * (size_t)nbirds);
+    for (int i = 0; i < nbirds; i++) {
+        const double scale = bary[i] ? 1.0 : (1.0 + baryv);
+        double lo = (freq_hz[i] - 0.5 * width_hz[i]) * T * scale;
+        double hi = (freq_hz[i] + 0.5 * width_hz[i]) * T * scale;
+        if (hi - lo < 1.0) {
+            const double mid = 0.5 * (hi + lo);
+            lo = mid - 0.5;
+            hi = mid + 0.5;
+        }
+        b[i].lo = lo;
+        b[i].hi = hi;
+        b[i].idx = i;
+    }
+    qsort(b, (size_t)nbirds, sizeof(OrcZapBin), orc_zapbin_cmp);
+    long long limit = N / 2 < namps ? N / 2 : namps;
+    int applied = 0;
+    for (int i = 0; i < nbirds; i++) {
+        if (b[i].lo >= (double)limit) break;
+        if (b[i].hi >= (double)limit) b[i].hi = (double)(limit - 1);
+        const double lobin = b[i].lo, hibin = b[i].hi;
+        long long ilo = (long long)floor(lobin), ihi = (long long)ceil(hibin);
+        if (ilo < 1) ilo = 1;
+        if (ihi > namps) ihi = namps;
+        if (ihi - ilo <= 0) continue;
+        if (!(lobin - 300.0 > 1.0)) continue;
+        const float mlo = orc_zap_median(amps, namps, (long long)(lobin - 300), 200);
+        const float mhi = orc_zap_median(amps, namps, (long long)(hibin - 100), 200);
+        const float avg = (float)sqrt(0.5 * (mlo + mhi) / -log(0.5));
+        for (long long k = ilo; k < ihi; k++) {
+            amps[2 * k] = avg;
+            amps[2 * k + 1] = 0.0f;
+        }
+        applied++;
+    }
+    free(b);
+    return applied;
+}

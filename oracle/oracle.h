@@ -171,6 +171,12 @@ int orc_realfft(const float *samples, long long N, float *out);
 /* rednoise alone (datpipe.c:143-253) of a packed realfft output in[N]. */
 int orc_rednoise(const float *in, long long N, double dt, float *out);
 
+/* Zaplist on an interleaved spectrum of namps bins (src/zap.c:77-118,
+ * 155-187): birds (freq_hz[i], width_hz[i], bary[i]) over T seconds of N
+ * samples; returns the number of intervals zapped. */
+int orc_zap(float *amps, long long namps, long long N, double T, const double *freq_hz,
+            const double *width_hz, const int *bary, int nbirds, double baryv);
+
 #ifdef __cplusplus
 }
 #endif

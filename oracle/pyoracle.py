@@ -106,6 +106,7 @@ def lib():
         L.orc_realfft_rednoise.argtypes = [P, LL, D, P]; L.orc_realfft_rednoise.restype = I
         L.orc_realfft.argtypes = [P, LL, P]; L.orc_realfft.restype = I
         L.orc_rednoise.argtypes = [P, LL, D, P]; L.orc_rednoise.restype = I
+        L.orc_zap.argtypes = [P, LL, LL, D, P, P, P, I, D]; L.orc_zap.restype = I
         L.free = C.CDLL(None).free
         L.free.argtypes = [P]
         _lib = L
@@ -307,6 +308,18 @@ def realfft(samples):
     if lib().orc_realfft(ptr(s), s.size, ptr(out)) != 0:
         raise ValueError("realfft failed")
     return out
+
+
+def zap(amps, N, T, birds, baryv=0.0):
+    """Zaplist applied to a copy of an interleaved spectrum; birds: list of
+    (freq Hz, width Hz, bary).  Returns (zapped copy, intervals applied)."""
+    a = np.ascontiguousarray(amps, dtype=np.float32).reshape(-1).copy()
+    f = np.ascontiguousarray([b[0] for b in birds], dtype=np.float64)
+    w = np.ascontiguousarray([b[1] for b in birds], dtype=np.float64)
+    y = np.ascontiguousarray([int(b[2]) for b in birds], dtype=np.int32)
+    n = lib().orc_zap(ptr(a), a.size // 2, int(N), float(T), ptr(f), ptr(w), ptr(y), len(birds),
+                      float(baryv))
+    return a, n
 
 
 def rednoise(packed, dt):

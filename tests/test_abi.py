@@ -48,7 +48,9 @@ def test_struct_layouts():
     assert abi.SearchParams.decimations.offset == 60 and abi.SearchParams.sigma.offset == 320
     assert abi.DirectPlan.row.offset == 48 and abi.DirectPlan.gA_im.offset == 136
     assert abi.Candidate.ducy.offset == 32
-    assert C.sizeof(abi.CsSearchRequest) == 104 and abi.CsSearchRequest.amps_out.offset == 64
+    assert C.sizeof(abi.CsSearchRequest) == 120 and abi.CsSearchRequest.amps_out.offset == 64
+    assert abi.CsSearchRequest.zaplist.offset == 104 and abi.CsSearchRequest.baryv.offset == 112
+    assert C.sizeof(abi.ZapBird) == 24 and C.sizeof(abi.ZapList) == 16
     assert abi.CsSearchRequest.lobin.offset == 48 and abi.CsSearchRequest.threshold.offset == 56
     assert abi.CsSearchRequest.ncands.offset == 72 and abi.CsSearchRequest.harm_remove.offset == 84
     assert abi.CsSearchRequest.dr_tol.offset == 88 and abi.CsSearchRequest.harm_tol.offset == 96
@@ -173,3 +175,41 @@ def test_fftfile_roundtrip(tmp_path):
     assert np.array_equal(got, a)
     L.fftfile_close(C.byref(ft))
     assert L.fftfile_open(C.byref(ft), str(tmp_path / "missing.fft").encode()) == -1
+
+
+def _birds(rng, n, namps):
+    out = []
+    for i in range(n):
+        f = rng.uniform(10, namps + 50)
+        w = rng.choice([0.05, 0.8, 3.0, 17.0, 60.0])
+        out.append((f, w, int(rng.random() < 0.3)))
+    out += [(namps - 3.0, 20.0, 0), (320.0, 30.0, 0), (301.2, 0.2, 1)]
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_zaplist_apply_matches_oracle(oracle, seed):
+    """zaplist_apply (host library) against the oracle: same bins, same
+    levels, bit for bit, with overlapping, narrow, barycentric, near-DC and
+    past-the-top birds."""
+    rng = np.random.default_rng(seed)
+    nbins = 1 << 14
+    amps = synth.spectrum_with_pulsar(nbins, 123.4, 8, 0.1, 10.0, seed)
+    spec = cs.Spectrum(amps.copy(), 2 * nbins, synth.DT)
+    birds = _birds(rng, 40, nbins)
+    # frequencies in Hz for this T
+    birds = [(f / spec.T, w / spec.T, b) for f, w, b in birds]
+    n = cs.zap(spec, birds, baryv=2e-4)
+    want, n_want = oracle.zap(amps.view(np.float32), spec.N, spec.T, birds, baryv=2e-4)
+    assert n == n_want and n > 10
+    assert np.array_equal(spec.amps.view(np.float32), want)
+
+
+def test_zaplist_read(tmp_path):
+    p = tmp_path / "birds.zaplist"
+    p.write_text("# freq width\n\n60.0 0.5\nB 100.5\t0.25\n   \nabc def\n50 -1\n0 2\n"
+                 "B0.5 0.1\n120.25 1e-2 trailing words\n")
+    z = cs.Zaplist.read(p)
+    assert z.tuples() == [(60.0, 0.5, 0), (100.5, 0.25, 1), (0.5, 0.1, 1), (120.25, 0.01, 0)]
+    with pytest.raises(OSError):
+        cs.Zaplist.read(tmp_path / "missing")

@@ -379,3 +379,80 @@ def test_measure_ducy_of_injected_profile(oracle):
                  dtype=oracle.CAND_DTYPE)
     d = oracle.measure_ducy(amps.view(np.float32), 2 * nbins, c)
     assert 0.05 <= d["ducy"][0] <= 0.15
+
+
+# ---- zaplist (src/zap.c) ---------------------------------------------------
+
+LN2 = -np.log(0.5)
+
+
+def _flat(namps, value=(1.0, 1.0)):
+    a = np.zeros(2 * namps, dtype=np.float32)
+    a[0::2], a[1::2] = value
+    return a
+
+
+def test_zap_closed_forms(oracle):
+    """T = 1 s, so Hz are bins.  A flat spectrum of power 2: each zapped bin
+    becomes f32(sqrt(0.5 * (2 + 2) / ln 2)); bins floor(lo) .. ceil(hi) - 1
+    change and nothing else."""
+    namps, N = 4096, 8192
+    a = _flat(namps)
+    avg = np.float32(np.sqrt(0.5 * np.float32(2.0 + 2.0) / LN2))
+    z, n = oracle.zap(a, N, 1.0, [(1000.25, 10.0, 0)])
+    assert n == 1
+    want = a.copy()
+    want[2 * 995:2 * 1006:2], want[2 * 995 + 1:2 * 1006:2] = avg, 0.0
+    assert np.array_equal(z, want)
+    # narrower than a bin: widened to one bin around its centre (1999.8 .. 2000.8)
+    z, n = oracle.zap(a, N, 1.0, [(2000.3, 0.1, 0)])
+    assert n == 1 and np.flatnonzero(z != a).tolist() == [2 * 1999, 2 * 1999 + 1, 4000, 4001]
+    # within 301 bins of DC: skipped (lo - 300 must exceed 1)
+    for f, ok in ((250.0, 0), (301.5, 0), (302.5, 1)):
+        z, n = oracle.zap(a, N, 1.0, [(f, 1.0, 0)])
+        assert n == ok and np.array_equal(z, a) == (not ok)
+    # the top: hi clipped to N/2 - 1; a bird starting at N/2 and everything
+    # after it (in low-edge order) is dropped
+    z, n = oracle.zap(a, N, 1.0, [(4097.0, 2.0, 0), (4095.0, 10.0, 0)])
+    assert n == 1
+    changed = np.flatnonzero(z[0::2] != a[0::2])
+    assert changed.tolist() == list(range(4090, 4095))
+
+
+def test_zap_medians(oracle):
+    """The level is the lower median of the 200 powers at lo-300 .. lo-101
+    and at hi-100 .. hi+99: with powers k^2 (k = 1..200 and 201..400,
+    shuffled) those are 100^2 and 300^2."""
+    namps, N = 4096, 8192
+    rng = np.random.default_rng(3)
+    a = np.zeros(2 * namps, dtype=np.float32)
+    a[2 * 700:2 * 900:2] = rng.permutation(np.arange(1, 201)).astype(np.float32)
+    a[2 * 920:2 * 1120:2] = rng.permutation(np.arange(201, 401)).astype(np.float32)
+    z, n = oracle.zap(a, N, 1.0, [(1010.0, 20.0, 0)])
+    avg = np.float32(np.sqrt(0.5 * np.float32(100.0 ** 2 + 300.0 ** 2) / LN2))
+    assert n == 1
+    assert np.all(z[2 * 1000:2 * 1020:2] == avg) and np.all(z[2 * 1000 + 1:2 * 1020:2] == 0)
+    assert z[2 * 999] == a[2 * 999] and z[2 * 1020] == a[2 * 1020]
+
+
+def test_zap_order_and_scaling(oracle):
+    """Intervals apply in low-edge order whatever the input order, each one
+    after the previous (a later median window sees earlier zapped bins);
+    non-'B' lines scale by 1 + baryv, 'B' lines do not."""
+    namps, N = 1 << 14, 1 << 15
+    rng = np.random.default_rng(4)
+    a = (rng.standard_normal(2 * namps) * 0.7).astype(np.float32)
+    b1, b2 = (1010.0, 20.0, 0), (1260.0, 20.0, 0)   # b2's low window covers b1's bins
+    both, n = oracle.zap(a, N, 1.0, [b2, b1])
+    assert n == 2
+    step, _ = oracle.zap(oracle.zap(a, N, 1.0, [b1])[0], N, 1.0, [b2])
+    assert np.array_equal(both, step)
+    alone, _ = oracle.zap(a, N, 1.0, [b2])
+    assert not np.array_equal(alone[2 * 1250:2 * 1270], both[2 * 1250:2 * 1270])
+    v = 1e-3
+    scaled, _ = oracle.zap(a, N, 1.0, [(3000.0, 4.0, 0)], baryv=v)
+    bary, _ = oracle.zap(a, N, 1.0, [(3000.0 * (1 + v), 4.0 * (1 + v), 1)], baryv=0.0)
+    assert np.array_equal(scaled, bary)
+    same, _ = oracle.zap(a, N, 1.0, [(3000.0, 4.0, 1)], baryv=v)
+    plain, _ = oracle.zap(a, N, 1.0, [(3000.0, 4.0, 0)])
+    assert np.array_equal(same, plain)
