@@ -170,6 +170,8 @@ struct GpuSession {
     const FastVariant *fast;
     hipStream_t st;
     hipEvent_t ev0, ev1;
+    void *d_block;                /* spectrum, records, samples and tables in one block */
+    bool recs_own;                /* d_recs regrown outside the block */
     float2 *d_amps;
     float *d_wtab;
     int2 *d_hinfo;
@@ -267,12 +269,6 @@ static hipError_t dalloc(GpuSession *S, void **p, size_t bytes)
     return S->async_alloc ? hipMallocAsync(p, bytes, S->st) : hipMalloc(p, bytes);
 }
 
-static hipError_t dcopy(GpuSession *S, void *dst, const void *src, size_t bytes)
-{
-    return S->async_alloc ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S->st)
-                          : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
-}
-
 static void dfree(GpuSession *S, void *p)
 {
     if (!p) return;
@@ -283,16 +279,9 @@ static void dfree(GpuSession *S, void *p)
 static void session_release(GpuSession *S)
 {
     if (!S) return;
-    dfree(S, S->d_amps);
-    dfree(S, S->d_wtab);
-    dfree(S, S->d_hinfo);
-    dfree(S, S->d_tiles);
-    dfree(S, S->d_chunks);
-    dfree(S, S->d_recs);
-    dfree(S, S->d_counter);
+    if (S->recs_own) dfree(S, S->d_recs);
+    dfree(S, S->d_block);
     dfree(S, S->d_stack);
-    dfree(S, S->d_shp);
-    dfree(S, S->d_samp);
     if (S->gexec) hipGraphExecDestroy(S->gexec);
     if (S->graph) hipGraphDestroy(S->graph);
     if (S->ev0) hipEventDestroy(S->ev0);
@@ -360,6 +349,7 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
     S->chunks.resize((size_t)S->nchunks);
     if (S->measured) S->h_shp.assign((size_t)S->nchunks * S->nslot * CS_NSEQ, 0.0);
     S->tiles.clear();
+    S->tiles.reserve((size_t)4 * (size_t)(S->total / CS_TILE + S->nchunks + 1));
     S->ht = 0;
     for (long long c = 0; c < S->nchunks; c++) {
         ChunkDesc &C = S->chunks[(size_t)c];
@@ -447,6 +437,7 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->nh = p->nharms;
     S->m = p->m;
     S->cstarts.assign(cstarts, cstarts + nchunks);
+    S->d_block = NULL; S->recs_own = false;
     S->d_amps = NULL; S->d_wtab = NULL; S->d_hinfo = NULL; S->d_tw = NULL;
     S->d_tiles = NULL; S->d_chunks = NULL; S->d_recs = NULL; S->d_counter = NULL;
     S->d_stack = NULL; S->st = NULL; S->ev0 = NULL; S->ev1 = NULL;
@@ -603,26 +594,52 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         fprintf(stderr, "Error: twiddle table upload failed\n");
         goto fail;
     }
-    HIPCHK(dalloc(S, (void **)&S->d_amps, (size_t)std::max<long long>(S->namps, 1) * sizeof(float2)));
-    HIPCHK(dalloc(S, (void **)&S->d_wtab, S->h_wtab.size() * sizeof(float)));
-    HIPCHK(dcopy(S, S->d_wtab, S->h_wtab.data(), S->h_wtab.size() * sizeof(float)));
-    HIPCHK(dalloc(S, (void **)&S->d_hinfo, S->h_hinfo.size() * sizeof(int2)));
-    HIPCHK(dcopy(S, S->d_hinfo, S->h_hinfo.data(), S->h_hinfo.size() * sizeof(int2)));
-    HIPCHK(dalloc(S, (void **)&S->d_chunks, std::max<size_t>(S->chunks.size(), 1) * sizeof(ChunkDesc)));
-    if (!S->chunks.empty())
-        HIPCHK(dcopy(S, S->d_chunks, S->chunks.data(), S->chunks.size() * sizeof(ChunkDesc)));
-    HIPCHK(dalloc(S, (void **)&S->d_tiles, std::max<size_t>(S->tiles.size(), 4) * sizeof(int)));
-    if (!S->tiles.empty())
-        HIPCHK(dcopy(S, S->d_tiles, S->tiles.data(), S->tiles.size() * sizeof(int)));
-    if (S->measured && S->nchunks > 0) {
-        const size_t nsamp = (size_t)S->nchunks * S->nslot * CS_SIGCAP;
-        HIPCHK(dalloc(S, (void **)&S->d_samp, nsamp * (S->f64 ? sizeof(double) : sizeof(float))));
-        HIPCHK(dalloc(S, (void **)&S->d_shp, S->h_shp.size() * sizeof(double)));
-        HIPCHK(dcopy(S, S->d_shp, S->h_shp.data(), S->h_shp.size() * sizeof(double)));
-    }
     S->cap = 1u << 16;
-    HIPCHK(dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)));
-    HIPCHK(dalloc(S, (void **)&S->d_counter, sizeof(unsigned int)));
+    {
+        /* one device block: spectrum, records, counter, measured-sigma
+         * samples, then the tables in upload order (one pinned copy) */
+        const bool samp = S->measured && S->nchunks > 0;
+        const size_t nsamp = samp ? (size_t)S->nchunks * S->nslot * CS_SIGCAP : 0;
+        size_t off = 0;
+        const size_t o_amps =
+            cs_part(&off, (size_t)std::max<long long>(S->namps, 1) * sizeof(float2));
+        const size_t o_recs = cs_part(&off, (size_t)S->cap * sizeof(CandRec));
+        const size_t o_cnt = cs_part(&off, sizeof(unsigned int));
+        const size_t o_samp = cs_part(&off, nsamp * (S->f64 ? sizeof(double) : sizeof(float)));
+        const size_t o_tab = off;
+        size_t toff = 0;
+        const size_t t_wtab = cs_part(&toff, S->h_wtab.size() * sizeof(float));
+        const size_t t_hinfo = cs_part(&toff, S->h_hinfo.size() * sizeof(int2));
+        const size_t t_chunks = cs_part(&toff, std::max<size_t>(S->chunks.size(), 1) * sizeof(ChunkDesc));
+        const size_t t_tiles = cs_part(&toff, std::max<size_t>(S->tiles.size(), 4) * sizeof(int));
+        const size_t t_shp = cs_part(&toff, (samp ? S->h_shp.size() : 0) * sizeof(double));
+        HIPCHK(dalloc(S, &S->d_block, o_tab + toff));
+        char *b = (char *)S->d_block;
+        S->d_amps = (float2 *)(b + o_amps);
+        S->d_recs = (CandRec *)(b + o_recs);
+        S->d_counter = (unsigned int *)(b + o_cnt);
+        S->d_samp = samp ? (void *)(b + o_samp) : NULL;
+        char *tb = b + o_tab;
+        S->d_wtab = (float *)(tb + t_wtab);
+        S->d_hinfo = (int2 *)(tb + t_hinfo);
+        S->d_chunks = (ChunkDesc *)(tb + t_chunks);
+        S->d_tiles = (int4 *)(tb + t_tiles);
+        S->d_shp = samp ? (double *)(tb + t_shp) : NULL;
+        char *h = (char *)cs_upload_buffer(toff);
+        if (!h) {
+            fprintf(stderr, "Error: pinned staging buffer of %zu bytes failed\n", toff);
+            goto fail;
+        }
+        memset(h, 0, toff);
+        memcpy(h + t_wtab, S->h_wtab.data(), S->h_wtab.size() * sizeof(float));
+        memcpy(h + t_hinfo, S->h_hinfo.data(), S->h_hinfo.size() * sizeof(int2));
+        if (!S->chunks.empty())
+            memcpy(h + t_chunks, S->chunks.data(), S->chunks.size() * sizeof(ChunkDesc));
+        if (!S->tiles.empty())
+            memcpy(h + t_tiles, S->tiles.data(), S->tiles.size() * sizeof(int));
+        if (samp) memcpy(h + t_shp, S->h_shp.data(), S->h_shp.size() * sizeof(double));
+        HIPCHK(cs_upload(tb, toff, S->st));
+    }
 
     S->args.amps = S->d_amps;
     S->args.namps = S->namps;
@@ -835,8 +852,9 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
         S->last_ms = ms;
         if (count <= S->cap) break;
         /* never truncate: grow to the exact count and search again */
-        dfree(S, S->d_recs);
+        if (S->recs_own) dfree(S, S->d_recs);
         S->d_recs = NULL;
+        S->recs_own = true;
         S->cap = count;
         if (dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
             fprintf(stderr, "Error: cannot hold %u candidate records\n", count);

@@ -10,6 +10,22 @@
 /* longest rednoise block (the pipeline's endwidth) the device path holds */
 #define CS_STAGE_MAXBLK 1024
 
+/* Table uploads through a per-thread pinned staging buffer: cs_upload_buffer
+ * returns host memory for `bytes` (waiting for this thread's previous upload
+ * to finish reading it), cs_upload copies its first `bytes` to dst with one
+ * asynchronous DMA on st. */
+__attribute__((visibility("hidden"))) void *cs_upload_buffer(size_t bytes);
+__attribute__((visibility("hidden"))) hipError_t cs_upload(void *dst, size_t bytes,
+                                                           hipStream_t st);
+
+/* offsets of consecutive 256-byte aligned parts of one device block */
+static inline size_t cs_part(size_t *off, size_t bytes)
+{
+    const size_t o = *off;
+    *off = (o + bytes + 255) & ~(size_t)255;
+    return o;
+}
+
 extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g,
                                                                        const float *samples,
                                                                        float2 *d_out,

@@ -275,16 +275,41 @@ __global__ void unpack_kernel(const double2 *__restrict__ W, float2 *__restrict_
 }
 
 /* lower median of each block's powers, mean = (float)(median / ln 2) */
+/* The rednoise block layout (datpipe.c:143-226): block 0 starts at bin 1;
+ * while the running bin count over T stays below endfreq each block is
+ * startwidth*ln(binnum) long (the ramp, tabulated on the host), after that
+ * every block is endwidth long up to the last bin (computed here). */
+struct BlockLayout {
+    const long long *bstart;      /* ramp blocks */
+    const int *blen;
+    int nramp;                    /* tabulated blocks */
+    int nblocks;                  /* all blocks */
+    int endwidth;
+    long long tail;               /* first bin of block nramp */
+    long long nc;                 /* bins */
+};
+
+__device__ __forceinline__ void block_of(const BlockLayout &L, int b, long long &s, int &n)
+{
+    if (b < L.nramp) {
+        s = L.bstart[b];
+        n = L.blen[b];
+    } else {
+        s = L.tail + (long long)(b - L.nramp) * L.endwidth;
+        const long long avail = L.nc - s;
+        n = avail < L.endwidth ? (int)avail : L.endwidth;
+    }
+}
+
 __global__ __launch_bounds__(64) void block_mean_kernel(const float2 *__restrict__ d,
-                                                        const long long *__restrict__ bstart,
-                                                        const int *__restrict__ blen,
-                                                        float *__restrict__ mean, int nblocks)
+                                                        BlockLayout L, float *__restrict__ mean)
 {
     __shared__ float p[CS_STAGE_MAXBLK];
     const int b = blockIdx.x;
-    if (b >= nblocks) return;
-    const long long s = bstart[b];
-    const int n = blen[b];
+    if (b >= L.nblocks) return;
+    long long s;
+    int n;
+    block_of(L, b, s, n);
     for (int e = threadIdx.x; e < n; e += 64) {
         const float2 v = d[s + e];
         const float a = v.x * v.x, c = v.y * v.y;
@@ -314,37 +339,63 @@ __device__ __forceinline__ float presto_invsqrtf(float x)
     return y * (1.5f - hxyy);
 }
 
-/* segment s: bins [sstart, sstart+slen) normalised by
- * invsqrtf(mean[mo] + dslope*(float)ii), ii = bin - sstart, where dslope comes
- * from blocks (sa, sb) with denominator sden, or is 1.0f (sa < 0), or the
- * segment is the first half block with the constant invsqrtf(mean[mo]) */
+/* Segment sg of nblocks + 1: segment 0 is the first half of block 0,
+ * normalised by the constant invsqrtf(mean[0]); segment k (1 <= k < nblocks)
+ * runs from the middle of block k-1 to the middle of block k; the last one
+ * from the middle of the last block to its end.  Bins of segment k are
+ * normalised by invsqrtf(mean[k-1] + slope*(float)ii), ii = bin - first bin,
+ * slope = f32((mean[k] - mean[k-1]) / (0.5*(len[k-1] + len[k]))) (the last
+ * segment reuses the last pair, or slope 1 with a single block). */
 __global__ __launch_bounds__(128) void rednoise_kernel(const float2 *__restrict__ d,
-                                                       float2 *__restrict__ out,
-                                                       const long long *__restrict__ sstart,
-                                                       const int *__restrict__ slen,
-                                                       const int4 *__restrict__ sinfo,
-                                                       const double *__restrict__ sden,
-                                                       const float *__restrict__ mean, int nseg)
+                                                       float2 *__restrict__ out, BlockLayout L,
+                                                       const float *__restrict__ mean)
 {
     const int sg = blockIdx.x;
-    if (sg >= nseg) return;
+    const int nbk = L.nblocks;
+    if (sg > nbk) return;
     if (sg == 0 && threadIdx.x == 0) out[0] = make_float2(1.0f, 0.0f);
-    const int4 inf = sinfo[sg];          /* (mo, sa, sb, first) */
-    const float mo = mean[inf.x];
+    long long s0;
+    int n, mo;
+    bool first = false;
     float slope = 1.0f;
-    if (inf.y >= 0) {
-        const float diff = mean[inf.z] - mean[inf.y];
-        slope = (float)((double)diff / sden[sg]);
+    if (sg == 0) {
+        long long bs;
+        int bl;
+        block_of(L, 0, bs, bl);
+        s0 = 1;
+        n = bl / 2;
+        mo = 0;
+        first = true;
+    } else {
+        const int k = sg < nbk ? sg : nbk - 1;      /* the block whose middle ends the segment */
+        long long sa = 0, sb = 0;
+        int la = 0, lb = 0;
+        if (sg < nbk) {
+            block_of(L, k - 1, sa, la);
+            block_of(L, k, sb, lb);
+            s0 = sa + la / 2;
+            n = la - la / 2 + lb / 2;
+            mo = k - 1;
+        } else {
+            block_of(L, k, sb, lb);
+            s0 = sb + lb / 2;
+            n = lb - lb / 2;
+            mo = k;
+            if (nbk >= 2) block_of(L, k - 1, sa, la);
+        }
+        if (sg < nbk || nbk >= 2) {
+            const float diff = mean[k] - mean[k - 1];
+            slope = (float)((double)diff / (0.5 * (double)(la + lb)));
+        }
     }
-    const long long s0 = sstart[sg];
-    const int n = slen[sg];
+    const float m0 = mean[mo];
     for (int e = threadIdx.x; e < n; e += blockDim.x) {
         float norm;
-        if (inf.w) {
-            norm = presto_invsqrtf(mo);
+        if (first) {
+            norm = presto_invsqrtf(m0);
         } else {
             const float sl = slope * (float)e;
-            norm = presto_invsqrtf(mo + sl);
+            norm = presto_invsqrtf(m0 + sl);
         }
         const float2 v = d[s0 + e];
         out[s0 + e] = make_float2(v.x * norm, v.y * norm);
@@ -400,25 +451,21 @@ struct CsStage {
     int nlev;
     LevelArgs lev[kMaxLevels];
     long long grid[kMaxLevels];
+    /* every buffer is a part of one device block; the tables (twiddles and
+     * rednoise layout) are uploaded with one copy */
+    void *d_block;
     double2 *d_z0, *d_z1;         /* ping-pong, n2 each */
     double2 *d_tw;
     float *d_x;                   /* samples */
     float2 *d_packed;             /* unpacked realfft output */
-    long long *d_bstart, *d_sstart;
-    int *d_blen, *d_slen;
-    int4 *d_sinfo;
-    double *d_sden;
-    float *d_mean;
-    int nblocks, nseg;
+    long long *d_bstart;          /* ramp blocks of the rednoise layout */
+    int *d_blen;
+    float *d_mean;                /* per block */
+    int nblocks, nramp, endwidth;
+    long long tail;               /* first bin after the ramp */
     hipStream_t st;
     int own_stream;
     bool async_alloc;             /* stream-ordered allocations on st */
-    /* host tables, kept alive for asynchronous uploads */
-    std::vector<double2> h_tw;
-    std::vector<long long> h_bstart, h_sstart;
-    std::vector<int> h_blen, h_slen;
-    std::vector<int4> h_sinfo;
-    std::vector<double> h_sden;
 };
 
 static hipError_t salloc(CsStage *g, void **p, size_t bytes)
@@ -426,10 +473,68 @@ static hipError_t salloc(CsStage *g, void **p, size_t bytes)
     return g->async_alloc ? hipMallocAsync(p, bytes, g->st) : hipMalloc(p, bytes);
 }
 
-static hipError_t scopy(CsStage *g, void *dst, const void *src, size_t bytes)
+namespace {
+
+/* per-thread pinned staging for table uploads */
+struct UploadBuf {
+    void *host = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+    ~UploadBuf()
+    {
+        if (pending && ev) hipEventSynchronize(ev);
+        if (ev) hipEventDestroy(ev);
+        if (host) hipHostFree(host);
+    }
+};
+thread_local UploadBuf tl_upload;
+
+}  // namespace
+
+__attribute__((visibility("hidden"))) void *cs_upload_buffer(size_t bytes)
 {
-    return g->async_alloc ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->st)
-                          : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    UploadBuf &u = tl_upload;
+    if (u.pending) {
+        hipEventSynchronize(u.ev);
+        u.pending = false;
+    }
+    if (bytes > u.cap) {
+        if (u.host) hipHostFree(u.host);
+        u.host = nullptr;
+        u.cap = 0;
+        const size_t c = std::max<size_t>(bytes + bytes / 2, (size_t)1 << 20);
+        if (hipHostMalloc(&u.host, c, hipHostMallocDefault) != hipSuccess) {
+            u.host = nullptr;
+            return nullptr;
+        }
+        u.cap = c;
+    }
+    return u.host;
+}
+
+__attribute__((visibility("hidden"))) hipError_t cs_upload(void *dst, size_t bytes,
+                                                           hipStream_t st)
+{
+    UploadBuf &u = tl_upload;
+    if (bytes == 0) return hipSuccess;
+    if (!u.host || bytes > u.cap) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpyAsync(dst, u.host, bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    if (!u.ev) {
+        e = hipEventCreateWithFlags(&u.ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            hipStreamSynchronize(st);
+            return e;
+        }
+    }
+    e = hipEventRecord(u.ev, st);
+    if (e != hipSuccess) {
+        hipStreamSynchronize(st);
+        return e;
+    }
+    u.pending = true;
+    return hipSuccess;
 }
 
 static void sfree(CsStage *g, void *p)
@@ -448,9 +553,7 @@ extern "C" CS_API void cs_gpu_stage_free(CsStage *g)
 {
     if (!g) return;
     hipSetDevice(g->device);
-    sfree(g, g->d_z0); sfree(g, g->d_z1); sfree(g, g->d_tw); sfree(g, g->d_x);
-    sfree(g, g->d_packed); sfree(g, g->d_bstart); sfree(g, g->d_sstart); sfree(g, g->d_blen);
-    sfree(g, g->d_slen); sfree(g, g->d_sinfo); sfree(g, g->d_sden); sfree(g, g->d_mean);
+    sfree(g, g->d_block);
     if (g->own_stream && g->st) hipStreamDestroy(g->st);
     delete g;
 }
@@ -488,6 +591,10 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
                         "%d^%d points (N = %lld)\n", kLmax, kMaxLevels, N);
         return nullptr;
     }
+    /* host tables, packed into the pinned staging buffer at the end */
+    std::vector<double2> h_tw;
+    std::vector<long long> h_bstart;
+    std::vector<int> h_blen;
     CsStage *g = new CsStage();
     g->N = N;
     g->n2 = n2;
@@ -503,7 +610,7 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
         g->own_stream = 1;
     }
     {
-        std::vector<double2> &tw = g->h_tw;
+        std::vector<double2> &tw = h_tw;
         long long B = 1;
         for (int l = 0; l < g->nlev; l++) {
             LevelArgs &a = g->lev[l];
@@ -538,13 +645,10 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
             }
             B *= a.L;
         }
-        SCHK(salloc(g, (void **)&g->d_tw, tw.size() * sizeof(double2)));
-        SCHK(scopy(g, g->d_tw, tw.data(), tw.size() * sizeof(double2)));
     }
     {
-        /* rednoise block layout (datpipe.c:143-226): depends on N, dt, widths only */
-        std::vector<long long> &bstart = g->h_bstart;
-        std::vector<int> &blen = g->h_blen;
+        /* rednoise block layout (datpipe.c:143-226): depends on N, dt, widths
+         * only; the ramp is tabulated, the endwidth blocks after it are not */
         const long long nc = n2;
         const float T = (float)((double)N * dt);
         int bufflen = startwidth;
@@ -553,76 +657,82 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
             goto fail;
         }
         long long pos = 1;
-        bstart.push_back(pos);
-        blen.push_back(bufflen);
+        h_bstart.push_back(pos);
+        h_blen.push_back(bufflen);
         pos += bufflen;
         long long binnum = 1 + bufflen;
         bufflen = (int)(startwidth * log((double)binnum));
+        bool tail = false;
+        g->endwidth = endwidth;
+        g->tail = nc;
+        long long ntail = 0;
         for (;;) {
             const long long avail = nc - pos;
             if (avail <= 0) break;
+            if (tail) {
+                g->tail = pos;
+                ntail = (avail + endwidth - 1) / endwidth;
+                break;
+            }
             const int nb = avail < bufflen ? (int)avail : bufflen;
-            bstart.push_back(pos);
-            blen.push_back(nb);
+            h_bstart.push_back(pos);
+            h_blen.push_back(nb);
             pos += nb;
             binnum += nb;
-            if ((float)binnum / T < endfreq) bufflen = (int)(startwidth * log((double)binnum));
-            else bufflen = endwidth;
+            if ((float)binnum / T < endfreq) {
+                bufflen = (int)(startwidth * log((double)binnum));
+            } else {
+                bufflen = endwidth;
+                tail = true;
+            }
             if (bufflen > endwidth) {
                 fprintf(stderr, "Error: rednoise block length %d exceeds buffer %d\n", bufflen,
                         endwidth);
                 goto fail;
             }
         }
-        const int nbk = (int)bstart.size();
-        std::vector<long long> &sstart = g->h_sstart;
-        std::vector<int> &slen = g->h_slen;
-        std::vector<int4> &sinfo = g->h_sinfo;
-        std::vector<double> &sden = g->h_sden;
-        /* first half block: constant norm */
-        sstart.push_back(1);
-        slen.push_back(blen[0] / 2);
-        sinfo.push_back(make_int4(0, -1, -1, 1));
-        sden.push_back(1.0);
-        for (int k = 1; k < nbk; k++) {
-            const int mo = blen[k - 1] / 2, mn = blen[k] / 2;
-            sstart.push_back(bstart[k - 1] + mo);
-            slen.push_back(blen[k - 1] - mo + mn);
-            sinfo.push_back(make_int4(k - 1, k - 1, k, 0));
-            sden.push_back(0.5 * (double)(blen[k - 1] + blen[k]));
+        if ((long long)h_bstart.size() + ntail > 0x7fffffffLL) {
+            fprintf(stderr, "Error: rednoise layout too large\n");
+            goto fail;
         }
-        {
-            const int k = nbk - 1, mo = blen[k] / 2;
-            sstart.push_back(bstart[k] + mo);
-            slen.push_back(blen[k] - mo);
-            if (nbk >= 2) {
-                sinfo.push_back(make_int4(k, k - 1, k, 0));
-                sden.push_back(0.5 * (double)(blen[k - 1] + blen[k]));
-            } else {
-                sinfo.push_back(make_int4(k, -1, -1, 0));
-                sden.push_back(1.0);
-            }
-        }
-        g->nblocks = nbk;
-        g->nseg = (int)sstart.size();
-        SCHK(salloc(g, (void **)&g->d_bstart, nbk * sizeof(long long)));
-        SCHK(salloc(g, (void **)&g->d_blen, nbk * sizeof(int)));
-        SCHK(salloc(g, (void **)&g->d_mean, nbk * sizeof(float)));
-        SCHK(salloc(g, (void **)&g->d_sstart, g->nseg * sizeof(long long)));
-        SCHK(salloc(g, (void **)&g->d_slen, g->nseg * sizeof(int)));
-        SCHK(salloc(g, (void **)&g->d_sinfo, g->nseg * sizeof(int4)));
-        SCHK(salloc(g, (void **)&g->d_sden, g->nseg * sizeof(double)));
-        SCHK(scopy(g, g->d_bstart, bstart.data(), nbk * sizeof(long long)));
-        SCHK(scopy(g, g->d_blen, blen.data(), nbk * sizeof(int)));
-        SCHK(scopy(g, g->d_sstart, sstart.data(), g->nseg * sizeof(long long)));
-        SCHK(scopy(g, g->d_slen, slen.data(), g->nseg * sizeof(int)));
-        SCHK(scopy(g, g->d_sinfo, sinfo.data(), g->nseg * sizeof(int4)));
-        SCHK(scopy(g, g->d_sden, sden.data(), g->nseg * sizeof(double)));
+        g->nramp = (int)h_bstart.size();
+        g->nblocks = (int)(g->nramp + ntail);
     }
-    SCHK(salloc(g, (void **)&g->d_z0, n2 * sizeof(double2)));
-    SCHK(salloc(g, (void **)&g->d_z1, n2 * sizeof(double2)));
-    SCHK(salloc(g, (void **)&g->d_x, N * sizeof(float)));
-    SCHK(salloc(g, (void **)&g->d_packed, n2 * sizeof(float2)));
+    {
+        /* one device block: work buffers, then the tables in upload order */
+        const size_t nbk = (size_t)g->nblocks, nramp = (size_t)g->nramp;
+        size_t off = 0;
+        const size_t o_z0 = cs_part(&off, (size_t)n2 * sizeof(double2));
+        const size_t o_z1 = cs_part(&off, (size_t)n2 * sizeof(double2));
+        const size_t o_x = cs_part(&off, (size_t)N * sizeof(float));
+        const size_t o_packed = cs_part(&off, (size_t)n2 * sizeof(float2));
+        const size_t o_mean = cs_part(&off, nbk * sizeof(float));
+        const size_t o_tab = off;
+        size_t toff = 0;
+        const size_t t_tw = cs_part(&toff, h_tw.size() * sizeof(double2));
+        const size_t t_bstart = cs_part(&toff, nramp * sizeof(long long));
+        const size_t t_blen = cs_part(&toff, nramp * sizeof(int));
+        SCHK(salloc(g, &g->d_block, o_tab + toff));
+        char *b = (char *)g->d_block;
+        g->d_z0 = (double2 *)(b + o_z0);
+        g->d_z1 = (double2 *)(b + o_z1);
+        g->d_x = (float *)(b + o_x);
+        g->d_packed = (float2 *)(b + o_packed);
+        g->d_mean = (float *)(b + o_mean);
+        char *tb = b + o_tab;
+        g->d_tw = (double2 *)(tb + t_tw);
+        g->d_bstart = (long long *)(tb + t_bstart);
+        g->d_blen = (int *)(tb + t_blen);
+        char *h = (char *)cs_upload_buffer(toff);
+        if (!h) {
+            fprintf(stderr, "Error: pinned staging buffer of %zu bytes failed\n", toff);
+            goto fail;
+        }
+        memcpy(h + t_tw, h_tw.data(), h_tw.size() * sizeof(double2));
+        memcpy(h + t_bstart, h_bstart.data(), nramp * sizeof(long long));
+        memcpy(h + t_blen, h_blen.data(), nramp * sizeof(int));
+        SCHK(cs_upload(tb, toff, g->st));
+    }
     return g;
 fail:
     cs_gpu_stage_free(g);
@@ -663,11 +773,18 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
         const long long nthr = nq > 0 ? nq : 1;
         unpack_kernel<<<(unsigned)((nthr + 255) / 256), 256, 0, st>>>(src, g->d_packed, g->N);
         SCHK(hipGetLastError());
-        block_mean_kernel<<<(unsigned)g->nblocks, 64, 0, st>>>(g->d_packed, g->d_bstart, g->d_blen,
-                                                              g->d_mean, g->nblocks);
+        BlockLayout L;
+        L.bstart = g->d_bstart;
+        L.blen = g->d_blen;
+        L.nramp = g->nramp;
+        L.nblocks = g->nblocks;
+        L.endwidth = g->endwidth;
+        L.tail = g->tail;
+        L.nc = g->n2;
+        block_mean_kernel<<<(unsigned)g->nblocks, 64, 0, st>>>(g->d_packed, L, g->d_mean);
         SCHK(hipGetLastError());
-        rednoise_kernel<<<(unsigned)g->nseg, 128, 0, st>>>(g->d_packed, d_out, g->d_sstart, g->d_slen,
-                                                           g->d_sinfo, g->d_sden, g->d_mean, g->nseg);
+        rednoise_kernel<<<(unsigned)g->nblocks + 1, 128, 0, st>>>(g->d_packed, d_out, L,
+                                                                  g->d_mean);
         SCHK(hipGetLastError());
     }
     return 0;
