@@ -98,6 +98,13 @@ class CsSearchInfo(C.Structure):
                 ("search_ms", C.c_double), ("post_ms", C.c_double)]
 
 
+class CsTuneResult(C.Structure):
+    _fields_ = [("nharms", C.c_int), ("m_padded", C.c_int), ("precision", C.c_int),
+                ("sigma", C.c_int), ("lane_order", C.c_int), ("xcd_stripe", C.c_int),
+                ("blocksize", C.c_int), ("ntried", C.c_int), ("default_ms", C.c_double),
+                ("best_ms", C.c_double)]
+
+
 class CsEngineStats(C.Structure):
     _fields_ = [("budget", C.c_longlong), ("in_use", C.c_longlong),
                 ("peak_in_use", C.c_longlong), ("nslots", C.c_int), ("running", C.c_int),
@@ -116,7 +123,8 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
                "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
                "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_gpu_session_set_graphs",
-               "cs_gpu_session_finalize", "cs_gpu_session_zap", "cs_engine_create",
+               "cs_gpu_session_finalize", "cs_gpu_session_zap", "cs_gpu_autotune",
+               "cs_gpu_tune_save", "cs_gpu_tune_load", "cs_engine_create",
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
                "cs_engine_stats", "cs_engine_free", "cs_host_alloc", "cs_host_free"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
@@ -233,6 +241,12 @@ def _declare(L):
     L.cs_gpu_session_stage.restype = I
     L.cs_gpu_session_set_graphs.argtypes = [P, I]
     L.cs_gpu_session_set_graphs.restype = I
+    L.cs_gpu_autotune.argtypes = [I, Ps(SearchParams), Ps(CsTuneResult)]
+    L.cs_gpu_autotune.restype = I
+    L.cs_gpu_tune_save.argtypes = [C.c_char_p]
+    L.cs_gpu_tune_save.restype = I
+    L.cs_gpu_tune_load.argtypes = [C.c_char_p]
+    L.cs_gpu_tune_load.restype = I
     L.cs_gpu_session_zap.argtypes = [P, Ps(ZapList), D, P]
     L.cs_gpu_session_zap.restype = I
     L.cs_gpu_session_finalize.argtypes = [P, Ps(CandidateList), I]

@@ -114,7 +114,15 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     const double r_hi = r->in.hifreq * T;
     long long total = (long long)floor((r_hi - r_lo) / lodr) + 1;
     if (total < 0) total = 0;
-    const int Nprof = r->in.blocksize > 0 ? r->in.blocksize : CS_GPU_DEFAULT_BLOCKSIZE;
+    int Nprof = r->in.blocksize > 0 ? r->in.blocksize : CS_GPU_DEFAULT_BLOCKSIZE;
+    if (r->in.blocksize < 0 && p->sigma == CS_SIGMA_ANALYTIC) {
+        /* the class's tuned chunk length, when it has one: the analytic
+         * metrics do not depend on it, the raw list's chunk order does */
+        CsTuneResult tr;
+        const int mp = p->m <= 16 ? 16 : 32;
+        if (cs_tune_lookup(E->device, p->nharms, mp, p->precision == CS_PREC_F64, false, &tr))
+            Nprof = tr.blocksize;
+    }
 
     FFTFile ft;
     memset(&ft, 0, sizeof(ft));

@@ -413,6 +413,21 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
     return 0;
 }
 
+/* Launch shape of the search kernel: lane order (0 residue classes, 1
+ * natural) and the XCD tile mapping (0 contiguous ranges, S > 0 stripes of S
+ * tiles).  Neither changes any result; a captured graph is dropped. */
+__attribute__((visibility("hidden"))) void cs_session_set_launch(GpuSession *S, int lane_order,
+                                                                 int xcd_stripe)
+{
+    if (!S) return;
+    S->args.grp = lane_order == 0 && S->q < CS_TILE ? CS_TILE / S->q : 1;
+    S->args.xcd_stripe = xcd_stripe > 0 ? xcd_stripe : 0;
+    if (S->gexec) hipGraphExecDestroy(S->gexec);
+    if (S->graph) hipGraphDestroy(S->graph);
+    S->gexec = NULL;
+    S->graph = NULL;
+}
+
 /* ext: run on the caller's stream with stream-ordered allocations (the
  * engine's concurrent sessions); null: own stream, plain allocations. */
 __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFTFile *ft, const SearchParams *p,
@@ -659,6 +674,13 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->args.recs = S->d_recs;
     S->args.counter = S->d_counter;
     S->args.cap = S->cap;
+    S->args.xcd_stripe = 0;
+    {
+        /* a tuned launch for this kernel class, when there is one */
+        CsTuneResult tr;
+        if (cs_tune_lookup(device, nh, mp, S->f64, S->measured, &tr))
+            cs_session_set_launch(S, tr.lane_order, tr.xcd_stripe);
+    }
     return S;
 fail:
     session_release(S);
@@ -695,7 +717,8 @@ static int enqueue_search(GpuSession *S, double threshold, float2 *stack_out)
     S->args.recs = S->d_recs;
     S->args.cap = S->cap;
     const int ntiles = S->args.ntiles;
-    const int nblk = ((ntiles + 7) / 8) * 8;
+    const int unit = 8 * std::max(S->args.xcd_stripe, 1);
+    const int nblk = ((ntiles + unit - 1) / unit) * unit;
     fast_launch_fn fn = S->f64 ? S->fast->fn64 : S->fast->fn;
     hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
     if (hipGetLastError() != hipSuccess) return -1;

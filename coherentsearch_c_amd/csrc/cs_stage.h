@@ -51,4 +51,13 @@ cs_session_create_on(const FFTFile *ft, const SearchParams *p, const DirectPlanS
 __attribute__((visibility("hidden"))) CandidateList
 cs_session_finalize(GpuSession *s, const CandidateList *in, int ncands);
 
+/* launch shape of the search kernel (no effect on results) */
+__attribute__((visibility("hidden"))) void cs_session_set_launch(GpuSession *s, int lane_order,
+                                                                 int xcd_stripe);
+
+/* the tuned launch of a kernel class (device, nharms, padded taps, f64,
+ * measured sigma), if cs_gpu_autotune has produced or loaded one */
+__attribute__((visibility("hidden"))) bool cs_tune_lookup(int device, int nh, int mp, bool f64,
+                                                          bool measured, CsTuneResult *out);
+
 #endif

@@ -343,6 +343,8 @@ struct FastArgs {
     void *samp;               /* measured sigma sample pass: |profile| samples per
                                  (chunk, slot), CS_SIGCAP of T each; null otherwise */
     int nslot;
+    int xcd_stripe;           /* 0: XCD x runs tiles [x*n/8, (x+1)*n/8); S > 0: stripes of
+                                 S consecutive tiles dealt to the XCDs in turn */
     float2 *stack_out;        /* optional stack dump of trials [dump_t0, +dump_n) */
     long long dump_t0;
     long long dump_n;
@@ -417,10 +419,17 @@ void fast_search_kernel(FastArgs a)
     __shared__ int s_lo[NH];
     __shared__ int s_C[NH];
 
-    /* XCD-aware mapping: consecutive tiles land on the same XCD's L2. */
+    /* XCD-aware mapping: workgroup b runs on XCD b % 8; consecutive tiles
+     * land on the same XCD's L2 (the grid is a multiple of 8, of 8*S with
+     * stripes). */
     const int nblk = gridDim.x;
-    const int per = nblk >> 3;
-    const int tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    int tile;
+    if (a.xcd_stripe > 0) {
+        const int S = a.xcd_stripe, i = blockIdx.x >> 3;
+        tile = ((i / S) * 8 + (blockIdx.x & 7)) * S + i % S;
+    } else {
+        tile = (blockIdx.x & 7) * (nblk >> 3) + (blockIdx.x >> 3);
+    }
     if (tile >= a.ntiles) return;
 
     const int4 Tl = a.tiles[tile];

@@ -348,6 +348,16 @@ def finalize(spec, params, cands, ncands=100):
     return take_list(d)
 
 
+def autotune(params, device=0):
+    """cs_gpu_autotune: the fastest launch shape (and, for analytic sigma,
+    chunk length) of this kernel class, measured on a synthetic spectrum and
+    used by later sessions and engine requests.  Returns a dict."""
+    r = abi.CsTuneResult()
+    if abi.host().cs_gpu_autotune(int(device), C.byref(params), C.byref(r)) != 0:
+        raise RuntimeError("no GPU kernel serves these parameters")
+    return {k: getattr(r, k) for k, _ in abi.CsTuneResult._fields_}
+
+
 def gpu_available(device=0):
     buf = C.create_string_buffer(256)
     rc = abi.host().cs_gpu_available(device, buf, 256)

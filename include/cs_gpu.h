@@ -154,6 +154,35 @@ CS_API int cs_gpu_session_stage(GpuSession *s, CsStage *g, const float *samples,
                                 float *amps_host);
 
 /* ------------------------------------------------------------------ */
+/* Autotuner (no reference counterpart).  For one kernel class (device,
+ * nharms, padded taps 16/32, precision, sigma mode) it times the search on a
+ * synthetic 2^22-sample noise spectrum for every candidate launch shape --
+ * the lane order inside a tile (trials in residue classes or in natural
+ * order), the mapping of tiles to the 8 XCDs (contiguous ranges or stripes),
+ * and, for analytic sigma, the chunk length (blocksize 16384, 65536 or
+ * 262144) -- and keeps the fastest.  Sessions created afterwards for the class
+ * use its launch shape, which changes no result.  The chunk length is advice:
+ * engine requests with blocksize < 0 take it (an analytic-sigma metric does
+ * not depend on it, but the raw list's chunk -> decimation -> trial order
+ * does, so it is never applied silently). */
+typedef struct {
+    int nharms, m_padded, precision, sigma;   /* the class */
+    int lane_order;        /* 0 residue classes, 1 natural order */
+    int xcd_stripe;        /* 0 contiguous range per XCD, S: stripes of S tiles */
+    int blocksize;         /* chosen chunk length (the default for measured sigma) */
+    int ntried;            /* launch shapes timed */
+    double default_ms;     /* search kernel time of the default shape */
+    double best_ms;        /* ... of the chosen one */
+} CsTuneResult;
+
+/* tunes the class of p on `device` (or returns the cached result); 0 on
+ * success, -1 if no GPU kernel serves p */
+CS_API int cs_gpu_autotune(int device, const SearchParams *p, CsTuneResult *out);
+/* the tuned classes as text (one line each) / read back; 0 on success */
+CS_API int cs_gpu_tune_save(const char *path);
+CS_API int cs_gpu_tune_load(const char *path);
+
+/* ------------------------------------------------------------------ */
 /* Search engine: continuous batching of whole searches (no reference
  * counterpart: the reference searches one file at a time behind prefetch
  * threads, src/cli.c:590-617).  `nslots` worker streams; a request is
@@ -172,7 +201,10 @@ typedef struct {
     const SearchParams *p;  /* copied at submit */
     double lofreq, hifreq;  /* search band (Hz); lobin != 100 overrides lofreq, as search_file */
     int lobin;
-    int blocksize;          /* 0: CS_GPU_DEFAULT_BLOCKSIZE */
+    int blocksize;          /* 0: CS_GPU_DEFAULT_BLOCKSIZE; < 0: the autotuner's chunk
+                               length for the class (analytic sigma; same metrics and,
+                               after the collapse, the same list; the raw list's
+                               chunk order differs) */
     double threshold;
     float *amps_out;        /* optional host buffer of N floats: the staged spectrum */
     /* ncands > 0: the final list instead of the raw one -- collapsed as

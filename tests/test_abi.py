@@ -51,6 +51,7 @@ def test_struct_layouts():
     assert C.sizeof(abi.CsSearchRequest) == 120 and abi.CsSearchRequest.amps_out.offset == 64
     assert abi.CsSearchRequest.zaplist.offset == 104 and abi.CsSearchRequest.baryv.offset == 112
     assert C.sizeof(abi.ZapBird) == 24 and C.sizeof(abi.ZapList) == 16
+    assert C.sizeof(abi.CsTuneResult) == 48 and abi.CsTuneResult.default_ms.offset == 32
     assert abi.CsSearchRequest.lobin.offset == 48 and abi.CsSearchRequest.threshold.offset == 56
     assert abi.CsSearchRequest.ncands.offset == 72 and abi.CsSearchRequest.harm_remove.offset == 84
     assert abi.CsSearchRequest.dr_tol.offset == 88 and abi.CsSearchRequest.harm_tol.offset == 96
