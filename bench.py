@@ -200,6 +200,8 @@ def main():
     ap.add_argument("--sweep-files", type=int, default=64)
     ap.add_argument("--serve-requests", type=int, default=500)
     ap.add_argument("--slots", type=int, default=4, help="engine worker streams")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner (default launch shapes)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -228,6 +230,16 @@ def main():
     log("staging (synth + realfft + rednoise): %.2f s" % (time.perf_counter() - t))
     pkw = dict(nharms=60, m=16, maxdecim=6)
     params = cs.make_params(**pkw)
+    tune = []
+    if not args.no_tune:
+        # launch shapes of every kernel class the bench runs (the sweep and the
+        # served stream mix nharms 16/32/60); later sessions use them
+        t = time.perf_counter()
+        for nh in (60, 32, 16):
+            r = cs.autotune(cs.make_params(nharms=nh, m=16, maxdecim=6))
+            tune.append({k: r[k] for k in ("nharms", "lane_order", "xcd_stripe", "blocksize",
+                                           "ntried", "default_ms", "best_ms")})
+        log("autotune: %s (%.2f s)" % (tune, time.perf_counter() - t))
     t = time.perf_counter()
     sess = cs.Session(spec, params, blocksize=65536)
     ht, trials, alg_bytes = sess.work()
@@ -284,6 +296,12 @@ def main():
         "upload_inclusive_ht_s": ht / (wall / args.steps + up_s),
         "upload_s": up_s,
     }
+    if tune:
+        out["autotune"] = {"spectrum": "2^22-sample noise, 0.1-125 Hz, m 16, f32, analytic "
+                                       "sigma; lane order 0 residue classes / 1 natural, "
+                                       "xcd_stripe 0 contiguous / S tiles; blocksize is "
+                                       "advice (requests keep 65536)",
+                           "classes": tune}
     if not args.no_sweep:
         out["dm_sweep"] = dm_sweep(args.sweep_files, args.slots)
     if not args.no_serve:
