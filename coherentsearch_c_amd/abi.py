@@ -91,7 +91,7 @@ class CsSearchRequest(C.Structure):
 
 
 class CsSearchInfo(C.Structure):
-    _fields_ = [("status", C.c_int), ("pad", C.c_int), ("harmonic_trials", C.c_longlong),
+    _fields_ = [("status", C.c_int), ("reused", C.c_int), ("harmonic_trials", C.c_longlong),
                 ("trials", C.c_longlong), ("footprint", C.c_longlong),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("plan_ms", C.c_double), ("stage_ms", C.c_double),
@@ -109,7 +109,8 @@ class CsEngineStats(C.Structure):
     _fields_ = [("budget", C.c_longlong), ("in_use", C.c_longlong),
                 ("peak_in_use", C.c_longlong), ("nslots", C.c_int), ("running", C.c_int),
                 ("peak_running", C.c_int), ("pad", C.c_int), ("queued", C.c_longlong),
-                ("completed", C.c_longlong)]
+                ("completed", C.c_longlong), ("plan_hits", C.c_longlong),
+                ("plans_resident", C.c_longlong)]
 
 
 assert C.sizeof(SimpleInf) == 80 and C.sizeof(FFTFile) == 168
@@ -126,7 +127,8 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_finalize", "cs_gpu_session_zap", "cs_gpu_autotune",
                "cs_gpu_tune_save", "cs_gpu_tune_load", "cs_engine_create",
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
-               "cs_engine_stats", "cs_engine_free", "cs_host_alloc", "cs_host_free"]
+               "cs_engine_stats", "cs_engine_free", "cs_engine_set_plan_cache", "cs_host_alloc",
+               "cs_host_free"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "direct_plans_build", "direct_plans_free", "cs_chunk_state",
                 "cs_harmonic_fits", "harmonic_change_trials", "chunk_starts", "cs_sinpi",
@@ -263,6 +265,8 @@ def _declare(L):
     L.cs_engine_wait.restype = I
     L.cs_engine_stats.argtypes = [P, Ps(CsEngineStats)]
     L.cs_engine_stats.restype = None
+    L.cs_engine_set_plan_cache.argtypes = [P, I]
+    L.cs_engine_set_plan_cache.restype = None
     L.cs_engine_free.argtypes = [P]
     L.cs_engine_free.restype = None
     L.cs_host_alloc.argtypes = [LL]

@@ -15,8 +15,16 @@
  * Requests that arrive while others run are admitted as slots and memory free
  * up, and their transfers and kernels overlap on the device.  Each request's
  * result is the raw candidate list of cs_gpu_session_run -- the same
- * arithmetic whether it ran alone or beside others (no state is shared
- * between requests except the constant twiddle table).
+ * arithmetic whether it ran alone or beside others.
+ *
+ * Plans stay resident: the session (spectrum, weight tables, chunk and tile
+ * tables, records) and the stage plan of a finished request are kept, keyed
+ * by the geometry (length, sample time, search options, band, chunk length),
+ * and a later request of the same geometry -- the next DM trial of a sweep --
+ * runs on them with no host planning, allocation or table upload, replaying
+ * its staging and search launches from captured hipGraphs.  Idle plans count
+ * against the HBM budget and are evicted, least recently used first, when a
+ * request needs the room or the cache is full.
  */
 #include <hip/hip_runtime.h>
 
@@ -39,7 +47,7 @@
 
 static_assert(sizeof(CsSearchRequest) == 120, "CsSearchRequest layout");
 static_assert(sizeof(CsSearchInfo) == 96, "CsSearchInfo layout");
-static_assert(sizeof(CsEngineStats) == 56, "CsEngineStats layout");
+static_assert(sizeof(CsEngineStats) == 72, "CsEngineStats layout");
 
 namespace {
 
@@ -58,9 +66,25 @@ struct EngReq {
     std::vector<ZapBird> zap;     /* the request's zaplist, copied at submit */
     ZapList zl{};
     int state = 0;                /* 0 queued, 1 running, 2 done */
+    long long reserved = 0;       /* HBM the request holds in the budget */
     CandidateList out{};
     CsSearchInfo info{};
     clk::time_point t_submit, t_start, t_end;
+};
+
+/* A resident plan: the session and stage plan of one geometry. */
+struct Plan {
+    long long N = 0;
+    double dt = 0.0, r_lo = 0.0, r_hi = 0.0;
+    int Nprof = 0;
+    SearchParams p{};             /* the session points here */
+    DirectPlanSet set{};
+    long long *cstarts = NULL;
+    GpuSession *S = NULL;
+    CsStage *g = NULL;
+    long long footprint = 0;
+    bool busy = false;
+    unsigned long long last_use = 0;
 };
 
 }  // namespace
@@ -70,6 +94,11 @@ struct CsEngine {
     long long budget = 0, in_use = 0, peak_in_use = 0;
     int nslots = 0, running = 0, peak_running = 0;
     long long completed = 0;
+    int max_plans = 8;            /* resident plans kept; 0: none */
+    std::vector<Plan *> plans;
+    unsigned long long use_clock = 0;
+    long long plan_hits = 0;
+    hipStream_t idle_st = NULL;   /* idle plans are bound here (their frees run on it) */
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<EngReq *> queue;
@@ -103,6 +132,44 @@ long long footprint_of(const CsSearchRequest *r, const SearchParams *p)
     return b;
 }
 
+void plan_free(Plan *P)
+{
+    if (!P) return;
+    if (P->g) cs_gpu_stage_free(P->g);
+    if (P->S) cs_gpu_session_free(P->S);
+    csx_plans_free(&P->set);
+    free(P->cstarts);
+    delete P;
+}
+
+bool plan_matches(const Plan *P, long long N, double dt, const SearchParams *p, double r_lo,
+                  double r_hi, int Nprof)
+{
+    return P->N == N && P->dt == dt && P->r_lo == r_lo && P->r_hi == r_hi &&
+           P->Nprof == Nprof && memcmp(&P->p, p, sizeof(SearchParams)) == 0;
+}
+
+/* under E->mu: drop idle plans, least recently used first, until `need`
+ * more bytes fit the budget (need < 0: until at most max_plans remain) */
+void evict_idle(CsEngine *E, long long need)
+{
+    for (;;) {
+        const bool over = need >= 0 ? E->in_use + need > E->budget
+                                    : (int)E->plans.size() > E->max_plans;
+        if (!over) return;
+        int victim = -1;
+        for (int i = 0; i < (int)E->plans.size(); i++)
+            if (!E->plans[i]->busy &&
+                (victim < 0 || E->plans[i]->last_use < E->plans[victim]->last_use))
+                victim = i;
+        if (victim < 0) return;
+        Plan *P = E->plans[victim];
+        E->plans.erase(E->plans.begin() + victim);
+        E->in_use -= P->footprint;
+        plan_free(P);
+    }
+}
+
 int run_request(CsEngine *E, EngReq *r, hipStream_t st)
 {
     const SearchParams *p = &r->p;
@@ -124,32 +191,68 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
             Nprof = tr.blocksize;
     }
 
-    FFTFile ft;
-    memset(&ft, 0, sizeof(ft));
-    ft.N = N;
-    ft.T = T;
-    ft.df = 1.0 / T;
-    ft.namps = N / 2;
-    ft.DC = 1.0;
-    DirectPlanSet set;
-    memset(&set, 0, sizeof(set));
-    long long *cstarts = NULL, nchunks = 0;
-    GpuSession *S = NULL;
-    CsStage *g = NULL;
-    int rc = -1;
     const clk::time_point t0 = clk::now();
     clk::time_point t1, t2, t3;
-    if (csx_plans_build(&set, p, r_lo) != 0) goto done;
-    if (csx_chunk_starts(&set, lodr, ft.namps, N, total, Nprof, &cstarts, &nchunks) != 0)
-        goto done;
-    S = cs_session_create_on(&ft, p, &set, cstarts, total, Nprof, nchunks, r_lo, lodr,
-                             E->device, st);
-    if (!S) goto done;
-    g = cs_stage_create_on(N, dt, 6, 100, 6.0, E->device, st);
-    if (!g) goto done;
+    Plan *P = NULL;
+    bool reused = false;
+    int rc = -1;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        for (Plan *c : E->plans)
+            if (!c->busy && plan_matches(c, N, dt, p, r_lo, r_hi, Nprof)) {
+                P = c;
+                break;
+            }
+        if (P) {
+            /* the plan's memory is already in the budget */
+            P->busy = true;
+            reused = true;
+            E->plan_hits++;
+            E->in_use -= r->reserved;
+            r->reserved = 0;
+        }
+    }
+    if (!P) {
+        P = new Plan();
+        P->N = N;
+        P->dt = dt;
+        P->r_lo = r_lo;
+        P->r_hi = r_hi;
+        P->Nprof = Nprof;
+        P->p = *p;
+        P->busy = true;
+        FFTFile ft;
+        memset(&ft, 0, sizeof(ft));
+        ft.N = N;
+        ft.T = T;
+        ft.df = 1.0 / T;
+        ft.namps = N / 2;
+        ft.DC = 1.0;
+        long long nchunks = 0;
+        if (csx_plans_build(&P->set, &P->p, r_lo) != 0 ||
+            csx_chunk_starts(&P->set, lodr, ft.namps, N, total, Nprof, &P->cstarts,
+                             &nchunks) != 0 ||
+            !(P->S = cs_session_create_on(&ft, &P->p, &P->set, P->cstarts, total, Nprof,
+                                          nchunks, r_lo, lodr, E->device, st)) ||
+            !(P->g = cs_stage_create_on(N, dt, 6, 100, 6.0, E->device, st))) {
+            plan_free(P);
+            return -1;
+        }
+    }
+    r->info.reused = reused ? 1 : 0;
+    GpuSession *S = P->S;
+    CsStage *g = P->g;
+    /* the plan runs on this worker's stream; graphs once it is reused */
+    cs_session_bind_stream(S, st, reused);
+    cs_stage_bind_stream(g, st);
     t1 = clk::now();
     r->info.plan_ms = ms_between(t0, t1);
-    if (r->zl.n > 0) {
+    if (reused) {
+        if (cs_session_stage_graph(S, g, r->in.samples) != 0) goto done;
+        if ((r->zl.n > 0 || r->in.amps_out) &&
+            cs_gpu_session_zap(S, &r->zl, r->in.baryv, r->in.amps_out) < 0)
+            goto done;
+    } else if (r->zl.n > 0) {
         if (cs_gpu_session_stage(S, g, r->in.samples, NULL) != 0 ||
             cs_gpu_session_zap(S, &r->zl, r->in.baryv, r->in.amps_out) < 0)
             goto done;
@@ -192,10 +295,33 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     }
     rc = 0;
 done:
-    if (g) cs_gpu_stage_free(g);
-    if (S) cs_gpu_session_free(S);
-    csx_plans_free(&set);
-    free(cstarts);
+    hipStreamSynchronize(st);
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        if (rc == 0 && E->max_plans > 0) {
+            /* keep the plan resident, idle, bound to the engine's idle stream */
+            cs_session_bind_stream(S, E->idle_st, false);
+            cs_stage_bind_stream(g, E->idle_st);
+            P->busy = false;
+            P->last_use = ++E->use_clock;
+            if (!reused) {
+                P->footprint = r->reserved;       /* the request's share moves to the plan */
+                r->reserved = 0;
+                E->plans.push_back(P);
+            }
+            evict_idle(E, -1);
+        } else {
+            if (reused) {
+                for (size_t i = 0; i < E->plans.size(); i++)
+                    if (E->plans[i] == P) {
+                        E->plans.erase(E->plans.begin() + (long)i);
+                        break;
+                    }
+                E->in_use -= P->footprint;
+            }
+            plan_free(P);
+        }
+    }
     return rc;
 }
 
@@ -211,12 +337,14 @@ void worker(CsEngine *E)
             E->cv_work.wait(lk, [&] {
                 if (E->queue.empty()) return E->stop;
                 const EngReq *h = E->queue.front();
+                if (E->in_use + h->footprint > E->budget) evict_idle(E, h->footprint);
                 return E->in_use == 0 || E->in_use + h->footprint <= E->budget;
             });
             if (E->queue.empty()) break;
             r = E->queue.front();
             E->queue.pop_front();
             E->in_use += r->footprint;
+            r->reserved = r->footprint;
             E->peak_in_use = std::max(E->peak_in_use, E->in_use);
             E->running++;
             E->peak_running = std::max(E->peak_running, E->running);
@@ -224,7 +352,6 @@ void worker(CsEngine *E)
             r->t_start = clk::now();
         }
         const int rc = st ? run_request(E, r, st) : -1;
-        if (st) hipStreamSynchronize(st);
         {
             std::lock_guard<std::mutex> lk(E->mu);
             r->t_end = clk::now();
@@ -234,7 +361,8 @@ void worker(CsEngine *E)
             r->info.run_ms = ms_between(r->t_start, r->t_end);
             r->info.latency_ms = ms_between(r->t_submit, r->t_end);
             r->state = 2;
-            E->in_use -= r->footprint;
+            E->in_use -= r->reserved;
+            r->reserved = 0;
             E->running--;
             E->completed++;
             E->done_order.push_back(r->id);
@@ -271,6 +399,11 @@ extern "C" CS_API CsEngine *cs_engine_create(int device, long long hbm_budget, i
     E->device = device;
     E->budget = hbm_budget;
     E->nslots = nslots > 0 ? nslots : 4;
+    if (hipStreamCreateWithFlags(&E->idle_st, hipStreamNonBlocking) != hipSuccess) {
+        fprintf(stderr, "Error: cannot create the engine's stream\n");
+        delete E;
+        return NULL;
+    }
     for (int i = 0; i < E->nslots; i++) E->workers.emplace_back(worker, E);
     return E;
 }
@@ -365,6 +498,8 @@ extern "C" CS_API void cs_engine_stats(CsEngine *E, CsEngineStats *s)
     s->peak_running = E->peak_running;
     s->queued = (long long)E->queue.size();
     s->completed = E->completed;
+    s->plan_hits = E->plan_hits;
+    s->plans_resident = (long long)E->plans.size();
 }
 
 extern "C" CS_API void cs_engine_free(CsEngine *E)
@@ -380,7 +515,22 @@ extern "C" CS_API void cs_engine_free(CsEngine *E)
         free(kv.second->out.v);
         delete kv.second;
     }
+    hipSetDevice(E->device);
+    for (Plan *P : E->plans) plan_free(P);
+    E->plans.clear();
+    if (E->idle_st) {
+        hipStreamSynchronize(E->idle_st);
+        hipStreamDestroy(E->idle_st);
+    }
     delete E;
+}
+
+extern "C" CS_API void cs_engine_set_plan_cache(CsEngine *E, int max_plans)
+{
+    if (!E) return;
+    std::lock_guard<std::mutex> lk(E->mu);
+    E->max_plans = max_plans > 0 ? max_plans : 0;
+    evict_idle(E, -1);
 }
 
 extern "C" CS_API void *cs_host_alloc(long long bytes)

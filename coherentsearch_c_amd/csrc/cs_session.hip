@@ -934,6 +934,25 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
 
 extern "C" CS_API void cs_gpu_session_free(GpuSession *S) { session_release(S); }
 
+/* Engine sessions: run on another worker's stream, replay launch graphs.
+ * The stream-ordered allocations stay valid on any stream of the device. */
+__attribute__((visibility("hidden"))) void cs_session_bind_stream(GpuSession *S, hipStream_t st,
+                                                                  bool graphs)
+{
+    if (!S || S->own_stream || !st) return;
+    S->st = st;
+    S->use_graph = graphs;
+}
+
+/* stage samples into the resident spectrum with the staging graph (engine
+ * plans reused across requests); no host copy, no synchronisation */
+__attribute__((visibility("hidden"))) int cs_session_stage_graph(GpuSession *S, CsStage *g,
+                                                                 const float *samples)
+{
+    if (!S || !g || !samples || cs_stage_N(g) != S->N || S->N / 2 != S->namps) return -1;
+    return cs_stage_enqueue_graph(g, samples, S->d_amps, S->st);
+}
+
 extern "C" CS_API int cs_gpu_session_set_graphs(GpuSession *S, int enable)
 {
     if (!S) return -1;

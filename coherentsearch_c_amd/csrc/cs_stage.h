@@ -31,6 +31,17 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
                                                                        float2 *d_out,
                                                                        hipStream_t st);
 
+/* the same, with the staging kernels replayed from a graph captured once per
+ * output buffer */
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue_graph(CsStage *g,
+                                                                             const float *samples,
+                                                                             float2 *d_out,
+                                                                             hipStream_t st);
+
+/* move a stage plan to another stream (its next operations run there) */
+extern "C" __attribute__((visibility("hidden"))) void cs_stage_bind_stream(CsStage *g,
+                                                                            hipStream_t st);
+
 extern "C" __attribute__((visibility("hidden"))) long long cs_stage_N(const CsStage *g);
 
 /* a stage plan on the caller's stream with stream-ordered allocations */
@@ -50,6 +61,15 @@ cs_session_create_on(const FFTFile *ft, const SearchParams *p, const DirectPlanS
  * empty with a message on stderr on failure */
 __attribute__((visibility("hidden"))) CandidateList
 cs_session_finalize(GpuSession *s, const CandidateList *in, int ncands);
+
+/* engine sessions: run on `st` from now on, replaying launch graphs when
+ * `graphs` (search graph keyed on threshold and record buffer) */
+__attribute__((visibility("hidden"))) void cs_session_bind_stream(GpuSession *s, hipStream_t st,
+                                                                  bool graphs);
+/* upload samples and stage them into the session's spectrum with the stage
+ * plan's graph, on the session's stream (not synchronised) */
+__attribute__((visibility("hidden"))) int cs_session_stage_graph(GpuSession *s, CsStage *g,
+                                                                 const float *samples);
 
 /* launch shape of the search kernel (no effect on results) */
 __attribute__((visibility("hidden"))) void cs_session_set_launch(GpuSession *s, int lane_order,

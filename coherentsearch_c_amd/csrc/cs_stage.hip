@@ -466,6 +466,9 @@ struct CsStage {
     hipStream_t st;
     int own_stream;
     bool async_alloc;             /* stream-ordered allocations on st */
+    hipGraph_t graph;             /* staging kernels captured for output g_out */
+    hipGraphExec_t gexec;
+    float2 *g_out;
 };
 
 static hipError_t salloc(CsStage *g, void **p, size_t bytes)
@@ -549,10 +552,18 @@ extern "C" __attribute__((visibility("hidden"))) long long cs_stage_N(const CsSt
     return g ? g->N : -1;
 }
 
+extern "C" __attribute__((visibility("hidden"))) void cs_stage_bind_stream(CsStage *g,
+                                                                            hipStream_t st)
+{
+    if (g && st && !g->own_stream) g->st = st;
+}
+
 extern "C" CS_API void cs_gpu_stage_free(CsStage *g)
 {
     if (!g) return;
     hipSetDevice(g->device);
+    if (g->gexec) hipGraphExecDestroy(g->gexec);
+    if (g->graph) hipGraphDestroy(g->graph);
     sfree(g, g->d_block);
     if (g->own_stream && g->st) hipStreamDestroy(g->st);
     delete g;
@@ -747,11 +758,9 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
 
 /* Enqueue staging of N host samples into d_out (float2[N/2]) on stream st
  * (the stage's own stream when st is null).  0 on success. */
-extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g, const float *samples,
-                                                                       float2 *d_out, hipStream_t st)
+/* the staging kernels (samples already in d_x) */
+static int stage_kernels(CsStage *g, float2 *d_out, hipStream_t st)
 {
-    if (!st) st = g->st;
-    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
     {
         const long long n2 = g->n2;
         to_complex_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, st>>>(g->d_x, g->d_z0, n2);
@@ -787,6 +796,49 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
                                                                   g->d_mean);
         SCHK(hipGetLastError());
     }
+    return 0;
+fail:
+    return -1;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g, const float *samples,
+                                                                       float2 *d_out, hipStream_t st)
+{
+    if (!st) st = g->st;
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    return stage_kernels(g, d_out, st);
+fail:
+    return -1;
+}
+
+/* The samples' upload, then the staging kernels replayed from a graph
+ * captured once per output buffer (kernels only).  0 on success. */
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue_graph(CsStage *g,
+                                                                             const float *samples,
+                                                                             float2 *d_out,
+                                                                             hipStream_t st)
+{
+    if (!st) st = g->st;
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    if (!g->gexec || g->g_out != d_out) {
+        if (g->gexec) hipGraphExecDestroy(g->gexec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        g->gexec = nullptr;
+        g->graph = nullptr;
+        SCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = stage_kernels(g, d_out, st);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &gr);
+        if (rc != 0 || e != hipSuccess) {
+            fprintf(stderr, "Error: staging graph capture failed\n");
+            if (e == hipSuccess && gr) hipGraphDestroy(gr);
+            return -1;
+        }
+        g->graph = gr;
+        SCHK(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
+        g->g_out = d_out;
+    }
+    SCHK(hipGraphLaunch(g->gexec, st));
     return 0;
 fail:
     return -1;

@@ -53,11 +53,15 @@ class PinnedSeries:
 
 
 class Engine:
-    def __init__(self, device=0, hbm_budget=0, nslots=4):
+    def __init__(self, device=0, hbm_budget=0, nslots=4, plan_cache=8):
+        """plan_cache: resident plans kept between requests (0: none); a
+        request with the geometry of an idle plan reuses it, replaying its
+        staging and search from graphs."""
         self.L = abi.host()
         self.h = self.L.cs_engine_create(device, int(hbm_budget), int(nslots))
         if not self.h:
             raise RuntimeError("cs_engine_create failed (see stderr)")
+        self.L.cs_engine_set_plan_cache(self.h, int(plan_cache))
         self._keep = {}
         self._lock = threading.Lock()
 

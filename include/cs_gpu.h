@@ -189,7 +189,8 @@ CS_API int cs_gpu_tune_load(const char *path);
  * admitted, oldest first, when its HBM footprint fits `hbm_budget` next to the
  * running ones (a request larger than the budget runs alone).  Each request
  * stages its raw series on the device (cs_gpu_stage_*) and searches it as
- * search_file would (src/search.c:1247-1406), returning the raw candidate
+ * search_file would (src/search.c:1247-1406) -- on a resident plan when an
+ * earlier request had the same geometry -- returning the raw candidate
  * list of cs_gpu_session_run, or the final list with duty cycles when the
  * request asks for it (the spectrum then never leaves the device). */
 typedef struct CsEngine CsEngine;
@@ -225,7 +226,7 @@ typedef struct {
 
 typedef struct {
     int status;             /* 0 ok, -1 failed (message on stderr) */
-    int pad;
+    int reused;             /* 1: ran on a resident plan of the same geometry */
     long long harmonic_trials, trials, footprint;
     double queue_ms;        /* submit -> admitted */
     double run_ms;          /* admitted -> done */
@@ -241,6 +242,8 @@ typedef struct {
     long long budget, in_use, peak_in_use;
     int nslots, running, peak_running, pad;
     long long queued, completed;
+    long long plan_hits;        /* requests that ran on a resident plan */
+    long long plans_resident;   /* plans kept now */
 } CsEngineStats;
 
 /* hbm_budget <= 0: 80 % of the device's free memory; nslots <= 0: 4 */
@@ -259,6 +262,11 @@ CS_API int cs_engine_wait(CsEngine *e, long long ticket, CandidateList *out,
 CS_API void cs_engine_stats(CsEngine *e, CsEngineStats *s);
 /* finishes every submitted request, then releases the engine */
 CS_API void cs_engine_free(CsEngine *e);
+/* resident plans kept between requests (default 8; 0 frees every plan when
+ * its request ends).  A request whose geometry -- N, dt, search options,
+ * band, chunk length -- matches an idle plan reuses it: no planning,
+ * allocation or table upload, staging and search replayed from graphs. */
+CS_API void cs_engine_set_plan_cache(CsEngine *e, int max_plans);
 
 /* pinned host memory for series and spectra (hipHostMalloc) */
 CS_API void *cs_host_alloc(long long bytes);

@@ -57,7 +57,8 @@ def test_struct_layouts():
     assert abi.CsSearchRequest.dr_tol.offset == 88 and abi.CsSearchRequest.harm_tol.offset == 96
     assert C.sizeof(abi.CsSearchInfo) == 96 and abi.CsSearchInfo.kernel_ms.offset == 56
     assert abi.CsSearchInfo.search_ms.offset == 80 and abi.CsSearchInfo.post_ms.offset == 88
-    assert C.sizeof(abi.CsEngineStats) == 56 and abi.CsEngineStats.completed.offset == 48
+    assert C.sizeof(abi.CsEngineStats) == 72 and abi.CsEngineStats.completed.offset == 48
+    assert abi.CsEngineStats.plans_resident.offset == 64 and abi.CsSearchInfo.reused.offset == 4
 
 
 def test_no_device_reports_error():

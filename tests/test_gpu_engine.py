@@ -169,3 +169,69 @@ def test_engine_final_lists_match_host_post():
             assert info_r["status"] == 0 and info_f["status"] == 0 and none is None
             say("final list", final.size, "post_ms %.3f" % info_f["post_ms"])
             _final_vs_host(spec, j["params"], final, raw, 50)
+
+
+def _dm_trials(n, N=1 << 19, seed=31):
+    """n series of one length (DM trials of one observation), nharms 16/32."""
+    jobs = []
+    for i in range(n):
+        x = synth.series_with_pulsar(N, 3.3e-3, 0.05, 10.0 + i, seed + i)
+        jobs.append(dict(samples=x, dt=synth.DT, threshold=6.0, blocksize=16384,
+                         params=cs.make_params(nharms=(16, 32)[i % 2], m=16, maxdecim=6)))
+    return jobs
+
+
+def test_resident_plans_reused_with_graphs(oracle):
+    """Requests of one geometry reuse the resident plan (no planning or
+    uploads; staging and search replayed from graphs, zaplist and spectrum
+    copies in between) and return the oracle's list of each staged spectrum,
+    bit for bit, as an engine without resident plans does."""
+    jobs = _dm_trials(6)
+    zap = [(61.0, 0.5, 0)]
+    res = {}
+    for cache in (8, 0):
+        with eng.Engine(nslots=1, plan_cache=cache) as E:
+            out = []
+            for i, j in enumerate(jobs):
+                t = E.submit(j["samples"], j["dt"], j["params"], threshold=j["threshold"],
+                             blocksize=j["blocksize"], zaplist=zap if i == 4 else None)
+                out.append(E.wait(t))
+            st = E.stats()
+        say("plan cache", cache, "stats", st, "reused", [o[1]["reused"] for o in out])
+        res[cache] = out
+        if cache:
+            assert [o[1]["reused"] for o in out] == [0, 0, 1, 1, 1, 1]
+            assert st["plan_hits"] == 4 and st["plans_resident"] == 2
+        else:
+            assert st["plan_hits"] == 0 and st["plans_resident"] == 0
+    for i, ((a, ia, sa), (b, ib, sb)) in enumerate(zip(res[8], res[0])):
+        assert ia["status"] == 0 and ib["status"] == 0
+        assert np.array_equal(sa.amps, sb.amps) and _same(a, b)
+        p = jobs[i]["params"]
+        o = oracle.Oracle(sa.amps, sa.N, sa.T,
+                          oracle.config(nharms=p.nharms, m=16, blocksize=16384))
+        want = o.search(6.0, 1)
+        assert a.size == want.size and np.array_equal(a["metric"], want["metric"])
+        o.close()
+
+
+def test_resident_plans_in_the_mix():
+    """Eight requests of two geometries on four slots: several plans of one
+    geometry may exist at once; every list equals the one-slot, no-cache run."""
+    jobs = _dm_trials(8, seed=41)
+    with eng.Engine(nslots=1, plan_cache=0) as E:
+        alone = [E.wait(_submit(E, j))[0] for j in jobs]
+    with eng.Engine(nslots=4, plan_cache=4) as E:
+        for _ in range(2):
+            tk = [_submit(E, j) for j in jobs]
+            mix = [E.wait(t)[0] for t in tk]
+            for a, b in zip(alone, mix):
+                assert _same(a, b)
+        st = E.stats()
+    say("mix with plans:", st)
+    assert st["plan_hits"] >= 8 and st["plans_resident"] <= 4
+
+
+def _submit(E, j):
+    return E.submit(j["samples"], j["dt"], j["params"], threshold=j["threshold"],
+                    blocksize=j["blocksize"])
