@@ -138,6 +138,38 @@ def dm_sweep(nfiles, slots, host_files=3):
     }
 
 
+def dm_sweep_resident(nfiles, slots, N=1 << 23):
+    """The DM trials of one observation: `nfiles` series of N samples, nharms
+    cycling 16/32/60 (three geometries).  Batched with resident plans (each
+    geometry's session and stage plan reused, staging and search replayed
+    from graphs) against one file at a time with a fresh plan per file and
+    one file at a time with resident plans."""
+    from coherentsearch_c_amd import engine as eng
+    jobs = eng.sweep_jobs(nfiles, seed=6, same_length=N)
+    with eng.Engine(nslots=1, plan_cache=0) as E:        # warm: code objects, pool
+        eng.run_plain_device(E, jobs[:2])
+    with eng.Engine(nslots=slots, plan_cache=8) as E:
+        fb, ib, wb = eng.run_batched(E, jobs)
+        st = E.stats()
+    ht = sum(i["harmonic_trials"] for i in ib)
+    with eng.Engine(nslots=1, plan_cache=0) as E:
+        fp, _, wp = eng.run_plain_device(E, jobs)
+    with eng.Engine(nslots=1, plan_cache=8) as E:
+        fr, _, wr = eng.run_plain_device(E, jobs)
+    log("dm sweep, %d x %d samples: batched+resident %.3f s (%.4g ht/s), plain %.3f s, "
+        "plain+resident %.3f s" % (nfiles, N, wb, ht / wb, wp, wr))
+    return {
+        "files": nfiles, "samples_per_file": N, "harmonic_trials": int(ht), "slots": slots,
+        "batched_resident_s": wb, "batched_resident_ht_s": ht / wb,
+        "plain_fresh_plans_s": wp, "plain_fresh_plans_ht_s": ht / wp,
+        "plain_resident_s": wr, "plain_resident_ht_s": ht / wr,
+        "ratio_vs_plain_fresh": wp / wb, "ratio_vs_plain_resident": wr / wb,
+        "plan_hits": st["plan_hits"], "mean_plan_ms_batched": float(np.mean([i["plan_ms"] for i in ib])),
+        "lists_identical": all(_same_lists(a, b) and _same_lists(a, c)
+                               for a, b, c in zip(fb, fp, fr)),
+    }
+
+
 def serving(nreq, slots, rate=20.0):
     """The served stream: Poisson arrivals at `rate`/s, continuous batching
     on the engine; latency = arrival -> collapsed list with duty cycles (each
@@ -304,6 +336,7 @@ def main():
                            "classes": tune}
     if not args.no_sweep:
         out["dm_sweep"] = dm_sweep(args.sweep_files, args.slots)
+        out["dm_sweep_one_observation"] = dm_sweep_resident(args.sweep_files, args.slots)
     if not args.no_serve:
         out["serving"] = serving(args.serve_requests, args.slots)
     if not args.no_cpu_baseline:

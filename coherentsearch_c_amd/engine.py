@@ -157,15 +157,16 @@ def _submit_job(engine, j, ncands=0):
                          out=j.get("out") if ncands <= 0 else None, ncands=ncands)
 
 
-def sweep_jobs(nfiles=64, seed=4, pinned=True, threshold=8.0, lo=22, hi=25):
+def sweep_jobs(nfiles=64, seed=4, pinned=True, threshold=8.0, lo=22, hi=25, same_length=None):
     """The batched DM sweep: `nfiles` series, lengths log-uniform in
     [2^lo, 2^hi] rounded up to even 7-smooth sizes, per-file seeded N(0,1)
     noise, a 3.7 ms pulsar in 8 files around the middle DM (S/N peaking at
-    18), nharms cycling 16/32/60, m 16, maxdecim 6, blocksize 65536.  With
-    `pinned`, series and spectrum buffers are pinned host memory."""
+    18), nharms cycling 16/32/60, m 16, maxdecim 6, blocksize 65536 (all of
+    length `same_length` when given).  With `pinned`, series and spectrum
+    buffers are pinned host memory."""
     from . import synth
     jobs = []
-    for m in synth.dm_sweep(nfiles, seed, lo=lo, hi=hi):
+    for m in synth.dm_sweep(nfiles, seed, lo=lo, hi=hi, same_length=same_length):
         x = synth.series_with_pulsar(m["N"], m["period"], m["duty"], m["snr"], m["seed"])
         s, o = x, None
         if pinned:

@@ -95,12 +95,15 @@ def smooth7(n):
         m += 1
 
 
-def dm_sweep(nfiles=64, seed=4, lo=22, hi=25, npulsar=8):
-    """Lengths log-uniform in [2^lo, 2^hi] rounded up to even 7-smooth sizes;
-    a 3.7 ms pulsar in the npulsar files around the middle DM with S/N
+def dm_sweep(nfiles=64, seed=4, lo=22, hi=25, npulsar=8, same_length=None):
+    """Lengths log-uniform in [2^lo, 2^hi] rounded up to even 7-smooth sizes
+    (or all `same_length`, the DM trials of one observation); a 3.7 ms pulsar
+    in the npulsar files around the middle DM with S/N
     18 * exp(-0.5 (d/2)^2) at DM offset d; nharms cycles 16, 32, 60."""
     rng = np.random.default_rng(seed)
     lens = [smooth7(int(2 ** rng.uniform(lo, hi))) for _ in range(nfiles)]
+    if same_length:
+        lens = [int(same_length)] * nfiles
     mid = nfiles // 2
     out = []
     for i, N in enumerate(lens):
