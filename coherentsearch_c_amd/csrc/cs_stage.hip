@@ -39,14 +39,21 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kLmax = 512;        /* longest sub-FFT of one level */
 constexpr int kCL = 2048;         /* columns x length held in LDS (32 KB of double2) */
+constexpr int kEPT = kCL / kThreads;  /* elements each thread moves to and from HBM */
 constexpr int kMaxLevels = 4;
 constexpr int kMaxStages = 10;
 
 struct LevelArgs {
     int L;                        /* this level's length */
-    int C;                        /* sub-FFTs per workgroup */
+    int C;                        /* sub-FFTs per workgroup, a power of two */
+    int lc;                       /* log2 C */
+    int ldr;                      /* LDS row stride: C, or C + 1 on the last level, whose
+                                   * loads run down the columns */
     int nrad;
     int rad[kMaxStages];
+    unsigned nsmag[kMaxStages];   /* ceil(2^32 / Ns) before each stage (0 while Ns = 1) */
+    unsigned Lmag;                /* ceil(2^32 / L) (0 when L = 1) */
+    unsigned nic;                 /* column groups per block, (S + C - 1) / C */
     long long S;                  /* product of the later levels (1 on the last) */
     long long B;                  /* product of the earlier levels */
     int twoff;                    /* offset of exp(-2 pi i t/L), t < L, in the table */
@@ -55,6 +62,13 @@ struct LevelArgs {
     int dL[kMaxLevels];
     long long dstride[kMaxLevels];
 };
+
+/* x / d for x < 2^21 and 2 <= d <= 2^10 from m = ceil(2^32 / d): x*m/2^32 exceeds
+ * x/d by less than 2^-11, while the fractional part of x/d is at most 1 - 1/d */
+__device__ __forceinline__ unsigned fdiv(unsigned x, unsigned m, unsigned d)
+{
+    return d == 1 ? x : __umulhi(x, m);
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b)
 {
@@ -110,15 +124,19 @@ __device__ __forceinline__ void dft_r(double2 (&v)[R], const double2 *__restrict
     }
 }
 
-/* one Stockham stage of radix R over the C columns in LDS (layout a*C + c):
+/* one Stockham stage of radix R over the C columns in LDS (row stride ldr):
  * butterfly j of column c reads a = j + r*L/R, twiddles by
- * exp(-2 pi i r (j mod Ns)/(Ns R)) and writes a = (j/Ns)*Ns*R + j mod Ns + r*Ns */
+ * exp(-2 pi i r (j mod Ns)/(Ns R)) -- table index r*(j mod Ns)*L/(Ns R) < L --
+ * and writes a = (j/Ns)*Ns*R + j mod Ns + r*Ns */
 template <int R>
-__device__ void stage(double2 *lds, int L, int C, int Ns, const double2 *__restrict__ tw)
+__device__ void stage(double2 *lds, const LevelArgs &a, int Ns, unsigned nsmag,
+                      const double2 *__restrict__ tw)
 {
     constexpr int BPT = (kCL / R + kThreads - 1) / kThreads;
-    const int nb = (L / R) * C;
+    const int L = a.L, lc = a.lc, ldr = a.ldr;
+    const int nb = (L / R) << lc;
     const int LR = L / R;
+    const int tstride = L / (Ns * R);
     double2 v[BPT][R];
     int dst[BPT];
 #pragma unroll
@@ -126,16 +144,17 @@ __device__ void stage(double2 *lds, int L, int C, int Ns, const double2 *__restr
         const int u = threadIdx.x + q * kThreads;
         dst[q] = -1;
         if (u < nb) {
-            const int c = u % C, j = u / C;
-            const int k = j % Ns;
+            const int c = u & ((1 << lc) - 1), j = u >> lc;
+            const int jq = (int)fdiv((unsigned)j, nsmag, (unsigned)Ns);
+            const int k = j - jq * Ns;
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                double2 x = lds[(j + r * LR) * C + c];
-                if (r > 0 && k > 0) x = cmul(x, tw[(long long)r * k * (L / (Ns * R)) % L]);
+                double2 x = lds[(j + r * LR) * ldr + c];
+                if (r > 0 && k > 0) x = cmul(x, tw[r * k * tstride]);
                 v[q][r] = x;
             }
             dft_r<R>(v[q], tw, L / R);
-            dst[q] = ((j / Ns) * Ns * R + k) * C + c;
+            dst[q] = (jq * Ns * R + k) * ldr + c;
         }
     }
     __syncthreads();
@@ -143,7 +162,7 @@ __device__ void stage(double2 *lds, int L, int C, int Ns, const double2 *__restr
     for (int q = 0; q < BPT; q++) {
         if (dst[q] >= 0) {
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[dst[q] + r * Ns * C] = v[q][r];
+            for (int r = 0; r < R; r++) lds[dst[q] + r * Ns * ldr] = v[q][r];
         }
     }
     __syncthreads();
@@ -167,83 +186,124 @@ __device__ __forceinline__ void level_stages(double2 *lds, const LevelArgs &a,
     int Ns = 1;
     for (int s = 0; s < a.nrad; s++) {
         const int R = a.rad[s];
-        if (R == 8) stage<8>(lds, a.L, a.C, Ns, tw);
-        else if (R == 4) stage<4>(lds, a.L, a.C, Ns, tw);
-        else if (R == 2) stage<2>(lds, a.L, a.C, Ns, tw);
+        const unsigned m = a.nsmag[s];
+        if (R == 8) stage<8>(lds, a, Ns, m, tw);
+        else if (R == 4) stage<4>(lds, a, Ns, m, tw);
+        else if (R == 2) stage<2>(lds, a, Ns, m, tw);
         else if constexpr (!POW2) {
-            if (R == 3) stage<3>(lds, a.L, a.C, Ns, tw);
-            else if (R == 5) stage<5>(lds, a.L, a.C, Ns, tw);
-            else stage<7>(lds, a.L, a.C, Ns, tw);
+            if (R == 3) stage<3>(lds, a, Ns, m, tw);
+            else if (R == 5) stage<5>(lds, a, Ns, m, tw);
+            else stage<7>(lds, a, Ns, m, tw);
         }
         Ns *= R;
     }
 }
 
-template <bool POW2>
-__global__ __launch_bounds__(kThreads) void fft_level_kernel(const double2 *__restrict__ in,
+/* element idx of the level's input: the complex sequence z, or on the first
+ * level the float series read as (x[2 idx], x[2 idx + 1]) */
+template <bool FLOATIN>
+__device__ __forceinline__ double2 load_in(const void *__restrict__ in, long long idx)
+{
+    if constexpr (FLOATIN) {
+        const float2 v = ((const float2 *)in)[idx];
+        return make_double2((double)v.x, (double)v.y);
+    } else {
+        return ((const double2 *)in)[idx];
+    }
+}
+
+/* One level of the multi-level transform.  Every thread issues all of its
+ * kEPT loads before the first LDS store, so a wave keeps kEPT 16-byte loads
+ * in flight per lane. */
+template <bool POW2, bool FLOATIN, bool LAST>
+__global__ __launch_bounds__(kThreads) void fft_level_kernel(const void *__restrict__ in,
                                                              double2 *__restrict__ out,
                                                              const double2 *__restrict__ twall,
-                                                             LevelArgs a, int last)
+                                                             LevelArgs a)
 {
-    __shared__ double2 lds[kCL];
-    const int L = a.L, C = a.C;
+    extern __shared__ double2 lds[];          /* L * ldr elements */
+    const int L = a.L, lc = a.lc, C = 1 << lc, ldr = a.ldr, n = L << lc;
     const double2 *tw = twall + a.twoff;
     const int tid = threadIdx.x;
-    if (!last) {
-        /* block b, columns i0 .. i0+C-1 of the S inner indices */
-        const long long nic = (a.S + C - 1) / C;
-        const long long b = blockIdx.x / nic;
-        const long long i0 = (blockIdx.x % nic) * C;
-        const long long base = b * (long long)L * a.S;
-        for (int e = tid; e < L * C; e += kThreads) {
-            const int ai = e / C, c = e % C;
-            const long long i = i0 + c;
-            lds[ai * C + c] = i < a.S ? in[base + ai * a.S + i] : make_double2(0.0, 0.0);
+    double2 r[kEPT];
+    if constexpr (!LAST) {
+        /* block b, columns i0 .. i0+C-1 of the S inner indices; element
+         * e = ai*C + c is row ai of column c, at LDS e */
+        const unsigned b = blockIdx.x / a.nic;
+        const long long i0 = (long long)(blockIdx.x - b * a.nic) << lc;
+        const long long S = a.S;
+        const long long base = (long long)b * L * S;
+        const int c = tid & (C - 1);
+        const long long i = i0 + c;
+        const bool col = i < S;
+#pragma unroll
+        for (int q = 0; q < kEPT; q++) {
+            const int e = tid + q * kThreads;
+            r[q] = make_double2(0.0, 0.0);
+            if (e < n && col) r[q] = load_in<FLOATIN>(in, base + (long long)(e >> lc) * S + i);
+        }
+#pragma unroll
+        for (int q = 0; q < kEPT; q++) {
+            const int e = tid + q * kThreads;
+            if (e < n) lds[e] = r[q];
         }
         __syncthreads();
         level_stages<POW2>(lds, a, tw);
-        const long long LS = (long long)L * a.S;
-        for (int e = tid; e < L * C; e += kThreads) {
-            const int k = e / C, c = e % C;
-            const long long i = i0 + c;
-            if (i < a.S) {
-                double2 x = lds[k * C + c];
-                const long long m = (i * k) % LS;
-                if (m) x = cmul(x, phase(m, LS));
-                out[base + k * a.S + i] = x;
+        /* output row k of column i, twiddled by exp(-2 pi i i*k/(L S)); this
+         * thread's rows are k0 + q*dk, so the twiddle advances by a fixed step */
+        const long long LS = (long long)L * S;
+        const int k0 = tid >> lc, dk = kThreads >> lc;
+        double2 w = phase(i * k0 % LS, LS);
+        const double2 step = phase(i * dk % LS, LS);
+#pragma unroll
+        for (int q = 0; q < kEPT; q++) {
+            const int e = tid + q * kThreads;
+            if (e < n && col) {
+                const int k = e >> lc;
+                double2 x = lds[e];
+                if (i * k) x = cmul(x, w);
+                out[base + (long long)k * S + i] = x;
             }
+            w = cmul(w, step);
         }
     } else {
         /* transposed indices r0 .. r0+C-1; r's digits give the block b */
-        const long long r0 = (long long)blockIdx.x * C;
+        const long long r0 = (long long)blockIdx.x << lc;
         __shared__ long long sb[16];
         if (tid < C) {
-            long long r = r0 + tid, b = 0;
+            long long rr = r0 + tid, b = 0;
             for (int d = 0; d < a.nd; d++) {
-                b += (r % a.dL[d]) * a.dstride[d];
-                r /= a.dL[d];
+                b += (rr % a.dL[d]) * a.dstride[d];
+                rr /= a.dL[d];
             }
             sb[tid] = (r0 + tid < a.B) ? b : -1;
         }
         __syncthreads();
-        for (int e = tid; e < L * C; e += kThreads) {
-            const int c = e / L, ai = e % L;
-            const long long b = sb[c];
-            lds[ai * C + c] = b >= 0 ? in[b * L + ai] : make_double2(0.0, 0.0);
+        int dst[kEPT];
+#pragma unroll
+        for (int q = 0; q < kEPT; q++) {
+            const int e = tid + q * kThreads;
+            r[q] = make_double2(0.0, 0.0);
+            dst[q] = -1;
+            if (e < n) {
+                const int c = (int)fdiv((unsigned)e, a.Lmag, (unsigned)L), ai = e - c * L;
+                const long long b = sb[c];
+                if (b >= 0) r[q] = load_in<FLOATIN>(in, b * L + ai);
+                dst[q] = ai * ldr + c;
+            }
         }
+#pragma unroll
+        for (int q = 0; q < kEPT; q++)
+            if (dst[q] >= 0) lds[dst[q]] = r[q];
         __syncthreads();
         level_stages<POW2>(lds, a, tw);
-        for (int e = tid; e < L * C; e += kThreads) {
-            const int k = e / C, c = e % C;
-            if (r0 + c < a.B) out[r0 + c + a.B * k] = lds[k * C + c];
+#pragma unroll
+        for (int q = 0; q < kEPT; q++) {
+            const int e = tid + q * kThreads;
+            const int k = e >> lc, c = e & (C - 1);
+            if (e < n && r0 + c < a.B) out[r0 + c + a.B * k] = lds[k * ldr + c];
         }
     }
-}
-
-__global__ void to_complex_kernel(const float *__restrict__ x, double2 *__restrict__ z, long long n)
-{
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) z[i] = make_double2((double)x[2 * i], (double)x[2 * i + 1]);
 }
 
 /* NR unpack of the N/2-point transform W into the packed real spectrum d */
@@ -301,6 +361,37 @@ __device__ __forceinline__ void block_of(const BlockLayout &L, int b, long long 
     }
 }
 
+/* ascending bitonic sort of 128 keys held as (k0, k1) = keys lane, lane + 64 */
+__device__ __forceinline__ void bitonic128(unsigned &k0, unsigned &k1, int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= 128; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j == 64) {
+                const unsigned lo = min(k0, k1), hi = max(k0, k1);
+                k0 = lo;
+                k1 = hi;
+            } else {
+                const bool lower = (lane & j) == 0;
+                const unsigned y0 = (unsigned)__shfl_xor((int)k0, j, 64);
+                const unsigned y1 = (unsigned)__shfl_xor((int)k1, j, 64);
+                /* element i = lane + 64 r sorts ascending within its k-run
+                 * when bit k of i is clear */
+                const bool up0 = (lane & k) == 0;
+                const bool up1 = ((lane + 64) & k) == 0;
+                k0 = (lower == up0) ? min(k0, y0) : max(k0, y0);
+                k1 = (lower == up1) ? min(k1, y1) : max(k1, y1);
+            }
+        }
+    }
+}
+
+/* Lower median of each block's powers, mean = (float)(median / ln 2): element
+ * (n-1)/2 of the block's powers in ascending order.  Powers are sums of
+ * squares, >= +0 (or NaN), so their bit patterns order as unsigned integers
+ * (a NaN above every number, as a comparison-based rank places it).  Blocks of
+ * up to 128 bins sort in one wave's registers; longer ones rank in LDS. */
 __global__ __launch_bounds__(64) void block_mean_kernel(const float2 *__restrict__ d,
                                                         BlockLayout L, float *__restrict__ mean)
 {
@@ -310,14 +401,34 @@ __global__ __launch_bounds__(64) void block_mean_kernel(const float2 *__restrict
     long long s;
     int n;
     block_of(L, b, s, n);
-    for (int e = threadIdx.x; e < n; e += 64) {
+    const int lane = threadIdx.x;
+    const int t = (n - 1) / 2;
+    if (n <= 128) {
+        unsigned k0 = 0xffffffffu, k1 = 0xffffffffu;
+        if (lane < n) {
+            const float2 v = d[s + lane];
+            const float a = v.x * v.x, c = v.y * v.y;
+            k0 = __float_as_uint(a + c);
+        }
+        if (lane + 64 < n) {
+            const float2 v = d[s + lane + 64];
+            const float a = v.x * v.x, c = v.y * v.y;
+            k1 = __float_as_uint(a + c);
+        }
+        bitonic128(k0, k1, lane);
+        if (lane == (t & 63)) {
+            const float x = __uint_as_float(t < 64 ? k0 : k1);
+            mean[b] = (float)((double)x / 0.69314718055994530942);
+        }
+        return;
+    }
+    for (int e = lane; e < n; e += 64) {
         const float2 v = d[s + e];
         const float a = v.x * v.x, c = v.y * v.y;
         p[e] = a + c;
     }
     __syncthreads();
-    const int t = (n - 1) / 2;
-    for (int e = threadIdx.x; e < n; e += 64) {
+    for (int e = lane; e < n; e += 64) {
         const float x = p[e];
         int lt = 0, eqb = 0;
         for (int f = 0; f < n; f++) {
@@ -409,7 +520,8 @@ bool factor7(long long n, std::vector<int> &pf)
     return n == 1;
 }
 
-/* split the length into levels of at most kLmax, as balanced as possible */
+/* split the length into levels of at most kLmax, as balanced as possible,
+ * in ascending order */
 bool plan_levels(long long n, std::vector<int> &Ls)
 {
     std::vector<int> pf;
@@ -426,6 +538,10 @@ bool plan_levels(long long n, std::vector<int> &Ls)
             prod[best] *= p;
         }
         if (ok) {
+            /* shortest level first: the first level reads the float series,
+             * and a short level holds the most columns per workgroup, so its
+             * row segments are the widest */
+            std::sort(prod.begin(), prod.end());
             Ls.assign(prod.begin(), prod.end());
             return true;
         }
@@ -626,15 +742,25 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
         for (int l = 0; l < g->nlev; l++) {
             LevelArgs &a = g->lev[l];
             a.L = Ls[l];
-            a.C = std::max(1, std::min(16, kCL / a.L));
+            a.lc = 0;
+            while ((2 << a.lc) <= std::min(16, kCL / a.L)) a.lc++;
+            a.C = 1 << a.lc;
+            a.ldr = l == g->nlev - 1 ? a.C + 1 : a.C;
             std::vector<int> r;
             radices(a.L, r);
             a.nrad = (int)r.size();
-            for (int i = 0; i < a.nrad; i++) a.rad[i] = r[i];
+            int Ns = 1;
+            for (int i = 0; i < a.nrad; i++) {
+                a.rad[i] = r[i];
+                a.nsmag[i] = Ns == 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + Ns - 1) / Ns);
+                Ns *= r[i];
+            }
+            a.Lmag = a.L == 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + a.L - 1) / a.L);
             long long S = 1;
             for (int q = l + 1; q < g->nlev; q++) S *= Ls[q];
             a.S = S;
             a.B = B;
+            a.nic = (unsigned)((S + a.C - 1) / a.C);
             a.twoff = (int)tw.size();
             for (int t = 0; t < a.L; t++) {
                 const double ang = -2.0 * M_PI * (double)t / (double)a.L;
@@ -762,21 +888,26 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
 static int stage_kernels(CsStage *g, float2 *d_out, hipStream_t st)
 {
     {
-        const long long n2 = g->n2;
-        to_complex_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, st>>>(g->d_x, g->d_z0, n2);
-        SCHK(hipGetLastError());
-        double2 *src = g->d_z0, *dst = g->d_z1;
+        double2 *src = nullptr, *dst = g->d_z0;
         for (int l = 0; l < g->nlev; l++) {
-            const int last = l == g->nlev - 1;
-            const int L = g->lev[l].L;
-            if ((L & (L - 1)) == 0)
-                fft_level_kernel<true><<<(unsigned)g->grid[l], kThreads, 0, st>>>(
-                    src, dst, g->d_tw, g->lev[l], last);
-            else
-                fft_level_kernel<false><<<(unsigned)g->grid[l], kThreads, 0, st>>>(
-                    src, dst, g->d_tw, g->lev[l], last);
+            const LevelArgs &a = g->lev[l];
+            const bool pow2 = (a.L & (a.L - 1)) == 0, first = l == 0, last = l == g->nlev - 1;
+            const void *in = first ? (const void *)g->d_x : (const void *)src;
+            const size_t lds = (size_t)a.L * a.ldr * sizeof(double2);
+            const dim3 grid((unsigned)g->grid[l]);
+#define CS_LEVEL(P, F, Z) \
+    fft_level_kernel<P, F, Z><<<grid, kThreads, lds, st>>>(in, dst, g->d_tw, a)
+            if (pow2) {
+                if (first) { if (last) CS_LEVEL(true, true, true); else CS_LEVEL(true, true, false); }
+                else { if (last) CS_LEVEL(true, false, true); else CS_LEVEL(true, false, false); }
+            } else {
+                if (first) { if (last) CS_LEVEL(false, true, true); else CS_LEVEL(false, true, false); }
+                else { if (last) CS_LEVEL(false, false, true); else CS_LEVEL(false, false, false); }
+            }
+#undef CS_LEVEL
             SCHK(hipGetLastError());
-            std::swap(src, dst);
+            src = dst;
+            dst = dst == g->d_z0 ? g->d_z1 : g->d_z0;
         }
         const long long nq = g->N >> 2;
         const long long nthr = nq > 0 ? nq : 1;
