@@ -142,7 +142,7 @@ HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "measure_ducy", "write_candidates", "cs_format_double",
                 "read_candidates_file", "inf_read", "inf_free", "zaplist_read",
                 "zaplist_free", "zaplist_apply", "fftfile_open",
-                "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_redfft",
+                "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_set_threads", "datpipe_redfft",
                 "datpipe_redfft_mem", "cs_open_input", "cs_write_inf", "cs_write_floats",
                 "search_file", "search_cache_free", "cs_main", "cs_main_list"]
 
@@ -195,6 +195,8 @@ def _declare(L):
     L.datpipe_create.argtypes = [I, I, D]
     L.datpipe_create.restype = P
     L.datpipe_free.argtypes = [P]
+    L.datpipe_set_threads.argtypes = [P, I]
+    L.datpipe_set_threads.restype = None
     L.datpipe_redfft_mem.argtypes = [P, P, LL, D, C.c_char_p, Ps(FFTFile)]
     L.datpipe_redfft_mem.restype = I
     L.cs_write_inf.argtypes = [C.c_char_p, LL, D, D, C.c_char_p]

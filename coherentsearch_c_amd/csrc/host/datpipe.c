@@ -10,6 +10,8 @@
  */
 #include "host_internal.h"
 
+#include <omp.h>
+
 #ifndef M_PI
 #define M_PI 3.141592653589793238462643383279502884
 #endif
@@ -17,7 +19,9 @@
 struct DatPipe {
     long long N;
     float *red;          /* N floats = N/2 complex, rednoise output */
-    double *work;        /* N doubles, complex FFT scratch */
+    double *work;        /* N doubles, the complex transform */
+    double *scratch;     /* N doubles, the long transform's second buffer */
+    int threads;         /* OpenMP threads of the transform (0: the default) */
     float *packed;       /* N floats, realfft output */
     int startwidth, endwidth;
     double endfreq;
@@ -63,11 +67,15 @@ static float nr_median(float *arr, int n)
     }
 }
 
-static int realfft_forward(const float *x, long long N, double *work, float *d)
+static int realfft_forward(const float *x, long long N, double *work, double *scratch,
+                           int threads, float *d)
 {
     const long long n2 = N / 2;
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+    #pragma omp parallel for num_threads(nt) schedule(static)
     for (long long i = 0; i < N; i++) work[i] = (double)x[i];
-    if (cs_fft_any(work, n2, -1) != 0) return -1;
+    if (cs_fft_work(work, scratch, n2, -1, threads) != 0) return -1;
+    #pragma omp parallel for num_threads(nt) schedule(static)
     for (long long i = 0; i < N; i++) d[i] = (float)work[i];
     const double cc = -0.5, theta = -2.0 * M_PI / (double)N;
     double wr = cos(theta), wi = sin(theta), tmp = sin(0.5 * theta);
@@ -92,86 +100,107 @@ static int realfft_forward(const float *x, long long N, double *work, float *d)
     return 0;
 }
 
+/* rednoise (src/datpipe.c:143-253) in three passes with the same arithmetic as
+ * the reference's single sweep: the block layout (it depends only on N, dt and
+ * the widths), every block's median power over threads, then every output
+ * segment -- from the middle of one block to the middle of the next,
+ * normalised by invsqrtf(mean_old + dslope*ii) -- over threads. */
 static int rednoise(const float *in, float *out, long long nc, long long N, double dt,
-                    int startwidth, int endwidth, double endfreq)
+                    int startwidth, int endwidth, double endfreq, int threads)
 {
-    float powbuf[1024], oldb[2048], newb[2048], outb[2048];
     if (endwidth > 1024) return -1;
     const float T = (float)((double)N * dt);
-    long long binnum = 1, numwrote = 1, pos = 1;
-    out[0] = 1.0f;
-    out[1] = 0.0f;
     int bufflen = startwidth;
     if (nc - 1 < bufflen) {
         fprintf(stderr, "Error: rednoise: too few Fourier bins (%lld)\n", nc);
         return -1;
     }
-    int nold = bufflen, nnew, mid_new, ii, ind;
-    memcpy(oldb, in + 2 * pos, sizeof(float) * 2 * (size_t)nold);
-    pos += nold;
-    int mid_old = nold / 2;
-    for (ii = 0; ii < nold; ii++)
-        powbuf[ii] = oldb[2 * ii] * oldb[2 * ii] + oldb[2 * ii + 1] * oldb[2 * ii + 1];
-    float mean_old = (float)(nr_median(powbuf, nold) / log(2.0)), mean_new;
-    float dslope = 1.0f, norm = fast_invsqrtf(mean_old);
-    for (ii = 0; ii < mid_old; ii++) {
-        outb[2 * ii] = oldb[2 * ii] * norm;
-        outb[2 * ii + 1] = oldb[2 * ii + 1] * norm;
-    }
-    memcpy(out + 2 * numwrote, outb, sizeof(float) * 2 * (size_t)mid_old);
-    numwrote += mid_old;
-    binnum += nold;
-    bufflen = (int)(startwidth * log((double)binnum));
-    for (;;) {
-        const long long avail = nc - pos;
-        if (avail <= 0) break;
-        nnew = avail < bufflen ? (int)avail : bufflen;
-        memcpy(newb, in + 2 * pos, sizeof(float) * 2 * (size_t)nnew);
-        pos += nnew;
-        mid_new = nnew / 2;
-        for (ii = 0; ii < nnew; ii++)
-            powbuf[ii] = newb[2 * ii] * newb[2 * ii] + newb[2 * ii + 1] * newb[2 * ii + 1];
-        mean_new = (float)(nr_median(powbuf, nnew) / log(2.0));
-        dslope = (float)((mean_new - mean_old) / (0.5 * (nold + nnew)));
-        ii = 0;
-        for (ind = mid_old; ind < nold; ii++, ind++) {
-            norm = fast_invsqrtf(mean_old + dslope * (float)ii);
-            outb[2 * ii] = oldb[2 * ind] * norm;
-            outb[2 * ii + 1] = oldb[2 * ind + 1] * norm;
+    long long cap = 1024, nb = 0;
+    long long *bs = (long long *)malloc((size_t)cap * sizeof(long long));
+    int *bl = (int *)malloc((size_t)cap * sizeof(int));
+    float *mean = NULL;
+    int rc = -1;
+    if (!bs || !bl) goto done;
+    {
+        long long pos = 1, binnum = 1;
+        int len = bufflen;                 /* block 0 is startwidth bins */
+        for (;;) {
+            if (nb == cap) {
+                cap *= 2;
+                long long *nbs = (long long *)realloc(bs, (size_t)cap * sizeof(long long));
+                if (!nbs) goto done;
+                bs = nbs;
+                int *nbl = (int *)realloc(bl, (size_t)cap * sizeof(int));
+                if (!nbl) goto done;
+                bl = nbl;
+            }
+            bs[nb] = pos;
+            bl[nb] = len;
+            nb++;
+            pos += len;
+            binnum += len;
+            if (nb == 1 || (float)binnum / T < endfreq)
+                bufflen = (int)(startwidth * log((double)binnum));
+            else
+                bufflen = endwidth;
+            if (nb > 1 && bufflen > endwidth) {
+                fprintf(stderr, "Error: rednoise block length %d exceeds buffer %d\n", bufflen,
+                        endwidth);
+                goto done;
+            }
+            const long long avail = nc - pos;
+            if (avail <= 0) break;
+            len = avail < bufflen ? (int)avail : bufflen;
+            if (len < 1 || len > 1024) {
+                fprintf(stderr, "Error: rednoise block length %d out of range\n", len);
+                goto done;
+            }
         }
-        for (ind = 0; ind < mid_new; ii++, ind++) {
-            norm = fast_invsqrtf(mean_old + dslope * (float)ii);
-            outb[2 * ii] = newb[2 * ind] * norm;
-            outb[2 * ii + 1] = newb[2 * ind + 1] * norm;
+    }
+    mean = (float *)malloc((size_t)nb * sizeof(float));
+    if (!mean) goto done;
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+    #pragma omp parallel for num_threads(nt) schedule(static)
+    for (long long b = 0; b < nb; b++) {
+        float powbuf[1024];
+        const float *v = in + 2 * bs[b];
+        for (int ii = 0; ii < bl[b]; ii++)
+            powbuf[ii] = v[2 * ii] * v[2 * ii] + v[2 * ii + 1] * v[2 * ii + 1];
+        mean[b] = (float)(nr_median(powbuf, bl[b]) / log(2.0));
+    }
+    out[0] = 1.0f;
+    out[1] = 0.0f;
+    {
+        const float norm = fast_invsqrtf(mean[0]);
+        for (int ii = 0; ii < bl[0] / 2; ii++) {
+            out[2 * (1 + ii)] = in[2 * (1 + ii)] * norm;
+            out[2 * (1 + ii) + 1] = in[2 * (1 + ii) + 1] * norm;
         }
-        memcpy(out + 2 * numwrote, outb, sizeof(float) * 2 * (size_t)ii);
-        numwrote += ii;
-        binnum += nnew;
-        if ((float)binnum / T < endfreq) bufflen = (int)(startwidth * log((double)binnum));
-        else bufflen = endwidth;
-        if (bufflen > endwidth) {
-            fprintf(stderr, "Error: rednoise block length %d exceeds buffer %d\n", bufflen,
-                    endwidth);
-            return -1;
+    }
+    /* segment k runs from the middle of block k-1 to the middle of block k
+     * (k < nb), or to the end of the last block (k = nb) */
+    #pragma omp parallel for num_threads(nt) schedule(static)
+    for (long long k = 1; k <= nb; k++) {
+        const long long o = k - 1;
+        float dslope = 1.0f;
+        if (k < nb)
+            dslope = (float)((mean[k] - mean[o]) / (0.5 * (bl[o] + bl[k])));
+        else if (nb >= 2)
+            dslope = (float)((mean[o] - mean[o - 1]) / (0.5 * (bl[o - 1] + bl[o])));
+        const long long s0 = bs[o] + bl[o] / 2;
+        const int n = bl[o] - bl[o] / 2 + (k < nb ? bl[k] / 2 : 0);
+        for (int ii = 0; ii < n; ii++) {
+            const float norm = fast_invsqrtf(mean[o] + dslope * (float)ii);
+            out[2 * (s0 + ii)] = in[2 * (s0 + ii)] * norm;
+            out[2 * (s0 + ii) + 1] = in[2 * (s0 + ii) + 1] * norm;
         }
-        memcpy(oldb, newb, sizeof(float) * 2 * (size_t)nnew);
-        nold = nnew;
-        mean_old = mean_new;
-        mid_old = mid_new;
     }
-    ii = 0;
-    for (ind = mid_old; ind < nold; ii++, ind++) {
-        norm = fast_invsqrtf(mean_old + dslope * (float)ii);
-        outb[2 * ii] = oldb[2 * ind] * norm;
-        outb[2 * ii + 1] = oldb[2 * ind + 1] * norm;
-    }
-    memcpy(out + 2 * numwrote, outb, sizeof(float) * 2 * (size_t)ii);
-    numwrote += ii;
-    if (numwrote != nc) {
-        fprintf(stderr, "Error: rednoise wrote %lld of %lld bins\n", numwrote, nc);
-        return -1;
-    }
-    return 0;
+    rc = 0;
+done:
+    free(bs);
+    free(bl);
+    free(mean);
+    return rc;
 }
 
 DatPipe *datpipe_create(int startwidth, int endwidth, double endfreq)
@@ -184,11 +213,17 @@ DatPipe *datpipe_create(int startwidth, int endwidth, double endfreq)
     return p;
 }
 
+void datpipe_set_threads(DatPipe *p, int threads)
+{
+    if (p) p->threads = threads > 0 ? threads : 0;
+}
+
 void datpipe_free(DatPipe *p)
 {
     if (!p) return;
     free(p->red);
     free(p->work);
+    free(p->scratch);
     free(p->packed);
     free(p);
 }
@@ -196,22 +231,24 @@ void datpipe_free(DatPipe *p)
 static int prepare(DatPipe *p, long long N)
 {
     if (p->N == N && p->red) return 0;
-    free(p->red); free(p->work); free(p->packed);
-    p->red = NULL; p->work = NULL; p->packed = NULL;
+    free(p->red); free(p->work); free(p->scratch); free(p->packed);
+    p->red = NULL; p->work = NULL; p->scratch = NULL; p->packed = NULL;
     p->N = 0;
     if (N <= 0 || (N & 1)) return -1;
     p->red = (float *)malloc((size_t)N * sizeof(float));
     p->work = (double *)malloc((size_t)N * sizeof(double));
+    p->scratch = (double *)malloc((size_t)N * sizeof(double));
     p->packed = (float *)malloc((size_t)N * sizeof(float));
-    if (!p->red || !p->work || !p->packed) return -1;
+    if (!p->red || !p->work || !p->scratch || !p->packed) return -1;
     p->N = N;
     return 0;
 }
 
 static int stage(DatPipe *p, const float *samples, long long N, double dt, FFTFile *ft)
 {
-    if (realfft_forward(samples, N, p->work, p->packed) != 0) return -1;
-    if (rednoise(p->packed, p->red, N / 2, N, dt, p->startwidth, p->endwidth, p->endfreq) != 0)
+    if (realfft_forward(samples, N, p->work, p->scratch, p->threads, p->packed) != 0) return -1;
+    if (rednoise(p->packed, p->red, N / 2, N, dt, p->startwidth, p->endwidth, p->endfreq,
+                 p->threads) != 0)
         return -1;
     ft->N = N;
     ft->T = (double)N * dt;

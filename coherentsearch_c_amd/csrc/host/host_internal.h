@@ -15,5 +15,9 @@
 /* Complex FFT of any length n in place (double, interleaved re/im),
  * sign -1 forward, +1 inverse (unnormalised).  0 on success. */
 int cs_fft_any(double *z, long long n, int sign);
+/* The same for long transforms, given n points of scratch (the multi-level
+ * cache-blocked path from 16384 points when n is 7-smooth) on nthreads
+ * OpenMP threads (0: the OpenMP default). */
+int cs_fft_work(double *z, double *scratch, long long n, int sign, int nthreads);
 
 #endif

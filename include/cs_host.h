@@ -84,6 +84,11 @@ CS_API void fftfile_close(FFTFile *ft);
 typedef struct DatPipe DatPipe;
 CS_API DatPipe *datpipe_create(int startwidth, int endwidth, double endfreq);
 CS_API void datpipe_free(DatPipe *p);
+/* No counterpart in the reference (its pipeline runs a single-threaded FFTW
+ * plan): OpenMP threads of the pipeline's complex transform, 0 for the OpenMP
+ * default.  Callers that stage several files at once give each pipeline a
+ * share of the cores. */
+CS_API void datpipe_set_threads(DatPipe *p, int threads);
 CS_API int  datpipe_redfft(DatPipe *p, const char *datpath, FFTFile *ft);
 CS_API int  datpipe_redfft_mem(DatPipe *p, const float *samples, long long N,
                                double dt, const char *path, FFTFile *ft);
