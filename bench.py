@@ -84,15 +84,15 @@ def _same_lists(a, b):
                                     for f in ("r", "nharm", "metric", "ducy"))
 
 
-def dm_sweep(nfiles, slots, host_files=3):
+def dm_sweep(nfiles, slots):
     """The batched DM sweep: the batched/served form (engine, `slots`
     streams, every file submitted at once, each request finishing on the
-    device: collapse, duty cycles from the resident spectrum) against
-    one-file-at-a-time forms on the same series: with device staging (a
-    one-slot engine, each file finished before the next starts) and the
-    reference-style flow (host realfft + rednoise, search_file, duty cycles;
-    timed on the first `host_files` files, whose harmonic-trials come from the
-    same searches)."""
+    device: collapse, duty cycles from the resident spectrum) against the
+    one-file-at-a-time forms on the same series: the reference's list mode
+    (two producer threads realfft + rednoise the next series on the host while
+    search_file runs the current one on the GPU, then duty cycles on the
+    host), and the same one-at-a-time order with device staging (a one-slot
+    engine, each file finished before the next starts)."""
     from coherentsearch_c_amd import engine as eng
     t = time.perf_counter()
     jobs = eng.sweep_jobs(nfiles)
@@ -108,15 +108,14 @@ def dm_sweep(nfiles, slots, host_files=3):
     with eng.Engine(nslots=1) as E:
         fp, _, wp = eng.run_plain_device(E, jobs)
     log("  plain, device staging: %.3f s, %.4g ht/s" % (wp, ht / wp))
-    k = min(host_files, nfiles)
-    fh, wh = eng.run_plain(jobs[:k])
-    hth = sum(ib[i]["harmonic_trials"] for i in range(k))
-    log("  plain, host staging (%d files): %.3f s, %.4g ht/s" % (k, wh, hth / wh))
+    threads = max(1, len(os.sched_getaffinity(0)) // 2)
+    fh, wh, stage_s = eng.run_reference_flow(jobs, producers=2, slots=4, threads=threads)
+    log("  plain, reference list mode: %.3f s, %.4g ht/s (host staging %.3f s per file on %d "
+        "threads per producer)" % (wh, ht / wh, float(np.mean(stage_s)), threads))
     # host staging differs from device staging in the last float bits of a few
     # bins (both FFTs are double precision): lists agree within the tolerance
     host_ok = True
-    for i in range(k):
-        a, b = fb[i], fh[i]
+    for a, b in zip(fb, fh):
         if a.size != b.size or not np.array_equal(a["nharm"], b["nharm"]) or \
                 np.any(np.abs(a["metric"] - b["metric"]) > 1e-4 * np.maximum(np.abs(b["metric"]), 1)):
             host_ok = False
@@ -125,12 +124,14 @@ def dm_sweep(nfiles, slots, host_files=3):
         "files": nfiles, "samples": int(sum(j["samples"].n for j in jobs)),
         "harmonic_trials": int(ht), "slots": slots,
         "batched_s": wb, "batched_ht_s": ht / wb,
+        "plain_reference_flow_s": wh, "plain_reference_flow_ht_s": ht / wh,
+        "ratio_vs_plain_reference_flow": wh / wb,
+        "reference_flow": "2 producers x %d host threads, 4 slots; mean host realfft+rednoise "
+                          "%.3f s per file" % (threads, float(np.mean(stage_s))),
         "plain_device_s": wp, "plain_device_ht_s": ht / wp,
         "ratio_vs_plain_device": wp / wb,
-        "plain_host_files": k, "plain_host_s": wh, "plain_host_ht_s": hth / wh,
-        "ratio_vs_plain_host": (ht / wb) / (hth / wh),
         "lists_identical_batched_vs_plain_device": all(_same_lists(a, b) for a, b in zip(fb, fp)),
-        "lists_match_host_staging_within_1e-4": host_ok,
+        "lists_match_reference_flow_within_1e-4": host_ok,
         "pulsar_files_snr_ge_8_detected": "%d/%d" % (sum(_found(fb[i], jobs[i]["period"])
                                                        for i in psr), len(psr)),
         "peak_concurrent_searches": st["peak_running"],
@@ -208,6 +209,27 @@ def serving(nreq, slots, rate=20.0):
     }
 
 
+def stage_only(sizes, reps=10):
+    """Device staging alone (realfft + rednoise of a resident series): each
+    size staged `reps` times after two warm-up passes; prints one JSON line
+    with the mean milliseconds per staging, copy of the series included."""
+    import coherentsearch_c_amd as cs
+    res = []
+    for N in sizes:
+        x = np.random.default_rng(1).standard_normal(N).astype(np.float32)
+        g = cs.Stage(N, 64e-6)
+        for _ in range(2):
+            g.run_host(x)
+        t = time.perf_counter()
+        for _ in range(reps):
+            g.run_host(x)
+        ms = (time.perf_counter() - t) / reps * 1e3
+        g.close()
+        log("stage N=%d: %.3f ms per staging (host copies included)" % (N, ms))
+        res.append({"samples": N, "ms_per_staging": ms})
+    print(json.dumps({"stage_only": res}), flush=True)
+
+
 def calibrate_copy(nbytes=256 << 20):
     import torch
     a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -237,7 +259,13 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--stage-only", type=int, nargs="*", default=None, metavar="SAMPLES",
+                    help="only time device staging (realfft + rednoise) of series of these "
+                         "lengths (default 2^24 and 3*5*7*2^17)")
     args = ap.parse_args()
+    if args.stage_only is not None:
+        stage_only(args.stage_only or [1 << 24, 3 * 5 * 7 * (1 << 17)])
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return

@@ -4,8 +4,10 @@
  * (<stem>.cohout per input unless a single input without --outdir, which
  * goes to -o or stdout), output-collision check, list files relative to the
  * list's directory, per-file search_file with one shared SearchCache, then
- * sort by S/N, truncate to --ncands, duty cycles and .cohout.  The search
- * always runs on the GPU (--gpu N picks the device, default 0).  Options that
+ * sort by S/N, truncate to --ncands, duty cycles and .cohout.  With more than
+ * one input, producer threads prepare the next inputs while the current one
+ * is searched (src/cli.c:556-686).  The search always runs on the GPU
+ * (--gpu N picks the device, default 0).  Options that
  * need the reference's CPU-only code paths (--metricstats, --normalize,
  * --sigma measured is allowed) or its plotting/zapping tools are rejected
  * with exit status 2; the rest of the CPU-only knobs are accepted and
@@ -15,6 +17,8 @@
 #include "host_internal.h"
 
 #include <errno.h>
+#include <omp.h>
+#include <pthread.h>
 #include <sys/stat.h>
 #include <sys/types.h>
 
@@ -181,6 +185,120 @@ static void mkdir_p(const char *path)
     free(p);
 }
 
+
+/* ---- input prefetch (src/cli.c:556-686) ---------------------------------
+ * PF_PROD producers prepare input i -- the .fft mapped, or the .dat read and
+ * realfft + rednoise'd -- into slot i % nslots as soon as the searcher has
+ * released input i - nslots; the searcher takes the inputs in order.  Each
+ * producer's transforms get an equal share of the OpenMP threads. */
+enum { PF_SLOTS = 4, PF_PROD = 2 };
+
+typedef struct Prefetch Prefetch;
+typedef struct {
+    Prefetch *pf;
+    int first;
+} PfArg;
+
+struct Prefetch {
+    char **paths;
+    int n, nslots, nprod, nstarted;
+    int released;                 /* inputs the searcher is done with */
+    int stop;
+    DatPipe *pipe[PF_SLOTS];
+    FFTFile ft[PF_SLOTS];
+    int held[PF_SLOTS];           /* index of the input in the slot, -1 for none */
+    int ok[PF_SLOTS];
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+    pthread_t th[PF_PROD];
+    PfArg arg[PF_PROD];
+};
+
+static void *pf_producer(void *v)
+{
+    const PfArg *a = (const PfArg *)v;
+    Prefetch *pf = a->pf;
+    for (int i = a->first; i < pf->n; i += pf->nprod) {
+        const int slot = i % pf->nslots;
+        pthread_mutex_lock(&pf->mu);
+        while (!pf->stop && i - pf->released >= pf->nslots) pthread_cond_wait(&pf->cv, &pf->mu);
+        const int stop = pf->stop;
+        pthread_mutex_unlock(&pf->mu);
+        if (stop) break;
+        int is_dat = 0;
+        const int ok = cs_open_input(pf->paths[i], pf->pipe[slot], &pf->ft[slot], &is_dat) == 0;
+        pthread_mutex_lock(&pf->mu);
+        pf->ok[slot] = ok;
+        pf->held[slot] = i;
+        pthread_cond_broadcast(&pf->cv);
+        pthread_mutex_unlock(&pf->mu);
+    }
+    return NULL;
+}
+
+static void pf_finish(Prefetch *pf)
+{
+    pthread_mutex_lock(&pf->mu);
+    pf->stop = 1;
+    pthread_cond_broadcast(&pf->cv);
+    pthread_mutex_unlock(&pf->mu);
+    for (int t = 0; t < pf->nstarted; t++) pthread_join(pf->th[t], NULL);
+    pthread_mutex_destroy(&pf->mu);
+    pthread_cond_destroy(&pf->cv);
+    for (int s = 0; s < pf->nslots; s++) datpipe_free(pf->pipe[s]);
+}
+
+static int pf_start(Prefetch *pf, char **paths, int n)
+{
+    memset(pf, 0, sizeof(*pf));
+    pf->paths = paths;
+    pf->n = n;
+    pf->nslots = n < PF_SLOTS ? n : PF_SLOTS;
+    pf->nprod = n < PF_PROD ? n : PF_PROD;
+    pthread_mutex_init(&pf->mu, NULL);
+    pthread_cond_init(&pf->cv, NULL);
+    const int share = omp_get_max_threads() / pf->nprod;
+    for (int s = 0; s < pf->nslots; s++) {
+        pf->held[s] = -1;
+        pf->pipe[s] = datpipe_create(6, 100, 6.0);
+        if (!pf->pipe[s]) {
+            pf_finish(pf);
+            return -1;
+        }
+        datpipe_set_threads(pf->pipe[s], share > 1 ? share : 1);
+    }
+    for (int t = 0; t < pf->nprod; t++) {
+        pf->arg[t].pf = pf;
+        pf->arg[t].first = t;
+        if (pthread_create(&pf->th[t], NULL, pf_producer, &pf->arg[t]) != 0) {
+            pf_finish(pf);
+            return -1;
+        }
+        pf->nstarted++;
+    }
+    return 0;
+}
+
+/* input i, once prepared: 1 with *ft filled, 0 if it could not be opened */
+static int pf_get(Prefetch *pf, int i, FFTFile *ft)
+{
+    const int slot = i % pf->nslots;
+    pthread_mutex_lock(&pf->mu);
+    while (pf->held[slot] != i) pthread_cond_wait(&pf->cv, &pf->mu);
+    *ft = pf->ft[slot];
+    const int ok = pf->ok[slot];
+    pthread_mutex_unlock(&pf->mu);
+    return ok;
+}
+
+static void pf_release(Prefetch *pf)
+{
+    pthread_mutex_lock(&pf->mu);
+    pf->released++;
+    pthread_cond_broadcast(&pf->cv);
+    pthread_mutex_unlock(&pf->mu);
+}
+
 static int run(int argc, char **argv, int list)
 {
     Opts o;
@@ -315,13 +433,26 @@ static int run(int argc, char **argv, int list)
     }
     SearchCache cache;
     memset(&cache, 0, sizeof(cache));
-    DatPipe *pipe = datpipe_create(6, 100, 6.0);
+    Prefetch pf;
+    int use_pf = 0;
+    if (o.nfiles > 1) {
+        use_pf = pf_start(&pf, o.files, o.nfiles) == 0;
+        if (!use_pf) fprintf(stderr, "Warning: cannot start the prefetch threads; running sequentially\n");
+    }
+    DatPipe *pipe = use_pf ? NULL : datpipe_create(6, 100, 6.0);
     int status = 0;
     for (int fi = 0; fi < o.nfiles; fi++) {
         const char *path = o.files[fi];
         FFTFile ft;
         int is_dat = 0;
-        if (cs_open_input(path, pipe, &ft, &is_dat) != 0) {
+        if (use_pf) {
+            if (!pf_get(&pf, fi, &ft)) {
+                fprintf(stderr, "Error: failed to prepare '%s'\n", path);
+                status = 1;
+                pf_release(&pf);
+                continue;
+            }
+        } else if (cs_open_input(path, pipe, &ft, &is_dat) != 0) {
             fprintf(stderr, "Error: failed to open '%s'\n", path);
             status = 1;
             continue;
@@ -331,6 +462,7 @@ static int run(int argc, char **argv, int list)
             fprintf(stderr, "Error: failed to apply zaplist to '%s'\n", path);
             status = 1;
             fftfile_close(&ft);
+            if (use_pf) pf_release(&pf);
             continue;
         }
         CandidateList c = search_file(&ft, &params, o.lofreq, o.hifreq, o.lobin, o.blocksize,
@@ -347,7 +479,9 @@ static int run(int argc, char **argv, int list)
         free(outfile);
         candlist_free(&d);
         fftfile_close(&ft);
+        if (use_pf) pf_release(&pf);
     }
+    if (use_pf) pf_finish(&pf);
     search_cache_free(&cache);
     datpipe_free(pipe);
     zaplist_free(&zap);

@@ -8,6 +8,9 @@ queued requests by HBM footprint, device staging of each raw series
   run_plain    the one-file-at-a-time GPU form the reference runs
                (src/cli.c:590-700): host realfft + rednoise, search_file
                (session per file, search, collapse), duty cycles, in turn;
+  run_reference_flow  the same with the reference list mode's prefetch:
+               producer threads stage upcoming series on the host while the
+               current one is searched;
   run_batched  every series submitted at once; each request collapses its
                list and measures duty cycles on the device from the resident
                spectrum (or on host threads from a copied-back spectrum);
@@ -17,6 +20,7 @@ queued requests by HBM footprint, device staging of each raw series
 """
 import concurrent.futures as cf
 import ctypes as C
+import os
 import threading
 import time
 
@@ -243,6 +247,94 @@ def run_plain(jobs, device=0, ncands=100):
                         device=device)
         out.append(finalize(spec, j["params"], c, ncands))
     return out, time.perf_counter() - t0
+
+
+class _HostSpectrum:
+    """A staged spectrum held in a DatPipe's buffer, passed to search_file and
+    measure_ducy as it is (no copy)."""
+
+    def __init__(self, ft):
+        self.ft = ft
+
+
+def run_reference_flow(jobs, device=0, ncands=100, producers=2, slots=4, threads=None):
+    """The reference's list mode on in-memory series (src/cli.c:556-686,
+    996-1082): `producers` threads realfft + rednoise upcoming series on the
+    host, input i into DatPipe slot i % `slots` once input i - `slots` is
+    released, while this thread takes the inputs in order: search_file on the
+    GPU with one SearchCache shared across inputs (a session is rebuilt
+    whenever the geometry changes), sort by S/N, keep `ncands`, duty cycles
+    on the host.  Each producer's transforms run on `threads` OpenMP threads
+    (default: an equal share of this process's cores).  Returns (final lists,
+    wall seconds, per-input host staging seconds)."""
+    L = abi.host()
+    n = len(jobs)
+    nslots, nprod = min(slots, n), min(producers, n)
+    if threads is None:
+        threads = max(1, len(os.sched_getaffinity(0)) // nprod)
+    pipes = [L.datpipe_create(6, 100, 6.0) for _ in range(nslots)]
+    for pp in pipes:
+        L.datpipe_set_threads(pp, threads)
+    fts = [abi.FFTFile() for _ in range(nslots)]
+    held, ok = [-1] * nslots, [False] * nslots
+    released, stop = [0], [False]
+    cv = threading.Condition()
+    stage_s = [0.0] * n
+
+    def producer(first):
+        for i in range(first, n, nprod):
+            slot = i % nslots
+            with cv:
+                while not stop[0] and i - released[0] >= nslots:
+                    cv.wait()
+                if stop[0]:
+                    return
+            s = jobs[i]["samples"]
+            a = s.a if isinstance(s, PinnedSeries) else np.ascontiguousarray(s, np.float32)
+            t = time.perf_counter()
+            rc = L.datpipe_redfft_mem(pipes[slot], a.ctypes.data_as(C.c_void_p), a.size,
+                                      jobs[i]["dt"], b"memory", C.byref(fts[slot]))
+            stage_s[i] = time.perf_counter() - t
+            with cv:
+                ok[slot] = rc == 0
+                held[slot] = i
+                cv.notify_all()
+
+    cache = abi.SearchCache()
+    out = []
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=producer, args=(t,), daemon=True) for t in range(nprod)]
+    for t in th:
+        t.start()
+    try:
+        for i, j in enumerate(jobs):
+            slot = i % nslots
+            with cv:
+                while held[slot] != i:
+                    cv.wait()
+            if not ok[slot]:
+                raise RuntimeError("host staging of input %d failed" % i)
+            spec = _HostSpectrum(fts[slot])
+            c = search_file(spec, j["params"], lofreq=j.get("lofreq", 0.1),
+                            hifreq=j.get("hifreq", 125.0), lobin=j.get("lobin", 100),
+                            blocksize=j.get("blocksize", 0), threshold=j.get("threshold", 8.0),
+                            device=device, cache=cache)
+            out.append(finalize(spec, j["params"], c, ncands))
+            L.fftfile_close(C.byref(fts[slot]))
+            with cv:
+                released[0] += 1
+                cv.notify_all()
+        wall = time.perf_counter() - t0
+    finally:
+        with cv:
+            stop[0] = True
+            cv.notify_all()
+        for t in th:
+            t.join()
+        L.search_cache_free(C.byref(cache))
+        for pp in pipes:
+            L.datpipe_free(pp)
+    return out, wall, stage_s
 
 
 def run_batched(engine, jobs, post_workers=8, ncands=100, device_post=True):
