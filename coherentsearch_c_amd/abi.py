@@ -113,6 +113,48 @@ class CsEngineStats(C.Structure):
                 ("plans_resident", C.c_longlong)]
 
 
+class BlockMetricStats(C.Structure):
+    _fields_ = [("block", C.c_int), ("k", C.c_int), ("Hk", C.c_int), ("nbins", C.c_int),
+                ("ngoodbins", C.c_double), ("flo", C.c_double), ("fhi", C.c_double),
+                ("n", C.c_longlong), ("min", C.c_double), ("median", C.c_double),
+                ("mean", C.c_double), ("std", C.c_double), ("max", C.c_double)]
+
+
+class MetricHistogram(C.Structure):
+    _fields_ = [("k", C.c_int), ("Hk", C.c_int), ("nbins", C.c_int), ("win", C.c_int),
+                ("flo", C.c_double), ("fhi", C.c_double), ("lo", C.c_double),
+                ("hi", C.c_double), ("invw", C.c_double),
+                ("counts", C.POINTER(C.c_longlong)), ("nb", C.c_int),
+                ("under", C.c_longlong), ("over", C.c_longlong), ("total", C.c_longlong),
+                ("sum", C.c_double), ("sumsq", C.c_double), ("vmin", C.c_double),
+                ("vmax", C.c_double)]
+
+
+class MetricStats(C.Structure):
+    _fields_ = [("hist_lo", C.c_double), ("hist_hi", C.c_double), ("hist_nb", C.c_int),
+                ("nwin", C.c_int), ("blocks", C.POINTER(BlockMetricStats)),
+                ("nblocks", C.c_size_t), ("capblocks", C.c_size_t),
+                ("hists", C.POINTER(MetricHistogram)), ("nhists", C.c_size_t),
+                ("caphists", C.c_size_t), ("whists", C.POINTER(MetricHistogram)),
+                ("nwhists", C.c_size_t), ("capwhists", C.c_size_t)]
+
+
+class NormK(C.Structure):
+    _fields_ = [("k", C.c_int), ("edges", C.POINTER(C.c_double)),
+                ("loc", C.POINTER(C.c_double)), ("scale", C.POINTER(C.c_double)),
+                ("nwin", C.c_int)]
+
+
+class MetricNorm(C.Structure):
+    _fields_ = [("ks", C.POINTER(NormK)), ("n", C.c_int)]
+
+
+class CsMetricBlock(C.Structure):
+    _fields_ = [("n", C.c_longlong), ("min", C.c_double), ("max", C.c_double),
+                ("sum", C.c_double), ("sumsq", C.c_double), ("ss", C.c_double),
+                ("median", C.c_double), ("under", C.c_longlong), ("over", C.c_longlong)]
+
+
 assert C.sizeof(SimpleInf) == 80 and C.sizeof(FFTFile) == 168
 assert C.sizeof(SearchParams) == 328 and C.sizeof(DirectPlan) == 144
 assert C.sizeof(Candidate) == 40 and C.sizeof(CandidateList) == 24
@@ -128,7 +170,8 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_tune_save", "cs_gpu_tune_load", "cs_engine_create",
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
                "cs_engine_stats", "cs_engine_free", "cs_engine_set_plan_cache", "cs_host_alloc",
-               "cs_host_free"]
+               "cs_host_free", "cs_gpu_session_metric_stats",
+               "cs_gpu_session_metric_candidates"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "direct_plans_build", "direct_plans_free", "cs_chunk_state",
                 "cs_harmonic_fits", "harmonic_change_trials", "chunk_starts", "cs_sinpi",
@@ -144,6 +187,10 @@ HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "zaplist_free", "zaplist_apply", "fftfile_open",
                 "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_set_threads", "datpipe_redfft",
                 "datpipe_redfft_mem", "cs_open_input", "cs_write_inf", "cs_write_floats",
+                "metricstats_init", "metricstats_free", "metricstats_append_block",
+                "metricstats_append_hist", "metricstats_append_whist", "hist_init", "hist_free",
+                "hist_push", "hist_merge", "hist_quantile", "log_edges", "window_index",
+                "build_metricnorm", "metricnorm_free", "metric_normalize", "report_metricstats",
                 "search_file", "search_cache_free", "cs_main", "cs_main_list"]
 
 _host = None
@@ -204,7 +251,39 @@ def _declare(L):
     L.cs_write_floats.argtypes = [C.c_char_p, P, LL]
     L.cs_write_floats.restype = I
     L.search_file.argtypes = [Ps(FFTFile), Ps(SearchParams), D, D, I, I, D, I, D, I, I,
-                              P, I, I, I, C.c_char_p, Ps(SearchCache), D, I]
+                              Ps(MetricStats), I, I, I, C.c_char_p, Ps(SearchCache), D, I]
+    L.metricstats_init.argtypes = [Ps(MetricStats), D, D, I, I]
+    L.metricstats_init.restype = None
+    L.metricstats_free.argtypes = [Ps(MetricStats)]
+    L.metricstats_free.restype = None
+    L.hist_init.argtypes = [Ps(MetricHistogram), I, I, I, D, D, D, D, I]
+    L.hist_init.restype = None
+    L.hist_free.argtypes = [Ps(MetricHistogram)]
+    L.hist_free.restype = None
+    L.hist_push.argtypes = [Ps(MetricHistogram), D]
+    L.hist_push.restype = None
+    L.hist_merge.argtypes = [Ps(MetricHistogram), Ps(MetricHistogram)]
+    L.hist_merge.restype = None
+    L.hist_quantile.argtypes = [Ps(MetricHistogram), D]
+    L.hist_quantile.restype = D
+    L.log_edges.argtypes = [D, D, I, Ps(D)]
+    L.log_edges.restype = None
+    L.window_index.argtypes = [Ps(D), I, D]
+    L.window_index.restype = I
+    L.metricstats_append_whist.argtypes = [Ps(MetricStats), MetricHistogram]
+    L.metricstats_append_whist.restype = None
+    L.metricstats_append_hist.argtypes = [Ps(MetricStats), MetricHistogram]
+    L.metricstats_append_hist.restype = None
+    L.metricstats_append_block.argtypes = [Ps(MetricStats), BlockMetricStats]
+    L.metricstats_append_block.restype = None
+    L.build_metricnorm.argtypes = [Ps(MetricStats)]
+    L.build_metricnorm.restype = Ps(MetricNorm)
+    L.metricnorm_free.argtypes = [Ps(MetricNorm)]
+    L.metricnorm_free.restype = None
+    L.metric_normalize.argtypes = [Ps(MetricNorm), I, D, D]
+    L.metric_normalize.restype = D
+    L.report_metricstats.argtypes = [Ps(MetricStats), C.c_char_p, Ps(SearchParams), D]
+    L.report_metricstats.restype = None
     L.search_file.restype = CandidateList
     L.search_cache_free.argtypes = [Ps(SearchCache)]
     L.cs_main.argtypes = [I, Ps(C.c_char_p)]
@@ -253,6 +332,11 @@ def _declare(L):
     L.cs_gpu_tune_load.restype = I
     L.cs_gpu_session_zap.argtypes = [P, Ps(ZapList), D, P]
     L.cs_gpu_session_zap.restype = I
+    L.cs_gpu_session_metric_stats.argtypes = [P, Ps(FFTFile), Ps(I), I, D, D, I,
+                                              Ps(CsMetricBlock), Ps(LL)]
+    L.cs_gpu_session_metric_stats.restype = I
+    L.cs_gpu_session_metric_candidates.argtypes = [P, D, I, Ps(I), Ps(D), Ps(D), Ps(D)]
+    L.cs_gpu_session_metric_candidates.restype = CandidateList
     L.cs_gpu_session_finalize.argtypes = [P, Ps(CandidateList), I]
     L.cs_gpu_session_finalize.restype = CandidateList
     L.cs_engine_create.argtypes = [I, LL, I]

@@ -201,6 +201,8 @@ struct GpuSession {
     bool own_stream;
     std::vector<float> h_wtab;    /* host copies kept alive for asynchronous uploads */
     std::vector<int2> h_hinfo;
+    void *d_mout;                 /* every trial's metric per slot (metric statistics) */
+    bool mout_ready;
 };
 
 /* cos and sin as two separate libm calls, through pointers the compiler cannot
@@ -282,6 +284,7 @@ static void session_release(GpuSession *S)
     if (S->recs_own) dfree(S, S->d_recs);
     dfree(S, S->d_block);
     dfree(S, S->d_stack);
+    dfree(S, S->d_mout);
     if (S->gexec) hipGraphExecDestroy(S->gexec);
     if (S->graph) hipGraphDestroy(S->graph);
     if (S->ev0) hipEventDestroy(S->ev0);
@@ -851,40 +854,12 @@ extern "C" CS_API CandidateList cs_gpu_session_run(GpuSession *S, const FFTFile 
     return cs_gpu_session_search_resident(S, threshold);
 }
 
-extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
-                                                               double threshold)
+/* The records of a pass, ordered by key (chunk, slot, trial: the reference's
+ * harvest order, src/cuda.cu:833-881), as candidates above threshold. */
+static CandidateList harvest(GpuSession *S, unsigned int count, double threshold)
 {
     CandidateList out;
     out.v = NULL; out.n = 0; out.cap = 0;
-    if (!S || hipSetDevice(S->device) != hipSuccess) return out;
-    unsigned int count = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
-        if (launch_search(S, threshold, NULL) != 0) {
-            fprintf(stderr, "Error: GPU search launch failed\n");
-            return out;
-        }
-        if (hipMemcpyAsync(&count, S->d_counter, sizeof(count), hipMemcpyDeviceToHost,
-                           S->st) != hipSuccess ||
-            hipStreamSynchronize(S->st) != hipSuccess) {
-            fprintf(stderr, "Error: GPU search failed: %s\n",
-                    hipGetErrorString(hipGetLastError()));
-            return out;
-        }
-        float ms = 0.0f;
-        hipEventElapsedTime(&ms, S->ev0, S->ev1);
-        S->last_ms = ms;
-        if (count <= S->cap) break;
-        /* never truncate: grow to the exact count and search again */
-        if (S->recs_own) dfree(S, S->d_recs);
-        S->d_recs = NULL;
-        S->recs_own = true;
-        S->cap = count;
-        if (dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
-            fprintf(stderr, "Error: cannot hold %u candidate records\n", count);
-            S->cap = 0;
-            return out;
-        }
-    }
     if (count == 0) return out;
     std::vector<CandRec> recs(count);
     if (hipMemcpyAsync(recs.data(), S->d_recs, (size_t)count * sizeof(CandRec),
@@ -930,6 +905,44 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
         out.v[out.n++] = cd;
     }
     return out;
+}
+
+
+extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
+                                                               double threshold)
+{
+    CandidateList out;
+    out.v = NULL; out.n = 0; out.cap = 0;
+    if (!S || hipSetDevice(S->device) != hipSuccess) return out;
+    unsigned int count = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        if (launch_search(S, threshold, NULL) != 0) {
+            fprintf(stderr, "Error: GPU search launch failed\n");
+            return out;
+        }
+        if (hipMemcpyAsync(&count, S->d_counter, sizeof(count), hipMemcpyDeviceToHost,
+                           S->st) != hipSuccess ||
+            hipStreamSynchronize(S->st) != hipSuccess) {
+            fprintf(stderr, "Error: GPU search failed: %s\n",
+                    hipGetErrorString(hipGetLastError()));
+            return out;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, S->ev0, S->ev1);
+        S->last_ms = ms;
+        if (count <= S->cap) break;
+        /* never truncate: grow to the exact count and search again */
+        if (S->recs_own) dfree(S, S->d_recs);
+        S->d_recs = NULL;
+        S->recs_own = true;
+        S->cap = count;
+        if (dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
+            fprintf(stderr, "Error: cannot hold %u candidate records\n", count);
+            S->cap = 0;
+            return out;
+        }
+    }
+    return harvest(S, count, threshold);
 }
 
 extern "C" CS_API void cs_gpu_session_free(GpuSession *S) { session_release(S); }
@@ -1342,4 +1355,471 @@ extern "C" CS_API int cs_gpu_session_zap(GpuSession *S, const ZapList *z, double
         return -1;
     }
     return n;
+}
+
+/* ---- metric statistics and normalisation -------------------------------
+ *
+ * The collection pass is the search kernel with mout set: every valid
+ * trial's metric lands at mout[slot*total + trial] (T = float or double, the
+ * profile precision, so the value is the metric exactly).  Then one
+ * workgroup per (chunk, slot) summarises its metrics (metric_stats_kernel),
+ * and a later pass scores them against per-window normalisations
+ * (metric_norm_kernel). */
+
+namespace {
+
+constexpr int kStatThreads = 256;
+constexpr int kStatMaxBins = 4096;
+constexpr int kDigitBits = 11;
+
+struct StatArgs {
+    const ChunkDesc *chunks;
+    const void *mout;
+    long long mstride;
+    int nslot;
+    const int *win;
+    int nwin;
+    double lo, hi, invw;
+    int nb;
+    CsMetricBlock *blocks;
+    unsigned long long *counts;
+};
+
+/* order-preserving unsigned keys of float / double values */
+__device__ __forceinline__ unsigned int okey(float x)
+{
+    const unsigned int u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ unsigned long long okey(double x)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ float ofloat(unsigned int k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k ^ 0x80000000u) : ~k);
+}
+__device__ __forceinline__ double ofloat(unsigned long long k)
+{
+    return __longlong_as_double((long long)((k & 0x8000000000000000ull)
+                                                ? (k ^ 0x8000000000000000ull) : ~k));
+}
+
+/* sum over the workgroup of one value per thread, in a fixed order */
+template <typename V>
+__device__ V wg_sum(V v, V *sh)
+{
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    V t = sh[0];
+    for (int i = 1; i < kStatThreads / 64; i++) t += sh[i];
+    return t;
+}
+template <typename V>
+__device__ V wg_min(V v, V *sh)
+{
+    for (int d = 32; d > 0; d >>= 1) {
+        const V o = __shfl_down(v, d, 64);
+        v = o < v ? o : v;
+    }
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    V t = sh[0];
+    for (int i = 1; i < kStatThreads / 64; i++) t = sh[i] < t ? sh[i] : t;
+    return t;
+}
+
+/* The value of rank r (0-based, ascending) among v[0..n): a most-significant
+ * digit radix selection over the values' order-preserving keys, kDigitBits
+ * per pass, counting in LDS. */
+template <typename T, typename K>
+__device__ T select_rank(const T *__restrict__ v, int n, long long r, unsigned int *cnt,
+                         unsigned long long *sh)
+{
+    constexpr int BITS = 8 * (int)sizeof(K);
+    K prefix = 0;
+    int done = 0;
+    while (done < BITS) {
+        const int width = BITS - done < kDigitBits ? BITS - done : kDigitBits;
+        const int shift = BITS - done - width;
+        const int nd = 1 << width;
+        for (int b = threadIdx.x; b < nd; b += kStatThreads) cnt[b] = 0u;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kStatThreads) {
+            const K k = okey(v[i]);
+            if (done == 0 || ((k ^ prefix) >> (BITS - done)) == 0)
+                atomicAdd(&cnt[(int)((k >> shift) & (K)(nd - 1))], 1u);
+        }
+        __syncthreads();
+        /* each thread sums its run of digits; thread 0 finds the run, then the
+         * digit, holding rank r */
+        const int per = (nd + kStatThreads - 1) / kStatThreads;
+        unsigned long long part = 0;
+        for (int b = threadIdx.x * per; b < (threadIdx.x + 1) * per && b < nd; b++) part += cnt[b];
+        __syncthreads();
+        sh[threadIdx.x] = part;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            long long rem = r;
+            int t = 0;
+            while (t < kStatThreads - 1 && rem >= (long long)sh[t]) rem -= (long long)sh[t++];
+            int d = t * per;
+            while (d < nd - 1 && rem >= (long long)cnt[d]) rem -= (long long)cnt[d++];
+            sh[kStatThreads] = (unsigned long long)d;
+            sh[kStatThreads + 1] = (unsigned long long)rem;
+        }
+        __syncthreads();
+        prefix |= (K)sh[kStatThreads] << shift;
+        r = (long long)sh[kStatThreads + 1];
+        done += width;
+        __syncthreads();
+    }
+    return (T)ofloat(prefix);
+}
+
+template <typename T, typename K>
+__global__ __launch_bounds__(kStatThreads) void metric_stats_kernel(StatArgs a)
+{
+    __shared__ unsigned int cnt[kStatMaxBins];
+    __shared__ unsigned long long sh[kStatThreads + 2];
+    __shared__ double shd[kStatThreads / 64];
+    __shared__ long long shl[kStatThreads / 64];
+    const int c = blockIdx.x / a.nslot, s = blockIdx.x % a.nslot;
+    const ChunkDesc *C = a.chunks + c;
+    int n = C->n;
+    if (s > 0 && C->nvalid[s] < n) n = C->nvalid[s];
+    if (n < 0) n = 0;
+    const T *v = (const T *)a.mout + (long long)s * a.mstride + C->i0;
+    for (int b = threadIdx.x; b < a.nb; b += kStatThreads) cnt[b] = 0u;
+    __syncthreads();
+    double mn = INFINITY, mx = -INFINITY, sum = 0.0, sq = 0.0;
+    long long under = 0, over = 0;
+    for (int i = threadIdx.x; i < n; i += kStatThreads) {
+        const double x = (double)v[i];
+        mn = x < mn ? x : mn;
+        mx = x > mx ? x : mx;
+        sum += x;
+        sq += x * x;
+        if (x < a.lo) {
+            under++;
+        } else if (x >= a.hi) {
+            over++;
+        } else {
+            int idx = (int)floor((x - a.lo) * a.invw);
+            if (idx < 0) idx = 0;
+            if (idx >= a.nb) idx = a.nb - 1;
+            atomicAdd(&cnt[idx], 1u);
+        }
+    }
+    mn = wg_min(mn, shd);
+    mx = -wg_min(-mx, shd);
+    sum = wg_sum(sum, shd);
+    sq = wg_sum(sq, shd);
+    under = wg_sum(under, shl);
+    over = wg_sum(over, shl);
+    const double mean = n > 0 ? sum / (double)n : 0.0;
+    double ss = 0.0;
+    for (int i = threadIdx.x; i < n; i += kStatThreads) {
+        const double d = (double)v[i] - mean;
+        ss += d * d;
+    }
+    ss = wg_sum(ss, shd);
+    const int w = a.win[(long long)c * a.nslot + s];
+    __syncthreads();
+    if (w >= 1 && w <= a.nwin) {
+        unsigned long long *dst = a.counts + ((long long)s * a.nwin + (w - 1)) * a.nb;
+        for (int b = threadIdx.x; b < a.nb; b += kStatThreads)
+            if (cnt[b]) atomicAdd(&dst[b], (unsigned long long)cnt[b]);
+    }
+    __syncthreads();
+    double med = NAN;
+    if (n > 0) {
+        const int half = n >> 1;
+        const T upper = select_rank<T, K>(v, n, half, cnt, sh);
+        if (n & 1) {
+            med = (double)upper;
+        } else {
+            const T lower = select_rank<T, K>(v, n, half - 1, cnt, sh);
+            med = 0.5 * ((double)lower + (double)upper);
+        }
+    }
+    if (threadIdx.x == 0) {
+        CsMetricBlock b;
+        b.n = n;
+        b.min = n > 0 ? mn : NAN;
+        b.max = n > 0 ? mx : NAN;
+        b.sum = sum;
+        b.sumsq = sq;
+        b.ss = ss;
+        b.median = med;
+        b.under = under;
+        b.over = over;
+        a.blocks[blockIdx.x] = b;
+    }
+}
+
+struct NormArgs {
+    const ChunkDesc *chunks;
+    const void *mout;
+    long long mstride;
+    int nslot;
+    int slot_k[CS_MAXSLOT];
+    int nwin;
+    double r_lo, lodr, T, threshold;
+    const int *has_norm;
+    const double *edges;          /* nwin + 1 per slot */
+    const double *loc, *scale;    /* nwin per slot */
+    CandRec *recs;
+    unsigned int *counter;
+    unsigned int cap;
+};
+
+/* The metrics of (chunk, slot) scored (M - loc)/scale in their frequency
+ * window (src/search.c:814-823, window_index src/search.c:479-492) and
+ * appended above threshold, one atomic per wave. */
+template <typename T>
+__global__ __launch_bounds__(kStatThreads) void metric_norm_kernel(NormArgs a)
+{
+    __shared__ double se[CS_MAXWIN + 1];
+    const int c = blockIdx.x / a.nslot, s = blockIdx.x % a.nslot;
+    const ChunkDesc *C = a.chunks + c;
+    int n = C->n;
+    if (s > 0 && C->nvalid[s] < n) n = C->nvalid[s];
+    const int norm = a.has_norm[s];
+    for (int i = threadIdx.x; i <= a.nwin; i += kStatThreads)
+        se[i] = a.edges[(long long)s * (a.nwin + 1) + i];
+    __syncthreads();
+    const T *v = (const T *)a.mout + (long long)s * a.mstride + C->i0;
+    const double rstart = a.r_lo + (double)C->i0 * a.lodr;
+    const int k = a.slot_k[s];
+    for (int j0 = 0; j0 < n; j0 += kStatThreads) {
+        const int j = j0 + threadIdx.x;
+        const bool in = j < n;
+        double score = in ? (double)v[j] : 0.0;
+        if (in && norm) {
+            const double rr = s == 0 ? rstart + (double)j * a.lodr
+                                     : (double)k * (rstart + (double)j * a.lodr);
+            const double f = rr / a.T;
+            int lo = 0, hi = a.nwin + 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (se[mid] <= f) lo = mid + 1;
+                else hi = mid;
+            }
+            int w = lo;
+            if (w < 1) w = 1;
+            if (w > a.nwin) w = a.nwin;
+            const long long o = (long long)s * a.nwin + (w - 1);
+            score = (score - a.loc[o]) / a.scale[o];
+        }
+        const bool pred = in && score > a.threshold;
+        const unsigned long long mask = __ballot(pred);
+        if (mask == 0ull) continue;
+        const int lane = __lane_id();
+        const int leader = __ffsll((long long)mask) - 1;
+        unsigned int base = 0;
+        if (lane == leader) base = atomicAdd(a.counter, (unsigned int)__popcll(mask));
+        base = __shfl(base, leader);
+        if (pred) {
+            const unsigned int pos = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+            if (pos < a.cap) {
+                CandRec rec;
+                rec.key = ((unsigned long long)c << 40) | ((unsigned long long)s << 32) |
+                          (unsigned int)j;
+                rec.metric = score;
+                a.recs[pos] = rec;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" CS_API int cs_gpu_session_metric_stats(GpuSession *S, const FFTFile *ft,
+                                                  const int *win, int nwin, double hist_lo,
+                                                  double hist_hi, int hist_nb,
+                                                  CsMetricBlock *blocks, long long *counts)
+{
+    if (!S || !win || !blocks || !counts) return -1;
+    if (nwin < 1 || nwin > CS_MAXWIN || hist_nb < 1 || hist_nb > kStatMaxBins ||
+        !(hist_hi > hist_lo)) {
+        fprintf(stderr, "Error: metric statistics need 1..%d windows, 1..%d histogram bins "
+                        "and hist_hi > hist_lo\n", CS_MAXWIN, kStatMaxBins);
+        return -1;
+    }
+    if (ft && cs_gpu_session_upload(S, ft) != 0) return -1;
+    if (hipSetDevice(S->device) != hipSuccess) return -1;
+    const size_t esz = S->f64 ? sizeof(double) : sizeof(float);
+    const long long nb2 = S->nchunks * S->nslot;
+    const size_t ncount = (size_t)S->nslot * nwin * hist_nb;
+    int *d_win = NULL;
+    CsMetricBlock *d_blocks = NULL;
+    unsigned long long *d_counts = NULL;
+    int rc = -1;
+    if (!S->d_mout &&
+        dalloc(S, &S->d_mout, (size_t)S->nslot * (size_t)S->total * esz) != hipSuccess) {
+        fprintf(stderr, "Error: cannot hold %lld x %d trial metrics on the device\n", S->total,
+                S->nslot);
+        S->d_mout = NULL;
+        return -1;
+    }
+    S->mout_ready = false;
+    if (hipMalloc(&d_win, (size_t)nb2 * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d_blocks, (size_t)nb2 * sizeof(CsMetricBlock)) != hipSuccess ||
+        hipMalloc(&d_counts, ncount * sizeof(unsigned long long)) != hipSuccess)
+        goto done;
+    if (hipMemcpyAsync(d_win, win, (size_t)nb2 * sizeof(int), hipMemcpyHostToDevice, S->st) !=
+            hipSuccess ||
+        hipMemcpyAsync(d_counts, counts, ncount * sizeof(unsigned long long),
+                       hipMemcpyHostToDevice, S->st) != hipSuccess)
+        goto done;
+    {
+        S->args.mout = S->d_mout;
+        S->args.mstride = S->total;
+        hipEventRecord(S->ev0, S->st);
+        const int e = enqueue_search(S, INFINITY, NULL);
+        S->args.mout = NULL;
+        if (e != 0) goto done;
+        hipEventRecord(S->ev1, S->st);
+        if (nb2 > 0) {
+            StatArgs a;
+            a.chunks = S->d_chunks;
+            a.mout = S->d_mout;
+            a.mstride = S->total;
+            a.nslot = S->nslot;
+            a.win = d_win;
+            a.nwin = nwin;
+            a.lo = hist_lo;
+            a.hi = hist_hi;
+            a.invw = (double)hist_nb / (hist_hi - hist_lo);
+            a.nb = hist_nb;
+            a.blocks = d_blocks;
+            a.counts = d_counts;
+            if (S->f64)
+                hipLaunchKernelGGL((metric_stats_kernel<double, unsigned long long>),
+                                   dim3((unsigned)nb2), dim3(kStatThreads), 0, S->st, a);
+            else
+                hipLaunchKernelGGL((metric_stats_kernel<float, unsigned int>),
+                                   dim3((unsigned)nb2), dim3(kStatThreads), 0, S->st, a);
+            if (hipGetLastError() != hipSuccess) goto done;
+        }
+    }
+    if (hipMemcpyAsync(blocks, d_blocks, (size_t)nb2 * sizeof(CsMetricBlock),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+        hipMemcpyAsync(counts, d_counts, ncount * sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess) {
+        fprintf(stderr, "Error: GPU metric statistics failed: %s\n",
+                hipGetErrorString(hipGetLastError()));
+        goto done;
+    }
+    {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, S->ev0, S->ev1);
+        S->last_ms = ms;
+    }
+    S->mout_ready = true;
+    rc = 0;
+done:
+    hipStreamSynchronize(S->st);
+    if (d_win) hipFree(d_win);
+    if (d_blocks) hipFree(d_blocks);
+    if (d_counts) hipFree(d_counts);
+    return rc;
+}
+
+extern "C" CS_API CandidateList cs_gpu_session_metric_candidates(GpuSession *S, double threshold,
+                                                                 int nwin, const int *has_norm,
+                                                                 const double *edges,
+                                                                 const double *loc,
+                                                                 const double *scale)
+{
+    CandidateList out;
+    out.v = NULL; out.n = 0; out.cap = 0;
+    if (!S || !has_norm || !edges || !loc || !scale) return out;
+    if (!S->mout_ready) {
+        fprintf(stderr, "Error: no kept metrics (run cs_gpu_session_metric_stats first)\n");
+        return out;
+    }
+    if (nwin < 1 || nwin > CS_MAXWIN) {
+        fprintf(stderr, "Error: 1..%d normalisation windows\n", CS_MAXWIN);
+        return out;
+    }
+    if (hipSetDevice(S->device) != hipSuccess) return out;
+    const int ns = S->nslot;
+    const size_t o_e = 0, o_l = o_e + (size_t)ns * (nwin + 1) * sizeof(double);
+    const size_t o_s = o_l + (size_t)ns * nwin * sizeof(double);
+    const size_t o_h = o_s + (size_t)ns * nwin * sizeof(double);
+    const size_t bytes = o_h + (size_t)ns * sizeof(int);
+    char *d = NULL;
+    unsigned int count = 0;
+    if (hipMalloc(&d, bytes) != hipSuccess) return out;
+    NormArgs a;
+    memset(&a, 0, sizeof(a));
+    a.chunks = S->d_chunks;
+    a.mout = S->d_mout;
+    a.mstride = S->total;
+    a.nslot = ns;
+    for (int s = 0; s < ns; s++) a.slot_k[s] = S->slot_k[s];
+    a.nwin = nwin;
+    a.r_lo = S->r_lo;
+    a.lodr = S->lodr;
+    a.T = S->T;
+    a.threshold = threshold;
+    a.edges = (const double *)(d + o_e);
+    a.loc = (const double *)(d + o_l);
+    a.scale = (const double *)(d + o_s);
+    a.has_norm = (const int *)(d + o_h);
+    if (hipMemcpyAsync(d + o_e, edges, (size_t)ns * (nwin + 1) * sizeof(double),
+                       hipMemcpyHostToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(d + o_l, loc, (size_t)ns * nwin * sizeof(double), hipMemcpyHostToDevice,
+                       S->st) != hipSuccess ||
+        hipMemcpyAsync(d + o_s, scale, (size_t)ns * nwin * sizeof(double),
+                       hipMemcpyHostToDevice, S->st) != hipSuccess ||
+        hipMemcpyAsync(d + o_h, has_norm, (size_t)ns * sizeof(int), hipMemcpyHostToDevice,
+                       S->st) != hipSuccess)
+        goto done;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        a.recs = S->d_recs;
+        a.counter = S->d_counter;
+        a.cap = S->cap;
+        hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
+        if (S->nchunks * ns > 0) {
+            if (S->f64)
+                hipLaunchKernelGGL(metric_norm_kernel<double>, dim3((unsigned)(S->nchunks * ns)),
+                                   dim3(kStatThreads), 0, S->st, a);
+            else
+                hipLaunchKernelGGL(metric_norm_kernel<float>, dim3((unsigned)(S->nchunks * ns)),
+                                   dim3(kStatThreads), 0, S->st, a);
+        }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&count, S->d_counter, sizeof(count), hipMemcpyDeviceToHost,
+                           S->st) != hipSuccess ||
+            hipStreamSynchronize(S->st) != hipSuccess) {
+            fprintf(stderr, "Error: GPU metric normalisation failed: %s\n",
+                    hipGetErrorString(hipGetLastError()));
+            goto done;
+        }
+        if (count <= S->cap) break;
+        if (S->recs_own) dfree(S, S->d_recs);
+        S->d_recs = NULL;
+        S->recs_own = true;
+        S->cap = count;
+        if (dalloc(S, (void **)&S->d_recs, (size_t)S->cap * sizeof(CandRec)) != hipSuccess) {
+            fprintf(stderr, "Error: cannot hold %u candidate records\n", count);
+            S->cap = 0;
+            goto done;
+        }
+    }
+    out = harvest(S, count, threshold);
+done:
+    hipStreamSynchronize(S->st);
+    hipFree(d);
+    return out;
 }

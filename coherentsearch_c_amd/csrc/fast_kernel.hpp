@@ -348,6 +348,10 @@ struct FastArgs {
     float2 *stack_out;        /* optional stack dump of trials [dump_t0, +dump_n) */
     long long dump_t0;
     long long dump_n;
+    void *mout;               /* metric collection (--metricstats/--normalize): every valid
+                                 trial's metric as T at mout[slot*mstride + i0 + j] instead
+                                 of above-threshold records; null otherwise */
+    long long mstride;
 };
 
 __device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, double metric,
@@ -370,6 +374,15 @@ __device__ __forceinline__ void emit(bool pred, int chunk, int slot, int j, doub
             a.recs[pos] = r;
         }
     }
+}
+
+/* The metric of trial j of the chunk starting at trial i0, in the dense
+ * per-slot metric array of the collection pass. */
+template <typename T>
+__device__ __forceinline__ void metric_out(bool pred, int slot, long long i0, int j, T metric,
+                                           const FastArgs &a)
+{
+    if (pred) ((T *)a.mout)[(long long)slot * a.mstride + i0 + j] = metric;
 }
 
 /* Measured sigma (src/boxcar.c:18-73, 138-167): the sample of (chunk, slot)
@@ -581,8 +594,11 @@ void fast_search_kernel(FastArgs a)
                 } else {
                     const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
                                                 C->valid[slot]);
-                    emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj,
-                         mv, a);
+                    if (a.mout)
+                        metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
+                    else
+                        emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot,
+                             lj, mv, a);
                 }
             }
         }
@@ -615,8 +631,11 @@ void fast_search_kernel(FastArgs a)
                 } else {
                     const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
                                             C->valid[slot]);
-                    emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot, lj,
-                         mv, a);
+                    if (a.mout)
+                        metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
+                    else
+                        emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot,
+                             lj, mv, a);
                 }
             }
         }
@@ -627,7 +646,8 @@ void fast_search_kernel(FastArgs a)
         return;
     }
     const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0]);
-    emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
+    if (a.mout) metric_out<T>(active, 0, i0, lj, mb, a);
+    else emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
 }
 
 }  // namespace csk

@@ -7,11 +7,11 @@
  * sort by S/N, truncate to --ncands, duty cycles and .cohout.  With more than
  * one input, producer threads prepare the next inputs while the current one
  * is searched (src/cli.c:556-686).  The search always runs on the GPU
- * (--gpu N picks the device, default 0).  Options that
- * need the reference's CPU-only code paths (--metricstats, --normalize,
- * --sigma measured is allowed) or its plotting/zapping tools are rejected
- * with exit status 2; the rest of the CPU-only knobs are accepted and
- * ignored.
+ * (--gpu N picks the device, default 0), including --metricstats and
+ * --normalize, which the reference runs on the CPU only: their report goes to
+ * stderr and <base>_metricstats.txt, _metricfap.txt and _metrichist.txt
+ * (src/cli.c:1049-1091).  The plotting options are rejected with exit
+ * status 2; the rest of the CPU-only knobs are accepted and ignored.
  */
 #define _GNU_SOURCE
 #include "host_internal.h"
@@ -32,6 +32,7 @@ typedef struct {
     cs_precision precision;
     cs_sigma sigma;
     int remove, harm_remove, verbose, gpu_device;
+    int metricstats, normalize;
 } Opts;
 
 static void usage(FILE *f, int list)
@@ -52,6 +53,8 @@ static void usage(FILE *f, int list)
 "      --noremove      --noharmremove              --gpu [N]         device (0)\n"
 "      --zaplist F     zap the listed birds (Hz, width Hz; 'B' lines barycentric)\n"
 "      --baryv V       v/c applied to non-'B' zaplist lines (0.0)\n"
+"      --metricstats   per-block and per-(k, frequency) metric statistics report\n"
+"      --normalize     metric in noise units per (k, frequency window), two passes\n"
 "  -v, --verbose       --threads N  --nowisdom  --wisdomfile F  --progressbar\n"
 "      --noprogress    --noplot     --help\n");
 }
@@ -367,8 +370,9 @@ static int run(int argc, char **argv, int list)
                 const long v = strtol(argv[i + 1], &e, 10);
                 if (e && e != argv[i + 1] && *e == '\0' && v >= 0) { o.gpu_device = (int)v; i++; }
             }
-        } else if (!strcmp(name, "--metricstats") || !strcmp(name, "--normalize") ||
-                   !strcmp(name, "--plot") || !strcmp(name, "--plotstem") ||
+        } else if (!strcmp(name, "--metricstats")) o.metricstats = 1;
+        else if (!strcmp(name, "--normalize")) o.normalize = 1;
+        else if (!strcmp(name, "--plot") || !strcmp(name, "--plotstem") ||
                    !strcmp(name, "--plotcols") || !strcmp(name, "--plotrows") ||
                    !strcmp(name, "--plot-candidates") || !strcmp(name, "--prime-wisdom")) {
             fprintf(stderr, "Error: option '%s' is not supported by this GPU build\n", name);
@@ -465,10 +469,14 @@ static int run(int argc, char **argv, int list)
             if (use_pf) pf_release(&pf);
             continue;
         }
+        /* src/cli.c:1049-1058 */
+        const int use_ms = o.metricstats || o.normalize;
+        MetricStats ms;
+        if (use_ms) metricstats_init(&ms, 0.0, 64.0, 3200, 16);
         CandidateList c = search_file(&ft, &params, o.lofreq, o.hifreq, o.lobin, o.blocksize,
                                       o.threshold, o.remove, o.drtol, o.harm_remove,
-                                      o.numharm, NULL, 0, o.verbose, 0, NULL, &cache, 1.0,
-                                      o.gpu_device);
+                                      o.numharm, use_ms ? &ms : NULL, o.normalize, o.verbose, 0,
+                                      NULL, &cache, 1.0, o.gpu_device);
         candlist_sort_metric_desc(&c);
         if ((int)c.n > o.ncands) c.n = (size_t)o.ncands;
         CandidateList d = measure_ducy(&ft, &c, &params);
@@ -476,6 +484,28 @@ static int run(int argc, char **argv, int list)
         const int names = list ? (o.nfiles > 1 ? o.nfiles : 2) : o.nfiles;
         char *outfile = output_path(path, o.outfilenm, o.outdir, names);
         if (write_candidates(&d, outfile ? outfile : "", o.threshold) != 0) status = 1;
+        if (use_ms) {
+            /* report base: -o, else the .cohout path without its suffix, else the
+             * input's name without directory and extension (src/cli.c:1075-1091) */
+            char *base = NULL;
+            if (o.outfilenm[0]) {
+                base = strdup(o.outfilenm);
+            } else if (outfile && *outfile) {
+                base = strdup(outfile);
+                char *dot = base ? strrchr(base, '.') : NULL;
+                if (dot && strcmp(dot, ".cohout") == 0) *dot = '\0';
+            } else {
+                char *stem = path_stem(ft.path ? ft.path : path);
+                if (stem) {
+                    const char *bn = strrchr(stem, '/');
+                    base = strdup(bn ? bn + 1 : stem);
+                    free(stem);
+                }
+            }
+            report_metricstats(&ms, base ? base : "candidates", &params, o.threshold);
+            free(base);
+            metricstats_free(&ms);
+        }
         free(outfile);
         candlist_free(&d);
         fftfile_close(&ft);

@@ -312,15 +312,58 @@ class Session:
             pass
 
 
+class MetricStats:
+    """A MetricStats (src/search.c:409-535) for search_file's --metricstats
+    collection, with the histogram range and window count of the CLI
+    (src/cli.c:1050); view() copies the statistics out."""
+
+    def __init__(self, lo=0.0, hi=64.0, nb=3200, nwin=16):
+        self.L = abi.host()
+        self.ms = abi.MetricStats()
+        self.L.metricstats_init(C.byref(self.ms), lo, hi, nb, nwin)
+
+    def view(self):
+        m = self.ms
+        blocks = [{f: getattr(m.blocks[i], f) for f, _ in abi.BlockMetricStats._fields_}
+                  for i in range(m.nblocks)]
+
+        def hist(h):
+            d = {f: getattr(h, f) for f, _ in abi.MetricHistogram._fields_ if f != "counts"}
+            d["counts"] = np.ctypeslib.as_array(h.counts, shape=(h.nb,)).copy()
+            return d
+        return dict(blocks=blocks, whists=[hist(m.whists[i]) for i in range(m.nwhists)],
+                    hists=[hist(m.hists[i]) for i in range(m.nhists)],
+                    nwin=m.nwin, lo=m.hist_lo, hi=m.hist_hi, nb=m.hist_nb)
+
+    def report(self, base, params, threshold):
+        """report_metricstats (src/cli.c:170-303)."""
+        self.L.report_metricstats(C.byref(self.ms), str(base).encode(), C.byref(params),
+                                  threshold)
+
+    def close(self):
+        if self.ms is not None:
+            self.L.metricstats_free(C.byref(self.ms))
+            self.ms = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def search_file(spec, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
                 threshold=8.0, remove=True, dr_tol=1.0, harm_remove=True, numharm=16,
-                device=0, cache=None, harm_tol=1.0):
-    """search.c:1247-1486 (GPU path): raw search then collapse."""
+                device=0, cache=None, harm_tol=1.0, metricstats=None, normalize=False):
+    """search.c:1247-1486 (GPU path): raw search then collapse.  metricstats
+    (a MetricStats) collects the per-block and per-window statistics;
+    normalize scores the candidates per (k, frequency window)."""
     L = abi.host()
     cl = L.search_file(C.byref(spec.ft), C.byref(params), lofreq, hifreq, lobin, blocksize,
-                       threshold, int(remove), dr_tol, int(harm_remove), numharm, None, 0, 0,
-                       0, None, C.byref(cache) if cache is not None else None, harm_tol,
-                       device)
+                       threshold, int(remove), dr_tol, int(harm_remove), numharm,
+                       C.byref(metricstats.ms) if metricstats is not None else None,
+                       int(bool(normalize)), 0, 0, None,
+                       C.byref(cache) if cache is not None else None, harm_tol, device)
     return take_list(cl)
 
 

@@ -183,6 +183,51 @@ CS_API int cs_gpu_tune_save(const char *path);
 CS_API int cs_gpu_tune_load(const char *path);
 
 /* ------------------------------------------------------------------ */
+/* Metric statistics and normalisation on the device (no counterpart in the
+ * reference's GPU library, whose caller refuses --metricstats/--normalize
+ * with --gpu, src/cli.c:911-915; they replace its CPU collection in
+ * chunk_after_fill/decim_pass and the second, normalised search_region of
+ * src/search.c:537-565, 605-747, 830-945, 1428-1440).
+ *
+ * Slot s of a session is the base (k = 1) for s = 0, then the decimations
+ * k > 1 of SearchParams.decimations in order; trial j of chunk c belongs to
+ * slot s when j < the chunk's length (base) or j < its count of trials below
+ * the decimated Nyquist frequency (src/search.c:619-629). */
+#define CS_MAXWIN 64        /* frequency windows per decimation (the CLI uses 16) */
+
+typedef struct {
+    long long n;            /* trials with a metric */
+    double min, max;
+    double sum, sumsq;      /* sums of x and x*x (a fixed-order tree on the device) */
+    double ss;              /* sum of (x - sum/n)^2 */
+    double median;          /* median_f64 (src/boxcar.c:62-71): the middle value, or the
+                               mean of the two middle values for even n */
+    long long under, over;  /* metrics below hist_lo / at or above hist_hi */
+} CsMetricBlock;
+
+/* One search of ft (NULL: the resident spectrum) keeping every trial's metric
+ * on the device (the measured-sigma passes first with --sigma measured).  For
+ * every (chunk c, slot s) it fills blocks[c*nslot + s] and adds the chunk's
+ * metrics in [hist_lo, hist_hi) to counts[(s*nwin + w-1)*hist_nb + bin] for
+ * w = win[c*nslot + s] (1..nwin; 0 adds nothing), bin = floor((x - hist_lo)
+ * * hist_nb/(hist_hi - hist_lo)) clamped to [0, hist_nb) -- hist_push's rule
+ * (src/search.c:430-445).  counts must be zeroed by the caller (it is added
+ * to).  hist_nb <= 4096.  0 on success. */
+CS_API int cs_gpu_session_metric_stats(GpuSession *s, const FFTFile *ft, const int *win,
+                                       int nwin, double hist_lo, double hist_hi, int hist_nb,
+                                       CsMetricBlock *blocks, long long *counts);
+/* After cs_gpu_session_metric_stats: the candidates of the kept metrics in
+ * the order cs_gpu_session_run gives, each scored (M - loc)/scale with the
+ * loc and scale of the window holding its frequency -- window_index's rule
+ * over edges[s*(nwin+1) ...] (src/search.c:479-492, 814-823) -- or M itself
+ * when has_norm[s] = 0; only scores above threshold.  loc and scale hold
+ * nwin values per slot. */
+CS_API CandidateList cs_gpu_session_metric_candidates(GpuSession *s, double threshold,
+                                                      int nwin, const int *has_norm,
+                                                      const double *edges, const double *loc,
+                                                      const double *scale);
+
+/* ------------------------------------------------------------------ */
 /* Search engine: continuous batching of whole searches (no reference
  * counterpart: the reference searches one file at a time behind prefetch
  * threads, src/cli.c:590-617).  `nslots` worker streams; a request is

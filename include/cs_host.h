@@ -115,13 +115,83 @@ typedef struct {
     void *gpu_session;   /* GpuSession* */
 } SearchCache;
 
+/* ---- metric statistics and normalisation: src/search.c:409-822 ---------- */
+/* coherent_search.h:262-267 */
+typedef struct {
+    int block, k, Hk, nbins;
+    double ngoodbins, flo, fhi;
+    long long n;
+    double min, median, mean, std, max;
+} BlockMetricStats;
+
+/* coherent_search.h:269-276 */
+typedef struct {
+    int k, Hk, nbins, win;
+    double flo, fhi, lo, hi, invw;
+    long long *counts;
+    int nb;
+    long long under, over, total;
+    double sum, sumsq, vmin, vmax;
+} MetricHistogram;
+
+/* coherent_search.h:278-287 */
+typedef struct {
+    double hist_lo, hist_hi;
+    int hist_nb, nwin;
+    BlockMetricStats *blocks;
+    size_t nblocks, capblocks;
+    MetricHistogram *hists;   /* one per decimation k */
+    size_t nhists, caphists;
+    MetricHistogram *whists;  /* one per (k, frequency window) */
+    size_t nwhists, capwhists;
+} MetricStats;
+
+/* coherent_search.h:391-402 */
+typedef struct {
+    int k;
+    double *edges;   /* nwin + 1 */
+    double *loc;
+    double *scale;
+    int nwin;
+} NormK;
+
+typedef struct {
+    NormK *ks;
+    int n;
+} MetricNorm;
+
+CS_API void metricstats_init(MetricStats *ms, double lo, double hi, int nb, int nwin);  /* search.c:493-498 */
+CS_API void metricstats_free(MetricStats *ms);                                         /* search.c:499-508 */
+CS_API void metricstats_append_block(MetricStats *ms, BlockMetricStats s);             /* search.c:509-517 */
+CS_API void metricstats_append_hist(MetricStats *ms, MetricHistogram h);               /* search.c:518-526 */
+CS_API void metricstats_append_whist(MetricStats *ms, MetricHistogram h);              /* search.c:527-535 */
+CS_API void hist_init(MetricHistogram *h, int k, int Hk, int win, double flo, double fhi,
+                      double lo, double hi, int nb);                                   /* search.c:412-423 */
+CS_API void hist_free(MetricHistogram *h);                                             /* search.c:424-429 */
+CS_API void hist_push(MetricHistogram *h, double x);                                   /* search.c:430-445 */
+CS_API void hist_merge(MetricHistogram *a, const MetricHistogram *b);                  /* search.c:446-453 */
+CS_API double hist_quantile(const MetricHistogram *h, double q);                       /* search.c:454-470 */
+CS_API void log_edges(double a, double b, int nwin, double *edges);                    /* search.c:471-478 */
+CS_API int window_index(const double *edges, int nwin, double f);                      /* search.c:479-492 */
+CS_API MetricNorm *build_metricnorm(const MetricStats *ms);                            /* search.c:752-801 */
+CS_API void metricnorm_free(MetricNorm *n);                                            /* search.c:802-813 */
+CS_API double metric_normalize(const MetricNorm *n, int k, double f, double M);        /* search.c:814-823 */
+/* cli.c:170-303: the stderr tables and <base>_metricstats.txt, _metricfap.txt,
+ * _metrichist.txt */
+CS_API void report_metricstats(const MetricStats *ms, const char *base, const SearchParams *p,
+                               double threshold);
+
 /* search.c:1247-1486 with the GPU path only.  gpu_device < 0 is an error
- * (this library has no CPU search).  metricstats/normalize/wisdom
- * arguments of the reference are accepted only at their off values. */
+ * (this library has no CPU search).  With metricstats or normalize the GPU
+ * keeps every trial's metric (cs_gpu_session_metric_stats): metricstats
+ * receives the per-block statistics and the per-(k, window) histograms, and
+ * normalize scores the candidates against the histograms' robust location
+ * and scale per frequency window, as the reference's two-pass CPU search
+ * does.  The wisdom arguments are accepted and ignored. */
 CS_API CandidateList search_file(const FFTFile *ft, const SearchParams *p,
                                  double lofreq, double hifreq, int lobin, int blocksize,
                                  double threshold, int remove, double dr_tol,
-                                 int harm_remove, int numharm, void *metricstats,
+                                 int harm_remove, int numharm, MetricStats *metricstats,
                                  int normalize, int verbose, int wisdom,
                                  const char *wisdom_file, SearchCache *cache,
                                  double harm_tol, int gpu_device);

@@ -329,3 +329,296 @@ def rednoise(packed, dt):
     if lib().orc_rednoise(ptr(s), s.size, dt, ptr(out)) != 0:
         raise ValueError("rednoise failed")
     return out
+
+
+# ---------------------------------------------------------------------------
+# Metric statistics and normalisation (--metricstats / --normalize), the
+# reference's single-threaded order: chunks in order, the base then the
+# decimations in configuration order, trials ascending.
+# ---------------------------------------------------------------------------
+
+def log_edges(a, b, nwin):
+    """src/search.c:471-478: nwin log-spaced windows from a to b."""
+    import math
+    a = max(a, np.finfo(np.float64).tiny)
+    if b < a * (1.0 + 1e-9):
+        b = a * (1.0 + 1e-9)
+    la, lb = math.log10(a), math.log10(b)      # libm, as the reference
+    return np.array([math.pow(10.0, la + (lb - la) * i / nwin) for i in range(nwin + 1)])
+
+
+def window_index(edges, nwin, f):
+    """src/search.c:479-492: 1 + the last window whose left edge is <= f,
+    clamped to 1..nwin (the count of edges <= f)."""
+    cnt = int(np.sum(np.asarray(edges[:nwin + 1]) <= f))
+    return min(max(cnt, 1), nwin)
+
+
+class Hist:
+    """src/search.c:412-453: nb equal bins over [lo, hi), under/over counts,
+    running sum, sum of squares, minimum and maximum."""
+
+    def __init__(self, k, Hk, win, flo, fhi, lo=0.0, hi=64.0, nb=3200):
+        self.k, self.Hk, self.nbins, self.win = k, Hk, 2 * Hk, win
+        self.flo, self.fhi, self.lo, self.hi, self.nb = flo, fhi, lo, hi, nb
+        self.invw = nb / (hi - lo)
+        self.counts = np.zeros(nb, dtype=np.int64)
+        self.under = self.over = self.total = 0
+        self.sum = self.sumsq = 0.0
+        self.vmin, self.vmax = np.inf, -np.inf
+
+    def push(self, v):
+        """hist_push over v in order (sequential double sums)."""
+        v = np.asarray(v, dtype=np.float64)
+        if v.size == 0:
+            return
+        self.total += v.size
+        self.sum = float(np.add.accumulate(np.concatenate([[self.sum], v]))[-1])
+        self.sumsq = float(np.add.accumulate(np.concatenate([[self.sumsq], v * v]))[-1])
+        self.vmin = min(self.vmin, float(v.min()))
+        self.vmax = max(self.vmax, float(v.max()))
+        under = v < self.lo
+        over = ~under & (v >= self.hi)
+        self.under += int(under.sum())
+        self.over += int(over.sum())
+        mid = v[~under & ~over]
+        idx = np.clip(np.floor((mid - self.lo) * self.invw).astype(np.int64), 0, self.nb - 1)
+        np.add.at(self.counts, idx, 1)
+
+    def merge(self, b):
+        self.counts += b.counts
+        self.under += b.under
+        self.over += b.over
+        self.total += b.total
+        self.sum += b.sum
+        self.sumsq += b.sumsq
+        self.vmin = min(self.vmin, b.vmin)
+        self.vmax = max(self.vmax, b.vmax)
+
+    def quantile(self, q):
+        """src/search.c:454-470: linear inside the bin crossing q*total."""
+        if self.total == 0:
+            return np.nan
+        target = q * self.total
+        cum = self.under
+        if cum >= target:
+            return self.lo
+        w = 1.0 / self.invw
+        for i in range(self.nb):
+            c = int(self.counts[i])
+            if cum + c >= target:
+                edge = self.lo + i * w
+                return edge if c == 0 else edge + (target - cum) / c * w
+            cum += c
+        return self.hi
+
+
+def _median(v):
+    """median_f64 (src/boxcar.c:62-71)."""
+    n = v.size
+    s = np.sort(v)
+    h = n // 2
+    return float(s[h]) if n & 1 else 0.5 * (float(s[h - 1]) + float(s[h]))
+
+
+def metric_stats(o, nwin=16, lo=0.0, hi=64.0, nb=3200):
+    """The statistics of src/search.c:989-1095 for the oracle's search o:
+    per (chunk, decimation) block statistics (src/search.c:537-565), the
+    per-(k, window) histograms (a chunk's metrics go to the window of its mean
+    frequency, src/search.c:864-868, 924-929) and the band-wide histogram per
+    k.  Returns dict(blocks=[...], whists=[Hist], hists=[Hist], statks=[k],
+    edges={k: array})."""
+    cfg = o.cfg
+    nh = cfg.nharms
+    r_lo, lodr, T, N = o.r_lo, o.lodr, o.T, o.N
+    r_hi = cfg.hifreq * T
+    statks = sorted(set(o.ks))
+    edges = {k: log_edges(k * r_lo / T, k * r_hi / T, nwin) for k in statks}
+    wh = {k: [Hist(k, nh // k, w + 1, edges[k][w], edges[k][w + 1], lo, hi, nb)
+              for w in range(nwin)] for k in statks}
+    blocks = []
+    nyq = N / 2.0
+    for c in range(o.nchunks):
+        i0, n, rs = o.chunk(c)
+        rmean = rs + (n - 1) * lodr / 2.0
+        M = o.metrics(c)
+        for s, k in enumerate(o.ks):
+            Hk = nh // k
+            if s == 0:
+                v = M[0, :n]
+            else:
+                j = np.arange(n, dtype=np.float64)
+                nv = int(np.sum(k * (rs + j * lodr) < nyq))
+                v = M[s, :nv]
+            w = window_index(edges[k], nwin, k * rmean / T)
+            wh[k][w - 1].push(v)
+            if v.size == 0:
+                continue
+            sm = float(np.add.accumulate(v)[-1])
+            mean = sm / v.size
+            ss = float(np.add.accumulate((v - mean) ** 2)[-1])
+            if s == 0:
+                ngood = min(N / 2.0 / rmean, float(nh))
+                flo, fhi = rs / T, (rs + (v.size - 1) * lodr) / T
+            else:
+                ngood = min(N / 2.0 / (k * rmean), float(Hk))
+                flo, fhi = k * rs / T, k * (rs + (v.size - 1) * lodr) / T
+            blocks.append(dict(block=c + 1, k=k, Hk=Hk, nbins=2 * Hk, ngoodbins=ngood,
+                               flo=flo, fhi=fhi, n=v.size, min=float(v.min()),
+                               median=_median(v), mean=mean,
+                               std=float(np.sqrt(ss / (v.size - 1))) if v.size > 1 else 0.0,
+                               max=float(v.max())))
+    blocks.sort(key=lambda b: (b["block"], b["k"]))
+    whists, hists = [], []
+    for k in statks:
+        g = Hist(k, nh // k, 0, edges[k][0], edges[k][nwin], lo, hi, nb)
+        for h in wh[k]:
+            g.merge(h)
+            whists.append(h)
+        hists.append(g)
+    return dict(blocks=blocks, whists=whists, hists=hists, statks=statks, edges=edges,
+                nwin=nwin, lo=lo, hi=hi, nb=nb)
+
+
+def build_metricnorm(ms, min_win_trials=200):
+    """src/search.c:752-801: per k and window, loc = median and scale =
+    84.13 % quantile - median, the band-wide values where a window has fewer
+    than min_win_trials trials or no spread (scale 1 if neither has)."""
+    def ls(h):
+        loc = h.quantile(0.5)
+        sc = h.quantile(0.8413) - loc
+        return loc, (sc if sc > 0.0 else np.nan)
+    norm = {}
+    for g in ms["hists"]:
+        gloc, gsc = ls(g)
+        if np.isnan(gsc):
+            gsc = 1.0
+        wins = [h for h in ms["whists"] if h.k == g.k]
+        if not wins:
+            continue
+        e = [h.flo for h in wins] + [wins[-1].fhi]
+        loc, sc = [], []
+        for h in wins:
+            l, s = ls(h)
+            if h.total < min_win_trials or np.isnan(s):
+                l, s = gloc, gsc
+            loc.append(l)
+            sc.append(s)
+        norm[g.k] = (np.array(e), np.array(loc), np.array(sc))
+    return norm
+
+
+def metric_candidates(o, threshold, norm=None):
+    """The candidates of src/search.c:605-747, 830-945 in the reference's
+    order with the metric, or with --normalize its score (M - loc)/scale in
+    the window of the trial's frequency (src/search.c:814-823)."""
+    cfg = o.cfg
+    nh, lodr, T, N = cfg.nharms, o.lodr, o.T, o.N
+    parts = []
+    for c in range(o.nchunks):
+        i0, n, rs = o.chunk(c)
+        M = o.metrics(c)
+        j = np.arange(n, dtype=np.float64)
+        for s, k in enumerate(o.ks):
+            r = rs + j * lodr if s == 0 else float(k) * (rs + j * lodr)
+            ok = np.ones(n, bool) if s == 0 else r < N / 2.0
+            m = M[s, :n].astype(np.float64)
+            if norm is not None and k in norm:
+                e, loc, sc = norm[k]
+                nwin = len(loc)
+                w = np.clip(np.searchsorted(e[:nwin + 1], r / T, side="right"), 1, nwin) - 1
+                m = (m - loc[w]) / sc[w]
+            keep = ok & (m > threshold)
+            if np.any(keep):
+                a = np.zeros(int(keep.sum()), dtype=CAND_DTYPE)
+                a["r"] = r[keep]
+                a["freq"] = r[keep] / T
+                a["metric"] = m[keep]
+                a["nharm"] = nh // k
+                a["ducy"] = np.nan
+                parts.append(a)
+    return np.concatenate(parts) if parts else np.zeros(0, dtype=CAND_DTYPE)
+
+
+def _fmt_threshold(x):
+    """cs_format_double (src/candidate.c:336-349): the shortest %g form that
+    reads back exactly, with ".0" appended to a bare integer."""
+    if np.isnan(x):
+        return "NaN"
+    if np.isinf(x):
+        return "Inf" if x > 0 else "-Inf"
+    s = "%.17g" % x
+    for p in range(1, 18):
+        s = "%.*g" % (p, x)
+        if float(s) == x:
+            break
+    if not any(ch in s for ch in ".eE"):
+        s += ".0"
+    return s
+
+
+def _hist_row(h, faps):
+    """src/cli.c:157-168."""
+    N = h.total
+    mean = h.sum / N if N > 0 else np.nan
+    std = np.sqrt(max(0.0, (h.sumsq - N * mean * mean) / (N - 1))) if N > 1 else 0.0
+    med = h.quantile(0.5)
+    fv = [h.quantile(1.0 - f) for f in faps]
+    ov = h.over / N if N > 0 else 0.0
+    return mean, std, med, fv, ov
+
+
+FAPS = [0.1, 0.01, 1e-3, 1e-4, 1e-5]
+
+
+def format_metricstats(ms, nharms, threshold):
+    """<base>_metricstats.txt (src/cli.c:238-253)."""
+    L = ["# Per-block, per-decimation metric statistics\n",
+         "# metric=boxcar nharms=%d threshold=%s\n" % (nharms, _fmt_threshold(threshold)),
+         "# nbins = 2*Hk; the pure-noise metric floor scales ~sqrt(nbins).\n",
+         "#%-7s %3s %4s %5s %10s %14s %14s %8s %9s %9s %9s %9s %9s\n"
+         % ("block", "k", "Hk", "nbins", "ngoodbins", "f_lo(Hz)", "f_hi(Hz)", "n", "min",
+            "median", "mean", "std", "max")]
+    for b in ms["blocks"]:
+        L.append("%-8d %3d %4d %5d %10.3f %14.8f %14.8f %8d %9.3f %9.3f %9.3f %9.3f %9.3f\n"
+                 % (b["block"], b["k"], b["Hk"], b["nbins"], b["ngoodbins"], b["flo"], b["fhi"],
+                    b["n"], b["min"], b["median"], b["mean"], b["std"], b["max"]))
+    return "".join(L)
+
+
+def format_metricfap(ms, nharms):
+    """<base>_metricfap.txt (src/cli.c:255-276)."""
+    L = ["# Per-(decimation, frequency-window) empirical false-alarm thresholds\n",
+         "# metric=boxcar nharms=%d\n" % nharms,
+         "# fap_X = metric value whose single-trial, single-decimation false-alarm prob is X "
+         "(in-window)\n",
+         "#%-3s %4s %5s %4s %13s %13s %10s %8s %8s %8s | %s\n"
+         % ("k", "Hk", "nbins", "win", "f_lo(Hz)", "f_hi(Hz)", "ntrials", "mean", "std",
+            "median", "fap1e-1 fap1e-2 fap1e-3 fap1e-4 fap1e-5")]
+    for h in ms["whists"]:
+        mean, std, med, fv, ov = _hist_row(h, FAPS)
+        s = "%-4d %4d %5d %4d %13.7f %13.7f %10d %8.3f %8.3f %8.3f |" % (
+            h.k, h.Hk, h.nbins, h.win, h.flo, h.fhi, h.total, mean, std, med)
+        s += "".join(" %7.3f" % v for v in fv)
+        if ov > FAPS[4]:
+            s += "  # overflow %.1e" % ov
+        L.append(s + "\n")
+    return "".join(L)
+
+
+def format_metrichist(ms, nharms):
+    """<base>_metrichist.txt (src/cli.c:278-302)."""
+    bw = (ms["hi"] - ms["lo"]) / ms["nb"]
+    L = ["# Per-(decimation, frequency-window) metric histograms (one streaming pass)\n",
+         "# metric=boxcar nharms=%d\n" % nharms,
+         "# bin = left edge of a bin of width %g; count = trials in [bin, bin+width)\n" % bw,
+         "#%-3s %4s %4s %13s %13s %12s %14s\n"
+         % ("k", "Hk", "win", "f_lo(Hz)", "f_hi(Hz)", "bin", "count")]
+    for h in ms["whists"]:
+        nz = np.nonzero(h.counts)[0]
+        if nz.size == 0:
+            continue
+        for b in range(nz[0], nz[-1] + 1):
+            L.append("%-4d %4d %4d %13.7f %13.7f %12.5f %14d\n"
+                     % (h.k, h.Hk, h.win, h.flo, h.fhi, ms["lo"] + b * bw, h.counts[b]))
+    return "".join(L)
