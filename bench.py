@@ -16,6 +16,7 @@ a bounded sample of the same workload), the upload-inclusive rate and the
 parity of the step's candidates.
 """
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -209,6 +210,41 @@ def serving(nreq, slots, rate=20.0):
     }
 
 
+def metric_stats_timing(spec, params, reps=3):
+    """--metricstats / --normalize at the C2 size: search_file collecting the
+    statistics, and normalising, beside the plain search_file on the same warm
+    session (each call uploads the spectrum and collapses its list)."""
+    import coherentsearch_c_amd as cs
+    from coherentsearch_c_amd import abi
+    cache = abi.SearchCache()
+    kw = dict(blocksize=65536, threshold=8.0, cache=cache)
+    base = cs.search_file(spec, params, **kw)
+    ms = cs.MetricStats()
+    cs.search_file(spec, params, metricstats=ms, **kw)       # warm: metric buffer
+    ms.close()
+    out = {}
+    for name, extra in (("plain", {}), ("metricstats", None), ("normalize", {"normalize": True})):
+        ts = []
+        for _ in range(reps):
+            m = cs.MetricStats() if extra is None else None
+            t = time.perf_counter()
+            c = cs.search_file(spec, params, **(dict(metricstats=m) if m else extra), **kw)
+            ts.append(time.perf_counter() - t)
+            if m:
+                v = m.view()
+                m.close()
+        out[name + "_ms"] = 1e3 * min(ts)
+        out[name + "_candidates"] = int(c.size)
+    out["blocks"] = len(v["blocks"])
+    out["windows"] = len(v["whists"])
+    out["metricstats_list_equals_plain"] = bool(
+        base.size == out["metricstats_candidates"])
+    abi.host().search_cache_free(C.byref(cache))
+    log("metric statistics: plain %.1f ms, --metricstats %.1f ms, --normalize %.1f ms"
+        % (out["plain_ms"], out["metricstats_ms"], out["normalize_ms"]))
+    return out
+
+
 def stage_only(sizes, reps=10):
     """Device staging alone (realfft + rednoise of a resident series): each
     size staged `reps` times after two warm-up passes; prints one JSON line
@@ -362,6 +398,7 @@ def main():
                                        "xcd_stripe 0 contiguous / S tiles; blocksize is "
                                        "advice (requests keep 65536)",
                            "classes": tune}
+    out["metric_stats"] = metric_stats_timing(spec, params)
     if not args.no_sweep:
         out["dm_sweep"] = dm_sweep(args.sweep_files, args.slots)
         out["dm_sweep_one_observation"] = dm_sweep_resident(args.sweep_files, args.slots)
