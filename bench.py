@@ -245,6 +245,33 @@ def metric_stats_timing(spec, params, reps=3):
     return out
 
 
+def wide_harmonics(spec, steps=5):
+    """The reference README's wider configurations on the same resident
+    spectrum: nharms 120 with maxdecim 16 (two 64-bit filled words, one wave
+    per SIMD), timed like the headline step."""
+    import coherentsearch_c_amd as cs
+    out = {}
+    for nh, md in ((120, 16), (128, 6)):
+        p = cs.make_params(nharms=nh, m=16, maxdecim=md)
+        sess = cs.Session(spec, p, blocksize=65536)
+        ht, trials, _ = sess.work()
+        sess.upload()
+        sess.search_resident(8.0)
+        kms = []
+        t = time.perf_counter()
+        for _ in range(steps):
+            sess.search_resident(8.0)
+            kms.append(sess.kernel_ms())
+        wall = (time.perf_counter() - t) / steps
+        sess.close()
+        out["nharms%d_maxdecim%d" % (nh, md)] = {
+            "trials": trials, "harmonic_trials": ht, "ms_per_search": 1e3 * wall,
+            "kernel_ms": float(np.mean(kms)), "ht_s": ht / wall}
+        log("nharms %d maxdecim %d: %.2f ms per search, %.4g ht/s" % (nh, md, 1e3 * wall,
+                                                                     ht / wall))
+    return out
+
+
 def stage_only(sizes, reps=10):
     """Device staging alone (realfft + rednoise of a resident series): each
     size staged `reps` times after two warm-up passes; prints one JSON line
@@ -399,6 +426,7 @@ def main():
                                        "advice (requests keep 65536)",
                            "classes": tune}
     out["metric_stats"] = metric_stats_timing(spec, params)
+    out["wide_harmonics"] = wide_harmonics(spec)
     if not args.no_sweep:
         out["dm_sweep"] = dm_sweep(args.sweep_files, args.slots)
         out["dm_sweep_one_observation"] = dm_sweep_resident(args.sweep_files, args.slots)

@@ -8,7 +8,7 @@
  *                                         A.im (WS = m + 4, 16-byte rows)
  *   hinfo   int2[nh]                      (rfloor0, first residue with carry)
  *   tw      float2[TW_SIZE]               twiddles, N(N-1)/2 + j for N <= TW_NMAX
- *   chunks  ChunkDesc[nchunks]            i0, n, filled mask, per-slot nvalid,
+ *   chunks  ChunkDesc[nchunks]            i0, n, filled mask (128 bits), per-slot nvalid,
  *                                         per-slot per-width 1/(sigma*sd_w)
  *   tiles   int4[ntiles]                  (chunk, first trial, count, 0)
  *   recs    CandRec[cap] + counter        above-threshold records
@@ -36,7 +36,7 @@ typedef struct {
     long long i0;
     int n;
     int pad;
-    unsigned long long filled;      /* bit h-1: harmonic h fits the chunk */
+    unsigned long long filled[2];   /* bit (h-1) & 63 of word (h-1) >> 6: harmonic h fits */
     int nvalid[CS_MAXSLOT];         /* trials below the decimated Nyquist */
     int valid[CS_MAXSLOT];          /* 1/sigma > 0 */
     float invsw[CS_MAXSLOT][CS_NSEQ];     /* f32 path: (float)(invsigma*shape_w) */
@@ -56,6 +56,16 @@ typedef struct {
     X(29) X(30) X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41)     \
     X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54)     \
     X(55) X(56) X(57) X(58) X(59) X(60) X(61) X(62) X(63) X(64)
+/* harmonic counts above 64 (up to 128, where the c2r twiddle table ends),
+ * f32 profiles and M = 16 padded taps only: the stack and profiles take
+ * about 440 registers, one wave per SIMD */
+#define CS_NH_BIG_LIST(X)                                                              \
+    X(65) X(66) X(67) X(68) X(69) X(70) X(71) X(72) X(73) X(74) X(75) X(76) X(77)     \
+    X(78) X(79) X(80) X(81) X(82) X(83) X(84) X(85) X(86) X(87) X(88) X(89) X(90)     \
+    X(91) X(92) X(93) X(94) X(95) X(96) X(97) X(98) X(99) X(100) X(101) X(102)        \
+    X(103) X(104) X(105) X(106) X(107) X(108) X(109) X(110) X(111) X(112) X(113)      \
+    X(114) X(115) X(116) X(117) X(118) X(119) X(120) X(121) X(122) X(123) X(124)      \
+    X(125) X(126) X(127) X(128)
 #define CS_KMAX 16
 
 static inline int cs_tw_off(int N) { return N * (N - 1) / 2; }

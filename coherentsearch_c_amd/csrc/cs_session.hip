@@ -41,7 +41,9 @@ typedef hipError_t (*fast_launch_fn)(const csk::FastArgs *, int, hipStream_t);
                                                                   int, hipStream_t);
 #define CS_DECL_NH(NH) CS_DECL_FAST(NH, 16, 16) CS_DECL_FAST(NH, 32, 16) \
     CS_DECL_FAST64(NH, 16, 16) CS_DECL_FAST64(NH, 32, 16)
+#define CS_DECL_NH_BIG(NH) CS_DECL_FAST(NH, 16, 16)
 CS_NH_LIST(CS_DECL_NH)
+CS_NH_BIG_LIST(CS_DECL_NH_BIG)
 
 namespace {
 
@@ -54,7 +56,8 @@ struct FastVariant {
 #define CS_ENTRY_FAST(NH, M, K)                                                    \
     {NH, M, K, cs_launch_fast_##NH##_##M##_##K, cs_launch_fast64_##NH##_##M##_##K},
 #define CS_ENTRY_NH(NH) CS_ENTRY_FAST(NH, 16, 16) CS_ENTRY_FAST(NH, 32, 16)
-const FastVariant kFast[] = {CS_NH_LIST(CS_ENTRY_NH)};
+#define CS_ENTRY_NH_BIG(NH) {NH, 16, 16, cs_launch_fast_##NH##_16_16, nullptr},
+const FastVariant kFast[] = {CS_NH_LIST(CS_ENTRY_NH) CS_NH_BIG_LIST(CS_ENTRY_NH_BIG)};
 
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
@@ -368,7 +371,7 @@ static int build_chunks(GpuSession *S, const DirectPlanSet *dp)
             const int f = csx_harmonic_fits(pl, S->lodr, S->namps, S->N, i0) &&
                           csx_harmonic_fits(pl, S->lodr, S->namps, S->N, i0 + n - 1);
             filled[h - 1] = (uint8_t)f;
-            if (f) C.filled |= 1ull << (h - 1);
+            if (f) C.filled[(h - 1) >> 6] |= 1ull << ((h - 1) & 63);
             nf_all += f;
         }
         S->ht += (long long)nf_all * n;
@@ -510,14 +513,16 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     /* the kernel of the smallest padded tap count M >= m */
     mp = m <= 16 ? 16 : (m <= 32 ? 32 : 0);
     for (const FastVariant &v : kFast) {
-        if (v.nh != nh || v.m != mp) continue;
+        if (v.nh != nh || v.m != mp || (S->f64 && !v.fn64)) continue;
         bool ok = true;
         for (int s = 1; s < S->nslot; s++) ok = ok && S->slot_k[s] <= v.kmax;
         if (ok) { S->fast = &v; break; }
     }
     if (!S->fast) {
-        fprintf(stderr, "Error: no GPU kernel for nharms=%d m=%d with these decimations\n",
-                nh, m);
+        fprintf(stderr, "Error: no GPU kernel for nharms=%d m=%d %s with these decimations "
+                        "(kernels: nharms 1-64 with m <= 32 in f32 or f64, nharms 65-128 with "
+                        "m <= 16 in f32, decimations up to %d)\n",
+                nh, m, S->f64 ? "f64" : "f32", CS_KMAX);
         goto fail;
     }
     S->q = dplans->plans[0].q;

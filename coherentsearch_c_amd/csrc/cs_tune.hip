@@ -225,7 +225,7 @@ extern "C" CS_API int cs_gpu_tune_load(const char *path)
                    &k.f64, &k.measured, &r.lane_order, &r.xcd_stripe, &r.blocksize,
                    &r.default_ms, &r.best_ms) != 11)
             continue;
-        if (k.nh < 1 || k.nh > 64 || (k.mp != 16 && k.mp != 32) || r.xcd_stripe < 0 ||
+        if (k.nh < 1 || k.nh > 128 || (k.mp != 16 && k.mp != 32) || r.xcd_stripe < 0 ||
             r.xcd_stripe > 4096 || r.lane_order < 0 || r.lane_order > 1 || r.blocksize < 1)
             continue;
         k.arch = arch;

@@ -414,18 +414,19 @@ __device__ __forceinline__ void sample_out(const T (&x)[NB], bool active, int lj
 }
 
 /* Occupancy target: f32 profiles 2 waves/SIMD for the largest stacks, 3
- * below; f64 profiles (twice the registers) 1 above 16 harmonics, 2 below. */
+ * below, 1 above 64 harmonics; f64 profiles (twice the registers) 1 above 16
+ * harmonics, 2 below. */
 template <typename T>
 constexpr int waves_for(int nh)
 {
-    return sizeof(T) == 4 ? (nh > 32 ? 2 : 3) : (nh > 16 ? 1 : 2);
+    return sizeof(T) == 4 ? (nh > 64 ? 1 : (nh > 32 ? 2 : 3)) : (nh > 16 ? 1 : 2);
 }
 
 template <int NH, int M, int KMAX, typename T>
 __global__ __launch_bounds__(CS_TILE) __attribute__((amdgpu_waves_per_eu(waves_for<T>(NH))))
 void fast_search_kernel(FastArgs a)
 {
-    static_assert(NH >= 1 && NH <= 64, "fast path holds up to 64 harmonics");
+    static_assert(NH >= 1 && NH <= 128, "fast path holds up to 128 harmonics");
     static_assert(M % 4 == 0, "tap count multiple of 4");
     __shared__ float2 win[CS_WIN_CAP(NH, M)];
     __shared__ int s_off[NH + 1];
@@ -448,7 +449,7 @@ void fast_search_kernel(FastArgs a)
     const int4 Tl = a.tiles[tile];
     const ChunkDesc *C = a.chunks + Tl.x;
     const long long i0 = C->i0;
-    const unsigned long long filled = C->filled;
+    const unsigned long long filled0 = C->filled[0], filled1 = C->filled[1];
     const int tid = threadIdx.x;
     /* Trials t and t+q have the same residue t*h*p mod q at every harmonic,
      * hence the same weight rows.  The first grp*q lanes take the tile's
@@ -487,13 +488,26 @@ void fast_search_kernel(FastArgs a)
     if (tid == 0) s_off[0] = 0;
     __syncthreads();
     if (tid < 64) {
-        /* inclusive scan of lengths over the first wave */
-        int v = (tid < NH) ? s_off[tid + 1] : 0;
-        for (int d = 1; d < 64; d <<= 1) {
-            const int u = __shfl_up(v, d);
-            if (tid >= d) v += u;
+        /* inclusive scan of lengths over the first wave; above 64 harmonics
+         * each lane takes a pair (2*tid, 2*tid+1) */
+        if constexpr (NH <= 64) {
+            int v = (tid < NH) ? s_off[tid + 1] : 0;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int u = __shfl_up(v, d);
+                if (tid >= d) v += u;
+            }
+            if (tid < NH) s_off[tid + 1] = v;
+        } else {
+            const int i0 = 2 * tid, i1 = 2 * tid + 1;
+            const int a0 = i0 < NH ? s_off[i0 + 1] : 0, a1 = i1 < NH ? s_off[i1 + 1] : 0;
+            int v = a0 + a1;
+            for (int d = 1; d < 64; d <<= 1) {
+                const int u = __shfl_up(v, d);
+                if (tid >= d) v += u;
+            }
+            if (i0 < NH) s_off[i0 + 1] = v - a1;
+            if (i1 < NH) s_off[i1 + 1] = v;
         }
-        if (tid < NH) s_off[tid + 1] = v;
     }
     __syncthreads();
     if (tid < NH) s_C[tid] += s_off[tid];
@@ -556,7 +570,7 @@ void fast_search_kernel(FastArgs a)
             });
             const float4 A = wr[M / 4];
             const float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
-            const bool fit = (filled >> (h - 1)) & 1ull;
+            const bool fit = (((h - 1) < 64 ? filled0 : filled1) >> ((h - 1) & 63)) & 1ull;
             X[h] = {fit ? p1 - p2 : 0.0f, fit ? p3 + p4 : 0.0f};
         });
     }

@@ -89,6 +89,31 @@ def test_other_configurations_bit_identical(oracle, nh, m, maxdecim):
     sess.close()
 
 
+@pytest.mark.parametrize("nh,maxdecim,sigma,blocksize", [(65, 6, "analytic", 2048),
+                                                         (72, 16, "analytic", 4096),
+                                                         (100, 6, "measured", 2048),
+                                                         (120, 16, "analytic", 8192),
+                                                         (128, 8, "analytic", 2048)])
+def test_above_64_harmonics_bit_identical(oracle, nh, maxdecim, sigma, blocksize):
+    """nharms 65-128 (m 16, f32): the filled-harmonic mask spans two words and
+    the upper harmonics stop fitting chunk by chunk in this band; the stack
+    and every per-trial metric are bit-identical to the oracle's."""
+    spec, p, o, sess = _case(oracle, nh, 16, nbins=1 << 15, blocksize=blocksize,
+                             maxdecim=maxdecim, sigma=sigma, r0=61.7)
+    say("nh", nh, "maxdecim", maxdecim, sigma, "trials", o.total, "chunks", o.nchunks)
+    for ci in sorted({0, o.nchunks // 2, o.nchunks - 1}):
+        i0, n, _ = o.chunk(ci)
+        want, filled = o.stack(ci)
+        got = sess.dump_stack(i0, n)
+        assert np.array_equal(got.transpose(1, 0, 2).view(np.uint32), want.view(np.uint32)), ci
+    got = sess.run(-np.inf)
+    want = o.search(-np.inf, 1)
+    assert got.size == want.size
+    for f in ("r", "nharm", "metric"):
+        assert np.array_equal(got[f], want[f]), f
+    sess.close()
+
+
 @pytest.mark.parametrize("nh,m,maxdecim", [(16, 16, 6), (32, 32, 6), (60, 16, 6), (7, 8, 3),
                                             (45, 16, 12)])
 def test_f64_every_trial_metric_bit_identical(oracle, nh, m, maxdecim):
@@ -225,6 +250,7 @@ def test_session_reuse_and_mismatch(oracle):
 def test_unsupported_configuration_fails_loudly():
     amps = synth.spectrum_with_pulsar(1 << 14, 300.1, 4, 0.05, 5.0, 1)
     spec = cs.Spectrum(amps, 1 << 15, synth.DT)
-    for kw in (dict(nharms=16, m=34), dict(nharms=65), dict(nharms=16, hidr=0.9)):
+    for kw in (dict(nharms=16, m=34), dict(nharms=129), dict(nharms=72, m=32),
+               dict(nharms=72, precision="f64"), dict(nharms=16, hidr=0.9)):
         with pytest.raises(RuntimeError):
             cs.Session(spec, cs.make_params(**kw), blocksize=2048)
