@@ -229,13 +229,17 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
         ft.namps = N / 2;
         ft.DC = 1.0;
         long long nchunks = 0;
-        if (csx_plans_build(&P->set, &P->p, r_lo) != 0 ||
+        /* the stage plan first and the samples' copy started on the stream,
+         * so that the DMA runs under the host's planning of the search */
+        if (!(P->g = cs_stage_create_on(N, dt, 6, 100, 6.0, E->device, st)) ||
+            cs_stage_copy_in(P->g, r->in.samples, st) != 0 ||
+            csx_plans_build(&P->set, &P->p, r_lo) != 0 ||
             csx_chunk_starts(&P->set, lodr, ft.namps, N, total, Nprof, &P->cstarts,
                              &nchunks) != 0 ||
             !(P->S = cs_session_create_on(&ft, &P->p, &P->set, P->cstarts, total, Nprof,
-                                          nchunks, r_lo, lodr, E->device, st)) ||
-            !(P->g = cs_stage_create_on(N, dt, 6, 100, 6.0, E->device, st))) {
+                                          nchunks, r_lo, lodr, E->device, st))) {
             plan_free(P);
+            hipStreamSynchronize(st);      /* the copy may still read the samples */
             return -1;
         }
     }
@@ -253,10 +257,10 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
             cs_gpu_session_zap(S, &r->zl, r->in.baryv, r->in.amps_out) < 0)
             goto done;
     } else if (r->zl.n > 0) {
-        if (cs_gpu_session_stage(S, g, r->in.samples, NULL) != 0 ||
+        if (cs_session_stage_loaded(S, g, NULL) != 0 ||
             cs_gpu_session_zap(S, &r->zl, r->in.baryv, r->in.amps_out) < 0)
             goto done;
-    } else if (cs_gpu_session_stage(S, g, r->in.samples, r->in.amps_out) != 0) {
+    } else if (cs_session_stage_loaded(S, g, r->in.amps_out) != 0) {
         goto done;
     }
     t2 = clk::now();

@@ -971,6 +971,25 @@ __attribute__((visibility("hidden"))) int cs_session_stage_graph(GpuSession *S, 
     return cs_stage_enqueue_graph(g, samples, S->d_amps, S->st);
 }
 
+/* stage samples already copied into g (cs_stage_copy_in on the session's
+ * stream) into the resident spectrum; the spectrum is copied to amps_host when
+ * given; synchronises */
+__attribute__((visibility("hidden"))) int cs_session_stage_loaded(GpuSession *S, CsStage *g,
+                                                                  float *amps_host)
+{
+    if (!S || !g || cs_stage_N(g) != S->N || S->N / 2 != S->namps) return -1;
+    if (cs_stage_run(g, S->d_amps, S->st) != 0) return -1;
+    if (amps_host &&
+        hipMemcpyAsync(amps_host, S->d_amps, (size_t)S->namps * sizeof(float2),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(S->st) != hipSuccess) {
+        fprintf(stderr, "Error: device staging failed: %s\n", hipGetErrorString(hipGetLastError()));
+        return -1;
+    }
+    return 0;
+}
+
 extern "C" CS_API int cs_gpu_session_set_graphs(GpuSession *S, int enable)
 {
     if (!S) return -1;

@@ -31,6 +31,15 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
                                                                        float2 *d_out,
                                                                        hipStream_t st);
 
+/* the two halves of cs_stage_enqueue: the samples' copy into the plan, and the
+ * staging kernels on samples already there (a caller can do host work while
+ * the copy runs) */
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_copy_in(CsStage *g,
+                                                                       const float *samples,
+                                                                       hipStream_t st);
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_run(CsStage *g, float2 *d_out,
+                                                                   hipStream_t st);
+
 /* the same, with the staging kernels replayed from a graph captured once per
  * output buffer */
 extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue_graph(CsStage *g,
@@ -70,6 +79,10 @@ __attribute__((visibility("hidden"))) void cs_session_bind_stream(GpuSession *s,
  * plan's graph, on the session's stream (not synchronised) */
 __attribute__((visibility("hidden"))) int cs_session_stage_graph(GpuSession *s, CsStage *g,
                                                                  const float *samples);
+/* stage samples already copied into g (cs_stage_copy_in, same stream) into the
+ * session's spectrum, copying it to amps_host when given; synchronised */
+__attribute__((visibility("hidden"))) int cs_session_stage_loaded(GpuSession *s, CsStage *g,
+                                                                  float *amps_host);
 
 /* launch shape of the search kernel (no effect on results) */
 __attribute__((visibility("hidden"))) void cs_session_set_launch(GpuSession *s, int lane_order,

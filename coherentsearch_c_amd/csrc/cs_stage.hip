@@ -942,6 +942,22 @@ fail:
     return -1;
 }
 
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_copy_in(CsStage *g, const float *samples,
+                                                                       hipStream_t st)
+{
+    if (!st) st = g->st;
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    return 0;
+fail:
+    return -1;
+}
+
+extern "C" __attribute__((visibility("hidden"))) int cs_stage_run(CsStage *g, float2 *d_out,
+                                                                   hipStream_t st)
+{
+    return stage_kernels(g, d_out, st ? st : g->st);
+}
+
 /* The samples' upload, then the staging kernels replayed from a graph
  * captured once per output buffer (kernels only).  0 on success. */
 extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue_graph(CsStage *g,
