@@ -113,6 +113,9 @@ def dm_sweep(nfiles, slots):
     fh, wh, stage_s = eng.run_reference_flow(jobs, producers=2, slots=4, threads=threads)
     log("  plain, reference list mode: %.3f s, %.4g ht/s (host staging %.3f s per file on %d "
         "threads per producer)" % (wh, ht / wh, float(np.mean(stage_s)), threads))
+    fl, wl, stage_l = eng.run_reference_flow(jobs, producers=2, slots=4, device_staging=True)
+    log("  plain, this library's list mode (producers stage on the GPU): %.3f s, %.4g ht/s"
+        % (wl, ht / wl))
     # host staging differs from device staging in the last float bits of a few
     # bins (both FFTs are double precision): lists agree within the tolerance
     host_ok = True
@@ -129,6 +132,8 @@ def dm_sweep(nfiles, slots):
         "ratio_vs_plain_reference_flow": wh / wb,
         "reference_flow": "2 producers x %d host threads, 4 slots; mean host realfft+rednoise "
                           "%.3f s per file" % (threads, float(np.mean(stage_s))),
+        "plain_list_mode_device_staging_s": wl, "plain_list_mode_device_staging_ht_s": ht / wl,
+        "ratio_vs_plain_list_mode_device_staging": wl / wb,
         "plain_device_s": wp, "plain_device_ht_s": ht / wp,
         "ratio_vs_plain_device": wp / wb,
         "lists_identical_batched_vs_plain_device": all(_same_lists(a, b) for a, b in zip(fb, fp)),

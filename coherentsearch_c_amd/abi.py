@@ -185,7 +185,8 @@ HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "measure_ducy", "write_candidates", "cs_format_double",
                 "read_candidates_file", "inf_read", "inf_free", "zaplist_read",
                 "zaplist_free", "zaplist_apply", "fftfile_open",
-                "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_set_threads", "datpipe_redfft",
+                "fftfile_close", "datpipe_create", "datpipe_free", "datpipe_set_threads", "datpipe_set_device",
+                "datpipe_redfft",
                 "datpipe_redfft_mem", "cs_open_input", "cs_write_inf", "cs_write_floats",
                 "metricstats_init", "metricstats_free", "metricstats_append_block",
                 "metricstats_append_hist", "metricstats_append_whist", "hist_init", "hist_free",
@@ -244,6 +245,8 @@ def _declare(L):
     L.datpipe_free.argtypes = [P]
     L.datpipe_set_threads.argtypes = [P, I]
     L.datpipe_set_threads.restype = None
+    L.datpipe_set_device.argtypes = [P, I]
+    L.datpipe_set_device.restype = None
     L.datpipe_redfft_mem.argtypes = [P, P, LL, D, C.c_char_p, Ps(FFTFile)]
     L.datpipe_redfft_mem.restype = I
     L.cs_write_inf.argtypes = [C.c_char_p, LL, D, D, C.c_char_p]

@@ -735,6 +735,7 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
     } else {
         SCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
         g->own_stream = 1;
+        g->async_alloc = true;        /* stream-ordered on its own stream too */
     }
     {
         std::vector<double2> &tw = h_tw;
@@ -997,16 +998,20 @@ extern "C" CS_API int cs_gpu_stage_run_host(CsStage *g, const float *samples, fl
     if (!g || !samples || !amps) return -1;
     float2 *d_out = nullptr;
     SCHK(hipSetDevice(g->device));
-    SCHK(hipMalloc(&d_out, g->n2 * sizeof(float2)));
+    /* stream-ordered: no device-wide synchronisation, so staging overlaps
+     * work on other streams (a search in progress) */
+    SCHK(hipMallocAsync((void **)&d_out, g->n2 * sizeof(float2), g->st));
     if (cs_stage_enqueue(g, samples, d_out, g->st) != 0) goto fail;
     SCHK(hipMemcpyAsync(amps, d_out, g->n2 * sizeof(float2), hipMemcpyDeviceToHost, g->st));
     if (packed)
         SCHK(hipMemcpyAsync(packed, g->d_packed, g->n2 * sizeof(float2), hipMemcpyDeviceToHost,
                             g->st));
+    SCHK(hipFreeAsync(d_out, g->st));
+    d_out = nullptr;
     SCHK(hipStreamSynchronize(g->st));
-    hipFree(d_out);
     return 0;
 fail:
-    if (d_out) hipFree(d_out);
+    if (d_out) hipFreeAsync(d_out, g->st);
+    hipStreamSynchronize(g->st);
     return -1;
 }

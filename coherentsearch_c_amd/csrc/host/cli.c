@@ -6,12 +6,15 @@
  * list's directory, per-file search_file with one shared SearchCache, then
  * sort by S/N, truncate to --ncands, duty cycles and .cohout.  With more than
  * one input, producer threads prepare the next inputs while the current one
- * is searched (src/cli.c:556-686).  The search always runs on the GPU
- * (--gpu N picks the device, default 0), including --metricstats and
- * --normalize, which the reference runs on the CPU only: their report goes to
- * stderr and <base>_metricstats.txt, _metricfap.txt and _metrichist.txt
- * (src/cli.c:1049-1091).  The plotting options are rejected with exit
- * status 2; the rest of the CPU-only knobs are accepted and ignored.
+ * is searched (src/cli.c:556-686).  .dat inputs are realfft'd and
+ * rednoise'd on the GPU (on the producers' own streams, beside the search)
+ * unless --host-staging asks for the host transform the reference uses.  The
+ * search always runs on the GPU (--gpu N picks the device, default 0),
+ * including --metricstats and --normalize, which the reference runs on the
+ * CPU only: their report goes to stderr and <base>_metricstats.txt,
+ * _metricfap.txt and _metrichist.txt (src/cli.c:1049-1091).  The plotting
+ * options are rejected with exit status 2; the rest of the CPU-only knobs are
+ * accepted and ignored.
  */
 #define _GNU_SOURCE
 #include "host_internal.h"
@@ -33,6 +36,7 @@ typedef struct {
     cs_sigma sigma;
     int remove, harm_remove, verbose, gpu_device;
     int metricstats, normalize;
+    int host_staging;
 } Opts;
 
 static void usage(FILE *f, int list)
@@ -55,6 +59,8 @@ static void usage(FILE *f, int list)
 "      --baryv V       v/c applied to non-'B' zaplist lines (0.0)\n"
 "      --metricstats   per-block and per-(k, frequency) metric statistics report\n"
 "      --normalize     metric in noise units per (k, frequency window), two passes\n"
+"      --host-staging  realfft + rednoise of .dat inputs on the host (default: on\n"
+"                      the GPU for lengths whose half is 7-smooth)\n"
 "  -v, --verbose       --threads N  --nowisdom  --wisdomfile F  --progressbar\n"
 "      --noprogress    --noplot     --help\n");
 }
@@ -251,7 +257,7 @@ static void pf_finish(Prefetch *pf)
     for (int s = 0; s < pf->nslots; s++) datpipe_free(pf->pipe[s]);
 }
 
-static int pf_start(Prefetch *pf, char **paths, int n)
+static int pf_start(Prefetch *pf, char **paths, int n, int device)
 {
     memset(pf, 0, sizeof(*pf));
     pf->paths = paths;
@@ -269,6 +275,7 @@ static int pf_start(Prefetch *pf, char **paths, int n)
             return -1;
         }
         datpipe_set_threads(pf->pipe[s], share > 1 ? share : 1);
+        datpipe_set_device(pf->pipe[s], device);
     }
     for (int t = 0; t < pf->nprod; t++) {
         pf->arg[t].pf = pf;
@@ -370,7 +377,8 @@ static int run(int argc, char **argv, int list)
                 const long v = strtol(argv[i + 1], &e, 10);
                 if (e && e != argv[i + 1] && *e == '\0' && v >= 0) { o.gpu_device = (int)v; i++; }
             }
-        } else if (!strcmp(name, "--metricstats")) o.metricstats = 1;
+        } else if (!strcmp(name, "--host-staging")) o.host_staging = 1;
+        else if (!strcmp(name, "--metricstats")) o.metricstats = 1;
         else if (!strcmp(name, "--normalize")) o.normalize = 1;
         else if (!strcmp(name, "--plot") || !strcmp(name, "--plotstem") ||
                    !strcmp(name, "--plotcols") || !strcmp(name, "--plotrows") ||
@@ -440,10 +448,11 @@ static int run(int argc, char **argv, int list)
     Prefetch pf;
     int use_pf = 0;
     if (o.nfiles > 1) {
-        use_pf = pf_start(&pf, o.files, o.nfiles) == 0;
+        use_pf = pf_start(&pf, o.files, o.nfiles, o.host_staging ? -1 : o.gpu_device) == 0;
         if (!use_pf) fprintf(stderr, "Warning: cannot start the prefetch threads; running sequentially\n");
     }
     DatPipe *pipe = use_pf ? NULL : datpipe_create(6, 100, 6.0);
+    if (pipe) datpipe_set_device(pipe, o.host_staging ? -1 : o.gpu_device);
     int status = 0;
     for (int fi = 0; fi < o.nfiles; fi++) {
         const char *path = o.files[fi];

@@ -6,7 +6,9 @@
  * endwidth 100, endfreq 6 Hz (running-median power normalisation with the
  * fast inverse square root and the lower-median quickselect).  The complex
  * transform is this library's double-precision FFT rounded to float where the
- * reference uses FFTW in single precision.
+ * reference uses FFTW in single precision.  With datpipe_set_device the same
+ * staging runs on the GPU (cs_gpu_stage_*), for lengths whose half has no
+ * prime factor above 7; other lengths stay on the host.
  */
 #include "host_internal.h"
 
@@ -25,6 +27,13 @@ struct DatPipe {
     float *packed;       /* N floats, realfft output */
     int startwidth, endwidth;
     double endfreq;
+    int device;          /* GPU for the staging, -1 for the host transform */
+    CsStage *gst;        /* device stage plan for (gN, gdt) */
+    long long gN;
+    double gdt;
+    int red_pinned;      /* red came from cs_host_alloc */
+    float *xin;          /* pinned sample buffer of the device staging, grow-only */
+    long long xcap;
 };
 
 static float fast_invsqrtf(float x)
@@ -210,7 +219,13 @@ DatPipe *datpipe_create(int startwidth, int endwidth, double endfreq)
     p->startwidth = startwidth > 0 ? startwidth : 6;
     p->endwidth = endwidth > 0 ? endwidth : 100;
     p->endfreq = endfreq > 0.0 ? endfreq : 6.0;
+    p->device = -1;
     return p;
+}
+
+void datpipe_set_device(DatPipe *p, int device)
+{
+    if (p) p->device = device >= 0 ? device : -1;
 }
 
 void datpipe_set_threads(DatPipe *p, int threads)
@@ -221,35 +236,106 @@ void datpipe_set_threads(DatPipe *p, int threads)
 void datpipe_free(DatPipe *p)
 {
     if (!p) return;
-    free(p->red);
+    if (p->gst) cs_gpu_stage_free(p->gst);
+    if (p->red_pinned) cs_host_free(p->red); else free(p->red);
+    if (p->xin) cs_host_free(p->xin);
     free(p->work);
     free(p->scratch);
     free(p->packed);
     free(p);
 }
 
+/* the output buffer for N samples; the host transform's buffers are
+ * allocated on first use (host_buffers) */
 static int prepare(DatPipe *p, long long N)
 {
-    if (p->N == N && p->red) return 0;
-    free(p->red); free(p->work); free(p->scratch); free(p->packed);
+    const int pin = p->device >= 0;
+    if (p->N == N && p->red && p->red_pinned == pin) return 0;
+    if (p->red_pinned) cs_host_free(p->red); else free(p->red);
+    free(p->work); free(p->scratch); free(p->packed);
     p->red = NULL; p->work = NULL; p->scratch = NULL; p->packed = NULL;
     p->N = 0;
     if (N <= 0 || (N & 1)) return -1;
-    p->red = (float *)malloc((size_t)N * sizeof(float));
-    p->work = (double *)malloc((size_t)N * sizeof(double));
-    p->scratch = (double *)malloc((size_t)N * sizeof(double));
-    p->packed = (float *)malloc((size_t)N * sizeof(float));
-    if (!p->red || !p->work || !p->scratch || !p->packed) return -1;
+    /* the device staging copies the spectrum back into pinned memory: a
+     * pageable destination would be copied through a bounce buffer */
+    p->red = pin ? (float *)cs_host_alloc(N * (long long)sizeof(float))
+                 : (float *)malloc((size_t)N * sizeof(float));
+    if (!p->red) return -1;
+    p->red_pinned = pin;
     p->N = N;
+    return 0;
+}
+
+/* room for N samples: pinned (grow-only) for the device staging, else malloc */
+static float *sample_buffer(DatPipe *p, long long N)
+{
+    if (p->device < 0) return (float *)malloc((size_t)N * sizeof(float));
+    if (p->xcap < N) {
+        if (p->xin) cs_host_free(p->xin);
+        p->xin = (float *)cs_host_alloc(N * (long long)sizeof(float));
+        p->xcap = p->xin ? N : 0;
+    }
+    return p->xin;
+}
+
+static void sample_release(DatPipe *p, float *x)
+{
+    if (x != p->xin) free(x);
+}
+
+static int host_buffers(DatPipe *p)
+{
+    const long long N = p->N;
+    if (!p->work) p->work = (double *)malloc((size_t)N * sizeof(double));
+    if (!p->scratch) p->scratch = (double *)malloc((size_t)N * sizeof(double));
+    if (!p->packed) p->packed = (float *)malloc((size_t)N * sizeof(float));
+    return p->work && p->scratch && p->packed ? 0 : -1;
+}
+
+/* 1 when every prime factor of n is at most 7 (the device transform's plans) */
+static int smooth7(long long n)
+{
+    if (n < 1) return 0;
+    for (int f = 2; f <= 7; f++)
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+/* realfft + rednoise on the GPU into p->red (cs_gpu_stage_*, the stage plan
+ * kept while N and dt repeat): 0 done, 1 when the device transform has no plan
+ * for this length (the host transform stages it), -1 on a device error */
+static int stage_device(DatPipe *p, const float *samples, long long N, double dt)
+{
+    if (!smooth7(N / 2)) return 1;
+    if (!p->gst || p->gN != N || p->gdt != dt) {
+        if (p->gst) cs_gpu_stage_free(p->gst);
+        p->gst = cs_gpu_stage_create(N, dt, p->startwidth, p->endwidth, p->endfreq, p->device);
+        p->gN = N;
+        p->gdt = dt;
+        if (!p->gst) {
+            fprintf(stderr, "Error: no device stage plan for N=%lld on GPU %d\n", N, p->device);
+            return -1;
+        }
+    }
+    if (cs_gpu_stage_run_host(p->gst, samples, p->red, NULL) != 0) {
+        fprintf(stderr, "Error: device staging failed for N=%lld on GPU %d\n", N, p->device);
+        return -1;
+    }
     return 0;
 }
 
 static int stage(DatPipe *p, const float *samples, long long N, double dt, FFTFile *ft)
 {
-    if (realfft_forward(samples, N, p->work, p->scratch, p->threads, p->packed) != 0) return -1;
-    if (rednoise(p->packed, p->red, N / 2, N, dt, p->startwidth, p->endwidth, p->endfreq,
-                 p->threads) != 0)
-        return -1;
+    const int dev = p->device >= 0 ? stage_device(p, samples, N, dt) : 1;
+    if (dev < 0) return -1;
+    if (dev > 0) {
+        if (host_buffers(p) != 0) return -1;
+        if (realfft_forward(samples, N, p->work, p->scratch, p->threads, p->packed) != 0)
+            return -1;
+        if (rednoise(p->packed, p->red, N / 2, N, dt, p->startwidth, p->endwidth, p->endfreq,
+                     p->threads) != 0)
+            return -1;
+    }
     ft->N = N;
     ft->T = (double)N * dt;
     ft->df = 1.0 / ft->T;
@@ -294,16 +380,16 @@ int datpipe_redfft(DatPipe *p, const char *datpath, FFTFile *ft)
         fprintf(stderr, "Error: cannot open '%s'\n", datpath);
         return -1;
     }
-    float *x = (float *)malloc((size_t)N * sizeof(float));
+    float *x = sample_buffer(p, N);
     const size_t got = x ? fread(x, sizeof(float), (size_t)N, f) : 0;
     fclose(f);
     if (got != (size_t)N) {
         fprintf(stderr, "Error: '%s' holds %zu floats, expected %lld\n", datpath, got, N);
-        free(x);
+        sample_release(p, x);
         return -1;
     }
     const int rc = stage(p, x, N, ft->inf.dt, ft);
-    free(x);
+    sample_release(p, x);
     return rc;
 }
 

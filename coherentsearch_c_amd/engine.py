@@ -257,7 +257,8 @@ class _HostSpectrum:
         self.ft = ft
 
 
-def run_reference_flow(jobs, device=0, ncands=100, producers=2, slots=4, threads=None):
+def run_reference_flow(jobs, device=0, ncands=100, producers=2, slots=4, threads=None,
+                       device_staging=False):
     """The reference's list mode on in-memory series (src/cli.c:556-686,
     996-1082): `producers` threads realfft + rednoise upcoming series on the
     host, input i into DatPipe slot i % `slots` once input i - `slots` is
@@ -265,8 +266,10 @@ def run_reference_flow(jobs, device=0, ncands=100, producers=2, slots=4, threads
     GPU with one SearchCache shared across inputs (a session is rebuilt
     whenever the geometry changes), sort by S/N, keep `ncands`, duty cycles
     on the host.  Each producer's transforms run on `threads` OpenMP threads
-    (default: an equal share of this process's cores).  Returns (final lists,
-    wall seconds, per-input host staging seconds)."""
+    (default: an equal share of this process's cores); with device_staging
+    the producers stage on the GPU instead (datpipe_set_device), as this
+    library's CLI does.  Returns (final lists, wall seconds, per-input
+    staging seconds)."""
     L = abi.host()
     n = len(jobs)
     nslots, nprod = min(slots, n), min(producers, n)
@@ -275,6 +278,8 @@ def run_reference_flow(jobs, device=0, ncands=100, producers=2, slots=4, threads
     pipes = [L.datpipe_create(6, 100, 6.0) for _ in range(nslots)]
     for pp in pipes:
         L.datpipe_set_threads(pp, threads)
+        if device_staging:
+            L.datpipe_set_device(pp, device)
     fts = [abi.FFTFile() for _ in range(nslots)]
     held, ok = [-1] * nslots, [False] * nslots
     released, stop = [0], [False]

@@ -89,6 +89,11 @@ CS_API void datpipe_free(DatPipe *p);
  * default.  Callers that stage several files at once give each pipeline a
  * share of the cores. */
 CS_API void datpipe_set_threads(DatPipe *p, int threads);
+/* No counterpart in the reference: stage on GPU `device` (realfft + rednoise
+ * by cs_gpu_stage_*, the spectrum copied back to the pipeline's buffer) for
+ * lengths whose half has no prime factor above 7, the host transform for the
+ * others; -1 (the default) stages every length on the host. */
+CS_API void datpipe_set_device(DatPipe *p, int device);
 CS_API int  datpipe_redfft(DatPipe *p, const char *datpath, FFTFile *ft);
 CS_API int  datpipe_redfft_mem(DatPipe *p, const float *samples, long long N,
                                double dt, const char *path, FFTFile *ft);

@@ -2,10 +2,11 @@
 prepared by the prefetch producers while the GPU searches.
 
 Bar: every .cohout the list run writes is byte-identical to the one-file path
-of the same library (host realfft + rednoise, search_file, sort, truncate,
-duty cycles) run on that input alone, whatever order the producers finish
-in; an input that cannot be opened sets exit status 1 and the others are
-still written.
+of the same library run on that input alone -- realfft + rednoise on the GPU
+(lengths whose half is 7-smooth) or on the host (the others, and every input
+with --host-staging), then search_file, sort, truncate, duty cycles --
+whatever order the producers finish in; an input that cannot be opened sets
+exit status 1 and the others are still written.
 """
 import ctypes as C
 import sys
@@ -47,23 +48,39 @@ def test_list_mode_with_prefetch(oracle, tmp_path):
     # an input that cannot be opened, in the middle of the list
     lst = tmp_path / "sweep.dat.list"
     lst.write_text("\n".join(names[:3] + ["missing_DM9.dat"] + names[3:]) + "\n")
-    outdir = tmp_path / "out"
-    argv = [b"coherent_search_list", b"--gpu", b"0", b"-n", b"32", b"--blocksize", b"8192",
-            b"-t", b"7", b"--outdir", str(outdir).encode(), str(lst).encode()]
-    arr = (C.c_char_p * len(argv))(*argv)
-    say("list of", len(names) + 1, "inputs")
-    assert L.cs_main_list(len(argv), arr) == 1
-    assert not (outdir / "missing_DM9.cohout").exists()
     p = cs.make_params(nharms=32)
-    for i, (name, x) in enumerate(zip(names, series)):
-        spec = cs.redfft(x, dt)
-        want = cs.finalize(spec, p, cs.search_file(spec, p, blocksize=8192, threshold=7.0))
-        text = (outdir / name.replace(".dat", ".cohout")).read_text()
-        say(name, want.size, "candidates")
-        assert text == oracle.format_candidates(want.astype(oracle.CAND_DTYPE)), name
-        if i == 1:
-            f0 = 1.0 / 7.3e-3
-            assert want.size and any(abs(f / f0 - round(f / f0)) < 2e-3 for f in want["freq"][:3])
+
+    def smooth7(n):
+        for f in (2, 3, 5, 7):
+            while n % f == 0:
+                n //= f
+        return n == 1
+
+    for host in (False, True):
+        outdir = tmp_path / ("out_host" if host else "out_dev")
+        argv = [b"coherent_search_list", b"--gpu", b"0", b"-n", b"32", b"--blocksize", b"8192",
+                b"-t", b"7", b"--outdir", str(outdir).encode(), str(lst).encode()]
+        if host:
+            argv.insert(1, b"--host-staging")
+        arr = (C.c_char_p * len(argv))(*argv)
+        say("list of", len(names) + 1, "inputs, host staging" if host else "device staging")
+        assert L.cs_main_list(len(argv), arr) == 1
+        assert not (outdir / "missing_DM9.cohout").exists()
+        for i, (name, x) in enumerate(zip(names, series)):
+            if host or not smooth7(x.size // 2):
+                spec = cs.redfft(x, dt)
+            else:
+                g = cs.Stage(x.size, dt)
+                spec = cs.Spectrum(g.run_host(x), x.size, dt)
+                g.close()
+            want = cs.finalize(spec, p, cs.search_file(spec, p, blocksize=8192, threshold=7.0))
+            text = (outdir / name.replace(".dat", ".cohout")).read_text()
+            say(name, want.size, "candidates")
+            assert text == oracle.format_candidates(want.astype(oracle.CAND_DTYPE)), name
+            if i == 1:
+                f0 = 1.0 / 7.3e-3
+                assert want.size and any(abs(f / f0 - round(f / f0)) < 2e-3
+                                         for f in want["freq"][:3])
 
 
 def test_reference_flow_equals_one_at_a_time():
@@ -83,3 +100,10 @@ def test_reference_flow_equals_one_at_a_time():
         assert a.size == b.size, i
         for f in ("r", "nharm", "metric", "ducy"):
             assert np.array_equal(a[f], b[f], equal_nan=True), (i, f)
+    # producers staging on the GPU: the lists of the device-staged inputs
+    # agree with the host-staged ones within the metric tolerance
+    dev, wall_d, _ = eng.run_reference_flow(jobs, producers=2, slots=2, device_staging=True)
+    say("device-staging producers %.3f s" % wall_d)
+    for i, (a, b) in enumerate(zip(dev, want)):
+        assert a.size == b.size and np.array_equal(a["nharm"], b["nharm"]), i
+        assert np.all(np.abs(a["metric"] - b["metric"]) <= 1e-4 * np.maximum(np.abs(b["metric"]), 1))
