@@ -159,6 +159,20 @@ assert C.sizeof(SimpleInf) == 80 and C.sizeof(FFTFile) == 168
 assert C.sizeof(SearchParams) == 328 and C.sizeof(DirectPlan) == 144
 assert C.sizeof(Candidate) == 40 and C.sizeof(CandidateList) == 24
 
+class CsDedispConfig(C.Structure):
+    _fields_ = [("numchan", C.c_int), ("lofreq", C.c_double), ("chan_wid", C.c_double),
+                ("dt", C.c_double), ("spectra_per_subint", C.c_int), ("nsub", C.c_int),
+                ("lodm", C.c_double), ("dmstep", C.c_double), ("numdms", C.c_int),
+                ("downsamp", C.c_int), ("numout", C.c_longlong)]
+
+
+class CsDedispInfo(C.Structure):
+    _fields_ = [("blocksperread", C.c_int), ("worklen", C.c_int), ("nsub", C.c_int),
+                ("max_offset", C.c_int), ("numout", C.c_longlong), ("written", C.c_longlong),
+                ("chunk", C.c_longlong), ("dsdt", C.c_double), ("pad", C.c_float),
+                ("pad_", C.c_int)]
+
+
 # Every symbol the public headers declare, per library.
 HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matches",
                "cs_gpu_session_run", "cs_gpu_session_free", "cs_gpu_search_region",
@@ -171,7 +185,12 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_engine_submit", "cs_engine_poll", "cs_engine_next_done", "cs_engine_wait",
                "cs_engine_stats", "cs_engine_free", "cs_engine_set_plan_cache", "cs_host_alloc",
                "cs_host_free", "cs_gpu_session_metric_stats",
-               "cs_gpu_session_metric_candidates"]
+               "cs_gpu_session_metric_candidates",
+               # include/cs_dedisp.h
+               "cs_gpu_dedisp_create", "cs_gpu_dedisp_free", "cs_gpu_dedisp_info",
+               "cs_gpu_dedisp_tables", "cs_gpu_dedisp_run", "cs_gpu_dedisp_series",
+               "cs_gpu_dedisp_copy_out", "cs_gpu_dedisp_last_ms", "cs_choose_good_N",
+               "cs_delay_from_dm"]
 HOST_SYMBOLS = ["params_defaults", "decimation_set", "trial_grid_rational",
                 "direct_plans_build", "direct_plans_free", "cs_chunk_state",
                 "cs_harmonic_fits", "harmonic_change_trials", "chunk_starts", "cs_sinpi",
@@ -362,6 +381,26 @@ def _declare(L):
     L.cs_host_alloc.restype = P
     L.cs_host_free.argtypes = [P]
     L.cs_host_free.restype = None
+    L.cs_gpu_dedisp_create.argtypes = [Ps(CsDedispConfig), LL, I]
+    L.cs_gpu_dedisp_create.restype = P
+    L.cs_gpu_dedisp_free.argtypes = [P]
+    L.cs_gpu_dedisp_free.restype = None
+    L.cs_gpu_dedisp_info.argtypes = [P, Ps(CsDedispInfo)]
+    L.cs_gpu_dedisp_info.restype = I
+    L.cs_gpu_dedisp_tables.argtypes = [P, P, P, P]
+    L.cs_gpu_dedisp_tables.restype = I
+    L.cs_gpu_dedisp_run.argtypes = [P, P, I]
+    L.cs_gpu_dedisp_run.restype = I
+    L.cs_gpu_dedisp_series.argtypes = [P, I]
+    L.cs_gpu_dedisp_series.restype = P
+    L.cs_gpu_dedisp_copy_out.argtypes = [P, I, I, P]
+    L.cs_gpu_dedisp_copy_out.restype = I
+    L.cs_gpu_dedisp_last_ms.argtypes = [P, Ps(D), Ps(D)]
+    L.cs_gpu_dedisp_last_ms.restype = D
+    L.cs_choose_good_N.argtypes = [LL]
+    L.cs_choose_good_N.restype = LL
+    L.cs_delay_from_dm.argtypes = [D, D]
+    L.cs_delay_from_dm.restype = D
     L.free_ = C.CDLL(None).free
     L.free_.argtypes = [P]
 

@@ -1,0 +1,572 @@
+/*
+ * De-dispersion of raw filterbank spectra into HBM-resident DM series
+ * (include/cs_dedisp.h): the reference prep front end's hot loop, PRESTO
+ * prepsubband's subband de-dispersion (src/prep_dedisp.c:1104-1427;
+ * third_party/presto/src/dispersion.c:216-345, cuda.cu:505-768).
+ *
+ * The reference walks the data in work blocks of `worklen` spectra, each
+ * de-dispersed against the next one (a two-block window), because its
+ * readers return one block at a time.  The arithmetic is a continuous
+ * formula:
+ *   sub[s][t]  = sum over the channels c of subband s, ascending, of
+ *                raw[t + idispdt[c]][c]                          (from 0.0f)
+ *   subds[s][u] = (sum_{k<ds} sub[s][u*ds + k]) / ds             (ds > 1)
+ *   dm[i][u]   = -0.0f + sum over s ascending of subds[s][u + off[i][s]]
+ * for u below the length the reference writes, then padding with the running
+ * mean of DM 0's written samples.  This file computes that formula in device
+ * passes of `chunk` spectra (a multiple of the raw block and at least one
+ * work block, sized to fill the GPU), so the pass size is the device's
+ * choice, not the reader's: the sums, their order and the written length are
+ * the reference's, the results bit-identical.
+ *
+ * Passes stream through two rings: raw spectra (three chunks: the pass reads
+ * chunks j and j+1 while chunk j+2 is copied in on a second stream; skipped
+ * when the spectra are already in HBM) and downsampled subbands (two chunks:
+ * a DM pass reads its chunk and the next one).  The DM series go straight to
+ * their resident [numdms][numout] array.
+ *
+ * Kernels: HBM-bound.  subband_kernel reads every raw sample once
+ * (numchan * sizeof(T) bytes per spectrum) and writes nsub floats per
+ * downsampled sample; dm_kernel reads nsub floats per output from the
+ * subband ring (L2-resident) and writes one float per DM and sample.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cs_dedisp.h"
+
+#define DCHK(x)                                                                    \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess) {                                                    \
+            fprintf(stderr, "Error: HIP %s failed at %s:%d: %s\n", #x, __FILE__,   \
+                    __LINE__, hipGetErrorString(e_));                              \
+            goto fail;                                                             \
+        }                                                                          \
+    } while (0)
+
+namespace {
+
+constexpr int kWaves = 4;              /* waves per workgroup: one subband or DM each */
+constexpr long long kChunkTarget = 1 << 18;
+constexpr long long kRingBytes = 1ll << 30;
+
+/* PRESTO's NEAREST_LONG: halves away from zero (src/prep_dedisp.c:37) */
+long nearest_long(double x)
+{
+    return (long)(x < 0 ? std::ceil(x - 0.5) : std::floor(x + 0.5));
+}
+
+double delay_from_dm(double dm, double f)
+{
+    return f == 0.0 ? 0.0 : dm / (0.000241 * f * f);
+}
+
+/* delays of n channels of width w from lofreq up; the doppler factor
+ * 1 + v/c is exactly 1 with v/c = 0 (dispersion.c:100-121) */
+void chan_delays(int n, double dm, double lofreq, double w, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = delay_from_dm(dm, lofreq + i * w);
+}
+
+/* every subband's highest channel (dispersion.c:131-150) */
+void subband_delays(int numchan, int nsub, double dm, double lofreq, double cw, double *out)
+{
+    const double sbw = cw * (numchan / nsub);
+    chan_delays(nsub, dm, lofreq + sbw - cw, sbw, out);
+}
+
+bool smooth11(long long n)
+{
+    for (int f = 2; f <= 11; f++)
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+bool power_of_10(long long n)
+{
+    while (n > 9 && n % 10 == 0) n /= 10;
+    return n == 1;
+}
+
+struct Tables {
+    int bpr = 0, worklen = 0, maxoff = 0, maxd = 0;
+    long long numout = 0, written = 0;
+    double dsdt = 0.0;
+    std::vector<double> dms;
+    std::vector<int> idispdt, offsets;
+};
+
+int build_tables(const CsDedispConfig &c, long long nspec, Tables &t)
+{
+    const int nch = c.numchan, nsub = c.nsub, nd = c.numdms, ds = c.downsamp, P = c.spectra_per_subint;
+    if (nch <= 0 || nsub <= 0 || nd <= 0 || ds <= 0 || P <= 0 || !(c.dt > 0.0)) {
+        fprintf(stderr, "Error: de-dispersion needs positive channels, subbands, DMs, "
+                        "downsampling, block length and dt\n");
+        return -1;
+    }
+    if (nsub > nch || nch % nsub) {
+        fprintf(stderr, "Error: %d subbands do not divide %d channels\n", nsub, nch);
+        return -1;
+    }
+    if (P % ds) {
+        fprintf(stderr, "Error:  The downsample factor (%d) must be a factor of the\n"
+                        "        blocklength (%d).\n", ds, P);
+        return -1;
+    }
+    if (nspec < P || nspec % P) {
+        fprintf(stderr, "Error: %lld spectra are not a whole number of %d-spectrum blocks\n",
+                nspec, P);
+        return -1;
+    }
+    t.dms.resize(nd);
+    double avgdm = 0.0;
+    for (int i = 0; i < nd; i++) {
+        t.dms[i] = c.lodm + i * c.dmstep;
+        avgdm += t.dms[i];
+    }
+    avgdm /= nd;
+    const double maxdm = t.dms[nd - 1];
+    t.dsdt = ds * c.dt;
+    const double bw = delay_from_dm(maxdm, c.lofreq) -
+                      delay_from_dm(maxdm, c.lofreq + (nch - 1) * c.chan_wid);
+    t.bpr = (int)(bw / c.dt) / P + 1;
+    t.worklen = P * t.bpr;
+    t.numout = c.numout > 0 ? c.numout : cs_choose_good_N(nspec / ds);
+    std::vector<double> cd(nch), sd(nsub);
+    const int cps = nch / nsub;
+    subband_delays(nch, nsub, avgdm, c.lofreq, c.chan_wid, sd.data());
+    chan_delays(nch, avgdm, c.lofreq, c.chan_wid, cd.data());
+    for (int s = 0; s < nsub; s++)
+        for (int j = 0; j < cps; j++) cd[s * cps + j] -= sd[s];
+    t.idispdt.resize(nch);
+    t.maxd = 0;
+    for (int ch = 0; ch < nch; ch++) {
+        const long d = nearest_long(cd[ch] / c.dt);
+        if (d < 0 || d > P) {
+            fprintf(stderr, "Error: there is more dispersion across a subband than time\n"
+                            "in a block of data (channel %d: %ld samples, block %d).\n",
+                    ch, d, P);
+            return -1;
+        }
+        t.idispdt[ch] = (int)d;
+        t.maxd = std::max(t.maxd, (int)d);
+    }
+    const int dsW = t.worklen / ds;
+    t.offsets.resize((size_t)nd * nsub);
+    t.maxoff = 0;
+    for (int i = 0; i < nd; i++) {
+        subband_delays(nch, nsub, t.dms[i], c.lofreq, c.chan_wid, sd.data());
+        const double top = sd[nsub - 1];
+        for (int s = 0; s < nsub; s++) {
+            const long o = nearest_long((sd[s] - top) / t.dsdt);
+            if (o < 0 || o > dsW) {
+                fprintf(stderr, "Error: subband offset %ld of DM %g outside [0, %d]\n", o,
+                        t.dms[i], dsW);
+                return -1;
+            }
+            t.offsets[(size_t)i * nsub + s] = (int)o;
+            t.maxoff = std::max(t.maxoff, (int)o);
+        }
+    }
+    /* The reference writes work block m only once work block m + 1 was
+     * read whole, and its first read takes two raw blocks: m runs while
+     * (m + 2) * worklen + P <= nspec (src/prep_dedisp.c:485-526,
+     * 1104-1290; backend_common.c:1706-1737). */
+    long long nblocks = (nspec - P) / t.worklen - 1;
+    if (nblocks < 0) nblocks = 0;
+    t.written = std::min(nblocks * dsW, t.numout);
+    return 0;
+}
+
+template <typename T>
+__device__ __forceinline__ float sample(const T *p)
+{
+    return (float)*p;
+}
+
+/* One downsampled subband sample per lane: wave w of the workgroup takes
+ * subband blockIdx.y * kWaves + w, lanes take consecutive samples u.  ra/rb:
+ * the pass's raw chunk and the next one (chunk spectra each, [t][numchan]). */
+template <typename T>
+__global__ void __launch_bounds__(64 * kWaves)
+subband_kernel(const T *__restrict__ ra, const T *__restrict__ rb, long long chunk, int nchan,
+               int cps, int nsub, int ds, const int *__restrict__ idisp, int nu,
+               float *__restrict__ out, int dsc)
+{
+    const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * kWaves + (threadIdx.x >> 6)));
+    const int u = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    if (s >= nsub || u >= nu) return;
+    const int c0 = s * cps;
+    float acc = 0.0f;
+    for (int k = 0; k < ds; k++) {
+        const long long t0 = (long long)u * ds + k;
+        float v = 0.0f;
+        for (int j = 0; j < cps; j++) {
+            const int ch = c0 + j;
+            const long long t = t0 + idisp[ch];
+            const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
+            v += sample(row + ch);
+        }
+        acc = ds > 1 ? acc + v : v;
+    }
+    out[(long long)s * dsc + u] = ds > 1 ? acc / (float)ds : acc;
+}
+
+/* One DM sample per lane: wave w takes DM blockIdx.y * kWaves + w.  sa/sb:
+ * the pass's subband chunk and the next ([s][dsc] each). */
+__global__ void __launch_bounds__(64 * kWaves)
+dm_kernel(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, int nsub,
+          const int *__restrict__ offs, int numdms, int nu, float *__restrict__ out,
+          long long numout, long long u0)
+{
+    const int i = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * kWaves + (threadIdx.x >> 6)));
+    const int u = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
+    if (i >= numdms || u >= nu) return;
+    const int *o = offs + (long long)i * nsub;
+    float acc = -0.0f;
+    for (int s = 0; s < nsub; s++) {
+        const int t = u + o[s];
+        acc += t < dsc ? sa[(long long)s * dsc + t] : sb[(long long)s * dsc + (t - dsc)];
+    }
+    out[(long long)i * numout + u0 + u] = acc;
+}
+
+__global__ void pad_kernel(float *__restrict__ out, long long numout, long long written,
+                           int numdms, float v)
+{
+    const long long n = numout - written;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n * numdms;
+         k += (long long)gridDim.x * blockDim.x) {
+        const long long i = k / n, u = k - i * n;
+        out[i * numout + written + u] = v;
+    }
+}
+
+}  // namespace
+
+struct CsDedisp {
+    CsDedispConfig c;
+    Tables t;
+    int device = 0;
+    long long nspec = 0, chunk = 0;
+    int dsc = 0;                   /* downsampled samples per chunk */
+    hipStream_t st = nullptr, cst = nullptr;
+    int *d_idisp = nullptr, *d_offs = nullptr;
+    float *d_sub = nullptr;        /* 2 x nsub x dsc */
+    float *d_out = nullptr;        /* numdms x numout */
+    void *d_ring = nullptr;        /* 3 chunks of raw spectra (host input) */
+    size_t ring_bytes = 0;
+    float *h_dm0 = nullptr;        /* pinned: DM 0's written samples */
+    float pad = 0.0f;
+    double last_ms = 0.0, last_sub_ms = 0.0, last_dm_ms = 0.0;
+};
+
+extern "C" CS_API long long cs_choose_good_N(long long n)
+{
+    if (n <= 0) return 0;
+    /* the first four digits of n (src: misc_utils.c:334-373); the good
+     * factors are 1000 and the multiples of 16 in (1000, 10000] without a
+     * prime factor above 11, ascending */
+    char digits[32];
+    snprintf(digits, sizeof(digits), "%lld", n);
+    digits[4] = '\0';
+    const long long first4 = atoll(digits);
+    long long g = 0;
+    for (long long v = 1000; v <= 10000; v++) {
+        if (v != 1000 && (v % 16 || !smooth11(v))) continue;
+        g = v;
+        if (v == first4 && n % v == 0 && power_of_10(n / v)) break;
+        if (v > first4) break;
+    }
+    while (g < n) g *= 10;
+    long long two = 2;
+    while (two < n) two *= 2;
+    return two < g ? two : g;
+}
+
+extern "C" CS_API double cs_delay_from_dm(double dm, double freq_mhz)
+{
+    return delay_from_dm(dm, freq_mhz);
+}
+
+extern "C" CS_API void cs_gpu_dedisp_free(CsDedisp *d)
+{
+    if (!d) return;
+    if (d->device < 0) {
+        delete d;
+        return;
+    }
+    hipSetDevice(d->device);
+    if (d->st) hipStreamSynchronize(d->st);
+    if (d->cst) hipStreamSynchronize(d->cst);
+    hipFree(d->d_idisp);
+    hipFree(d->d_offs);
+    hipFree(d->d_sub);
+    hipFree(d->d_out);
+    hipFree(d->d_ring);
+    if (d->h_dm0) hipHostFree(d->h_dm0);
+    if (d->st) hipStreamDestroy(d->st);
+    if (d->cst) hipStreamDestroy(d->cst);
+    delete d;
+}
+
+extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long long nspec,
+                                                 int device)
+{
+    if (!c) return nullptr;
+    CsDedisp *d = new CsDedisp();
+    d->c = *c;
+    d->device = device;
+    d->nspec = nspec;
+    if (build_tables(*c, nspec, d->t) != 0) {
+        delete d;
+        return nullptr;
+    }
+    {
+        const Tables &t = d->t;
+        const int P = c->spectra_per_subint, nsub = c->nsub;
+        /* a pass: whole raw blocks, at least one work block, about 2^18
+         * spectra, and a raw ring within 1 GiB at 4 bytes a sample */
+        long long want = std::max<long long>(t.worklen, kChunkTarget);
+        const long long cap = kRingBytes / (3ll * c->numchan * 4);
+        if (want > cap) want = std::max<long long>(t.worklen, cap);
+        d->chunk = (want + P - 1) / P * P;
+        d->dsc = (int)(d->chunk / c->downsamp);
+        if (device < 0) return d;     /* tables only */
+        DCHK(hipSetDevice(device));
+        DCHK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
+        DCHK(hipStreamCreateWithFlags(&d->cst, hipStreamNonBlocking));
+        DCHK(hipMalloc(&d->d_idisp, sizeof(int) * t.idispdt.size()));
+        DCHK(hipMalloc(&d->d_offs, sizeof(int) * t.offsets.size()));
+        DCHK(hipMemcpy(d->d_idisp, t.idispdt.data(), sizeof(int) * t.idispdt.size(),
+                       hipMemcpyHostToDevice));
+        DCHK(hipMemcpy(d->d_offs, t.offsets.data(), sizeof(int) * t.offsets.size(),
+                       hipMemcpyHostToDevice));
+        DCHK(hipMalloc(&d->d_sub, sizeof(float) * 2 * (size_t)nsub * d->dsc));
+        DCHK(hipMalloc(&d->d_out, sizeof(float) * (size_t)c->numdms * t.numout));
+        if (t.written > 0)
+            DCHK(hipHostMalloc((void **)&d->h_dm0, sizeof(float) * (size_t)t.written,
+                               hipHostMallocDefault));
+    }
+    return d;
+fail:
+    cs_gpu_dedisp_free(d);
+    return nullptr;
+}
+
+extern "C" CS_API int cs_gpu_dedisp_info(const CsDedisp *d, CsDedispInfo *o)
+{
+    if (!d || !o) return -1;
+    memset(o, 0, sizeof(*o));
+    o->blocksperread = d->t.bpr;
+    o->worklen = d->t.worklen;
+    o->nsub = d->c.nsub;
+    o->max_offset = d->t.maxoff;
+    o->numout = d->t.numout;
+    o->written = d->t.written;
+    o->chunk = d->chunk;
+    o->dsdt = d->t.dsdt;
+    o->pad = d->pad;
+    return 0;
+}
+
+extern "C" CS_API int cs_gpu_dedisp_tables(const CsDedisp *d, double *dms, int *idispdt,
+                                           int *offsets)
+{
+    if (!d) return -1;
+    if (dms) memcpy(dms, d->t.dms.data(), sizeof(double) * d->t.dms.size());
+    if (idispdt) memcpy(idispdt, d->t.idispdt.data(), sizeof(int) * d->t.idispdt.size());
+    if (offsets) memcpy(offsets, d->t.offsets.data(), sizeof(int) * d->t.offsets.size());
+    return 0;
+}
+
+extern "C" CS_API const float *cs_gpu_dedisp_series(const CsDedisp *d, int i)
+{
+    if (!d || !d->d_out || i < 0 || i >= d->c.numdms) return nullptr;
+    return d->d_out + (size_t)i * d->t.numout;
+}
+
+extern "C" CS_API int cs_gpu_dedisp_copy_out(const CsDedisp *d, int i0, int n, float *host)
+{
+    if (!d || !d->d_out || !host || i0 < 0 || n < 0 || i0 + n > d->c.numdms) return -1;
+    DCHK(hipSetDevice(d->device));
+    DCHK(hipMemcpy(host, d->d_out + (size_t)i0 * d->t.numout,
+                   sizeof(float) * (size_t)n * d->t.numout, hipMemcpyDeviceToHost));
+    return 0;
+fail:
+    return -1;
+}
+
+extern "C" CS_API double cs_gpu_dedisp_last_ms(const CsDedisp *d, double *subband_ms,
+                                               double *dm_ms)
+{
+    if (!d) return -1.0;
+    if (subband_ms) *subband_ms = d->last_sub_ms;
+    if (dm_ms) *dm_ms = d->last_dm_ms;
+    return d->last_ms;
+}
+
+template <typename T>
+static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
+{
+    const Tables &t = d->t;
+    const CsDedispConfig &c = d->c;
+    const int nch = c.numchan, nsub = c.nsub, cps = nch / nsub, ds = c.downsamp;
+    const long long C = d->chunk, dsc = d->dsc;
+    const long long tsub = t.written > 0 ? t.written + t.maxoff : 0;
+    const long long rows = tsub * ds + t.maxd;        /* raw spectra the passes read */
+    const long long nk1 = (tsub + dsc - 1) / dsc, nk2 = (t.written + dsc - 1) / dsc;
+    const size_t row_bytes = sizeof(T) * (size_t)nch;
+    std::vector<hipEvent_t> ev;
+    hipEvent_t raw_ready[3] = {}, k1_done[3] = {};
+    int rc = -1;
+    auto new_event = [&](hipEvent_t *e, unsigned flags) {
+        if (hipEventCreateWithFlags(e, flags) != hipSuccess) return false;
+        ev.push_back(*e);
+        return true;
+    };
+    std::vector<hipEvent_t> k1s(nk1), k1e(nk1), k2s(nk2), k2e(nk2);
+    const T *ring = nullptr;
+    auto chunk_ptr = [&](long long j) -> const T * {
+        return on_device ? spectra + j * C * nch : ring + (j % 3) * C * nch;
+    };
+    /* host spectra: chunk j of the input into ring slot j % 3 once the pass
+     * that last read that slot (j - 3's pass reads j - 3 and j - 2) is done */
+    auto copy_chunk = [&](long long j) -> bool {
+        const long long r0 = j * C, nr = std::min(C, rows - r0);
+        if (on_device || nr <= 0) return true;
+        if (j >= 3 && hipStreamWaitEvent(d->cst, k1_done[j % 3], 0) != hipSuccess) return false;
+        if (hipMemcpyAsync((void *)(ring + (j % 3) * C * nch), spectra + r0 * nch,
+                           row_bytes * (size_t)nr, hipMemcpyDefault, d->cst) != hipSuccess)
+            return false;
+        return hipEventRecord(raw_ready[j % 3], d->cst) == hipSuccess;
+    };
+    DCHK(hipSetDevice(d->device));
+    for (int i = 0; i < 3; i++) {
+        if (!new_event(&raw_ready[i], hipEventDisableTiming) ||
+            !new_event(&k1_done[i], hipEventDisableTiming))
+            goto fail;
+    }
+    for (auto *v : {&k1s, &k1e, &k2s, &k2e})
+        for (auto &e : *v)
+            if (!new_event(&e, hipEventDefault)) goto fail;
+    if (!on_device && nk1 > 0) {
+        const size_t need = 3 * (size_t)C * row_bytes;
+        if (d->ring_bytes < need) {
+            hipFree(d->d_ring);
+            d->d_ring = nullptr;
+            d->ring_bytes = 0;
+            DCHK(hipMalloc(&d->d_ring, need));
+            d->ring_bytes = need;
+        }
+        ring = (const T *)d->d_ring;
+    }
+    if (!copy_chunk(0) || !copy_chunk(1)) goto fail;
+    for (long long j = 0; j < nk1; j++) {
+        const long long u0 = j * dsc, nu = std::min(dsc, tsub - u0);
+        float *sub = d->d_sub + (j & 1) * (size_t)nsub * dsc;
+        if (!on_device) {
+            DCHK(hipStreamWaitEvent(d->st, raw_ready[j % 3], 0));
+            if ((j + 1) * C < rows) DCHK(hipStreamWaitEvent(d->st, raw_ready[(j + 1) % 3], 0));
+        }
+        DCHK(hipEventRecord(k1s[j], d->st));
+        {
+            const dim3 grid((unsigned)((nu + 63) / 64), (unsigned)((nsub + kWaves - 1) / kWaves));
+            subband_kernel<T><<<grid, 64 * kWaves, 0, d->st>>>(
+                chunk_ptr(j), chunk_ptr(j + 1), C, nch, cps, nsub, ds, d->d_idisp, (int)nu,
+                sub, (int)dsc);
+            DCHK(hipGetLastError());
+        }
+        DCHK(hipEventRecord(k1e[j], d->st));
+        DCHK(hipEventRecord(k1_done[j % 3], d->st));
+        if (j >= 1 && j - 1 < nk2) {
+            const long long m = j - 1, v0 = m * dsc, nv = std::min(dsc, t.written - v0);
+            DCHK(hipEventRecord(k2s[m], d->st));
+            const dim3 grid((unsigned)((nv + 63) / 64), (unsigned)((c.numdms + kWaves - 1) / kWaves));
+            dm_kernel<<<grid, 64 * kWaves, 0, d->st>>>(
+                d->d_sub + (m & 1) * (size_t)nsub * dsc, d->d_sub + ((m + 1) & 1) * (size_t)nsub * dsc,
+                (int)dsc, nsub, d->d_offs, c.numdms, (int)nv, d->d_out, t.numout, v0);
+            DCHK(hipGetLastError());
+            DCHK(hipEventRecord(k2e[m], d->st));
+        }
+        if (!copy_chunk(j + 2)) goto fail;
+    }
+    /* the last DM pass: its subbands end inside the last subband chunk */
+    for (long long m = std::max<long long>(nk1 - 1, 0); m < nk2; m++) {
+        const long long v0 = m * dsc, nv = std::min(dsc, t.written - v0);
+        DCHK(hipEventRecord(k2s[m], d->st));
+        const dim3 grid((unsigned)((nv + 63) / 64), (unsigned)((c.numdms + kWaves - 1) / kWaves));
+        dm_kernel<<<grid, 64 * kWaves, 0, d->st>>>(
+            d->d_sub + (m & 1) * (size_t)nsub * dsc, d->d_sub + ((m + 1) & 1) * (size_t)nsub * dsc,
+            (int)dsc, nsub, d->d_offs, c.numdms, (int)nv, d->d_out, t.numout, v0);
+        DCHK(hipGetLastError());
+        DCHK(hipEventRecord(k2e[m], d->st));
+    }
+    /* the padding value: the running mean of DM 0's written samples, in
+     * order, as update_stats keeps it (misc_utils.h:114-127) */
+    {
+        double avg = 0.0;
+        if (t.written > 0) {
+            DCHK(hipMemcpyAsync(d->h_dm0, d->d_out, sizeof(float) * (size_t)t.written,
+                                hipMemcpyDeviceToHost, d->st));
+            DCHK(hipStreamSynchronize(d->st));
+            for (long long u = 0; u < t.written; u++) {
+                const double dev = (double)d->h_dm0[u] - avg;
+                avg += dev / ((double)u + 1.0);
+            }
+        }
+        d->pad = (float)avg;
+        if (t.numout > t.written) {
+            pad_kernel<<<1024, 256, 0, d->st>>>(d->d_out, t.numout, t.written, c.numdms, d->pad);
+            DCHK(hipGetLastError());
+        }
+        DCHK(hipStreamSynchronize(d->st));
+        DCHK(hipStreamSynchronize(d->cst));
+    }
+    {
+        double s1 = 0.0, s2 = 0.0;
+        float ms = 0.0f;
+        for (long long j = 0; j < nk1; j++)
+            if (hipEventElapsedTime(&ms, k1s[j], k1e[j]) == hipSuccess) s1 += ms;
+        for (long long m = 0; m < nk2; m++)
+            if (hipEventElapsedTime(&ms, k2s[m], k2e[m]) == hipSuccess) s2 += ms;
+        d->last_sub_ms = s1;
+        d->last_dm_ms = s2;
+        d->last_ms = 0.0;
+        if (nk1 > 0 && hipEventElapsedTime(&ms, k1s[0], nk2 > 0 ? k2e[nk2 - 1] : k1e[nk1 - 1]) ==
+                           hipSuccess)
+            d->last_ms = ms;
+    }
+    rc = 0;
+fail:
+    if (rc != 0) {
+        hipStreamSynchronize(d->st);
+        hipStreamSynchronize(d->cst);
+    }
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+    return rc;
+}
+
+extern "C" CS_API int cs_gpu_dedisp_run(CsDedisp *d, const void *spectra, int is_uint8)
+{
+    if (!d || !spectra) return -1;
+    if (d->device < 0) {
+        fprintf(stderr, "Error: a tables-only de-dispersion plan (device -1) cannot run\n");
+        return -1;
+    }
+    hipPointerAttribute_t attr;
+    bool on_device = false;
+    if (hipPointerGetAttributes(&attr, spectra) == hipSuccess)
+        on_device = attr.type == hipMemoryTypeDevice && attr.device == d->device;
+    (void)hipGetLastError();
+    return is_uint8 ? run_passes(d, (const uint8_t *)spectra, on_device)
+                    : run_passes(d, (const float *)spectra, on_device);
+}

@@ -937,17 +937,19 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue(CsStage *g
                                                                        float2 *d_out, hipStream_t st)
 {
     if (!st) st = g->st;
-    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyDefault, st));
     return stage_kernels(g, d_out, st);
 fail:
     return -1;
 }
 
+/* samples in host memory or in this device's memory (a de-dispersed series
+ * resident in HBM): the copy's direction follows the pointer */
 extern "C" __attribute__((visibility("hidden"))) int cs_stage_copy_in(CsStage *g, const float *samples,
                                                                        hipStream_t st)
 {
     if (!st) st = g->st;
-    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyDefault, st));
     return 0;
 fail:
     return -1;
@@ -967,7 +969,7 @@ extern "C" __attribute__((visibility("hidden"))) int cs_stage_enqueue_graph(CsSt
                                                                              hipStream_t st)
 {
     if (!st) st = g->st;
-    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyHostToDevice, st));
+    SCHK(hipMemcpyAsync(g->d_x, samples, g->N * sizeof(float), hipMemcpyDefault, st));
     if (!g->gexec || g->g_out != d_out) {
         if (g->gexec) hipGraphExecDestroy(g->gexec);
         if (g->graph) hipGraphDestroy(g->graph);

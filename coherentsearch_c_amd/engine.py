@@ -72,7 +72,9 @@ class Engine:
     def submit(self, samples, dt, params, lofreq=0.1, hifreq=125.0, lobin=100, blocksize=0,
                threshold=8.0, spectrum=None, out=None, ncands=0, numharm=16, dr_tol=1.0,
                harm_tol=1.0, remove=True, harm_remove=True, zaplist=None, baryv=0.0):
-        """Queue one search of a raw float32 series; returns its ticket.
+        """Queue one search of a raw float32 series (host memory, a
+        PinnedSeries, or a series resident on the engine's device such as a
+        de-dispersed DM series, dedisp.DeviceSeries); returns its ticket.
 
         ncands > 0: the request returns the final list (collapsed, sorted by
         S/N, cut to ncands, duty cycles measured on the device) and the
@@ -83,20 +85,25 @@ class Engine:
         staged spectrum before the search."""
         if spectrum is None:
             spectrum = ncands <= 0
-        s = samples.a if isinstance(samples, PinnedSeries) else samples
-        s = np.ascontiguousarray(s, dtype=np.float32)
-        if s.size % 2:
+        if hasattr(samples, "ptr") and hasattr(samples, "n"):
+            # a series already in this device's memory (dedisp.DeviceSeries)
+            s, ptr, n = samples, int(samples.ptr), int(samples.n)
+        else:
+            s = samples.a if isinstance(samples, PinnedSeries) else samples
+            s = np.ascontiguousarray(s, dtype=np.float32)
+            ptr, n = s.ctypes.data, s.size
+        if n % 2:
             raise ValueError("the series needs an even number of samples")
         amps = None
         if spectrum:
             amps = out.a if isinstance(out, PinnedSeries) else out
             if amps is None:
-                amps = np.zeros(s.size, dtype=np.float32)
-            if amps.dtype != np.float32 or amps.size != s.size or not amps.flags.c_contiguous:
+                amps = np.zeros(n, dtype=np.float32)
+            if amps.dtype != np.float32 or amps.size != n or not amps.flags.c_contiguous:
                 raise ValueError("spectrum buffer must be N contiguous float32")
         req = abi.CsSearchRequest()
-        req.samples = s.ctypes.data
-        req.N = s.size
+        req.samples = ptr
+        req.N = n
         req.dt = float(dt)
         req.p = C.pointer(params)
         req.lofreq, req.hifreq, req.lobin = float(lofreq), float(hifreq), int(lobin)
@@ -114,7 +121,7 @@ class Engine:
             t = self.L.cs_engine_submit(self.h, C.byref(req))
             if t < 0:
                 raise ValueError("request refused")
-            self._keep[t] = (s, amps, params, float(dt), samples)
+            self._keep[t] = (n, amps, params, float(dt), samples)
         return t
 
     def next_done(self, timeout_ms=-1.0):
@@ -127,12 +134,12 @@ class Engine:
         info = abi.CsSearchInfo()
         self.L.cs_engine_wait(self.h, ticket, C.byref(cl), C.byref(info))
         with self._lock:
-            s, amps, params, dt, _ = self._keep.pop(ticket)
+            n, amps, params, dt, _ = self._keep.pop(ticket)
         raw = take_list(cl)
         d = {k: getattr(info, k) for k, _ in abi.CsSearchInfo._fields_ if k != "pad"}
         spec = None
         if amps is not None and info.status == 0:
-            spec = Spectrum(amps, s.size, dt)
+            spec = Spectrum(amps, n, dt)
         return raw, d, spec
 
     def stats(self):

@@ -130,3 +130,37 @@ def serving_requests(n=500, rate=20.0, seed=5, lo=20, hi=25):
                          period=float(np.exp(rng.uniform(np.log(1e-3), np.log(0.5)))) if puls else 0.0,
                          snr=float(rng.uniform(8, 20)) if puls else 0.0, duty=0.05))
     return reqs
+
+
+def filterbank_with_pulsar(nspec, nchan, lofreq, chan_wid, dt, dm, period, duty, snr, seed,
+                           dtype=np.float32):
+    """[nspec, nchan] spectra, lowest channel first: per-channel Gaussian noise
+    plus a von Mises pulsar delayed in each channel by the cold-plasma law,
+    DM / (0.000241 f^2) s with f in MHz, relative to the highest channel.  snr
+    is the S/N of the series de-dispersed at `dm` (sqrt(nspec) * rms of the
+    summed pulse over the summed noise).  float32 noise is N(0, 1); uint8 is
+    round(96 + 8 N(0, 1) + pulse) clipped to [0, 255]."""
+    rng = np.random.default_rng(seed)
+    scale = 8.0 if dtype == np.uint8 else 1.0
+    x = rng.standard_normal((nchan, nspec), dtype=np.float32)
+    if dtype == np.uint8:
+        x *= scale
+        x += 96.0
+    if snr > 0 and period > 0:
+        kappa = np.log(2.0) / (1.0 - np.cos(np.pi * duty))
+        nb = 4096
+        prof = np.exp(kappa * (np.cos(2 * np.pi * (np.arange(nb) + 0.5) / nb) - 1.0))
+        prof -= prof.mean()
+        prof *= 1.0 / np.sqrt(np.mean(prof ** 2))
+        amp = scale * snr / (np.sqrt(nspec) * np.sqrt(nchan))
+        phase0 = rng.uniform()
+        ftop = lofreq + (nchan - 1) * chan_wid
+        t = np.arange(nspec, dtype=np.float64) * dt
+        for c in range(nchan):
+            f = lofreq + c * chan_wid
+            delay = dm / (0.000241 * f * f) - dm / (0.000241 * ftop * ftop)
+            ph = np.mod((t - delay) / period + phase0, 1.0)
+            x[c] += (amp * prof[(ph * nb).astype(np.int64) % nb]).astype(np.float32)
+    if dtype == np.uint8:
+        x = np.clip(np.rint(x), 0, 255).astype(np.uint8)
+    return np.ascontiguousarray(x.T)

@@ -241,7 +241,9 @@ CS_API CandidateList cs_gpu_session_metric_candidates(GpuSession *s, double thre
 typedef struct CsEngine CsEngine;
 
 typedef struct {
-    const float *samples;   /* host, N floats; must stay valid until the request is done */
+    const float *samples;   /* N floats in host memory or in the engine device's memory
+                               (e.g. a de-dispersed series, cs_gpu_dedisp_*); must stay
+                               valid until the request is done */
     long long N;            /* even */
     double dt;
     const SearchParams *p;  /* copied at submit */

@@ -2267,3 +2267,297 @@ int orc_zap(float *amps, long long namps, long long N, double T, const double *f
     free(b);
     return applied;
 }
+
+/* ------------------------------------------------------------------ */
+/* PRESTO prepsubband de-dispersion as the prep front end runs it      */
+/* (nobary, no clipping, masks or zero-DM): the delay tables of        */
+/* src/prep_dedisp.c:320-331, 362-367, 405-470 with                    */
+/* third_party/presto/src/dispersion.c:100-215 and misc_utils.c:325-373,*/
+/* then the write loop of src/prep_dedisp.c:485-530, 745-790 over      */
+/* get_data's CPU branch (src/prep_dedisp.c:1104-1290), read_subbands  */
+/* and prep_subbands (backend_common.c:936-1000, 1684-1738),           */
+/* dedisp_subbands and float_dedisp (dispersion.c:216-263, 300-345).   */
+/* ------------------------------------------------------------------ */
+static long orc_nearest_long(double x)
+{
+    return (long)(x < 0 ? ceil(x - 0.5) : floor(x + 0.5));
+}
+
+double orc_delay_from_dm(double dm, double f)
+{
+    return f == 0.0 ? 0.0 : dm / (0.000241 * f * f);
+}
+
+/* per-channel delays from the lowest channel up (the doppler factor is
+ * 1 + v/c with v/c = 0, i.e. exactly 1) */
+static void orc_chan_delays(int n, double dm, double lofreq, double width, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = orc_delay_from_dm(dm, lofreq + i * width);
+}
+
+/* delays of every subband's highest channel */
+static void orc_subband_delays(int numchan, int nsub, double dm, double lofreq, double cw,
+                               double *out)
+{
+    const double sbw = cw * (numchan / nsub);
+    orc_chan_delays(nsub, dm, lofreq + sbw - cw, sbw, out);
+}
+
+static int orc_smooth11(long long n)
+{
+    for (int f = 2; f <= 11; f++)
+        while (n % f == 0) n /= f;
+    return n == 1;
+}
+
+static int orc_pow10(long long n)
+{
+    while (n > 9 && n % 10 == 0) n /= 10;
+    return n == 1;
+}
+
+/* The length of the good-factor table is 114: 1000 and every multiple of 16
+ * in (1000, 10000] without a prime factor above 11, ascending. */
+long long orc_choose_good_N(long long n)
+{
+    if (n <= 0) return 0;
+    char digits[32];
+    snprintf(digits, sizeof(digits), "%lld", n);
+    digits[4] = '\0';
+    const long long first4 = atoll(digits);
+    long long g = 0;
+    for (long long v = 1000; v <= 10000; v++) {
+        if (v != 1000 && (v % 16 || !orc_smooth11(v))) continue;
+        g = v;
+        if (v == first4 && n % v == 0 && orc_pow10(n / v)) break;
+        if (v > first4) break;
+    }
+    while (g < n) g *= 10;
+    long long two = 2;
+    while (two < n) two *= 2;
+    return two < g ? two : g;
+}
+
+int orc_dedisp_tables(const OrcDedispConfig *c, long long nspec, double *dms, int *idispdt,
+                      int *offsets, long long *numout)
+{
+    const int nch = c->numchan, nsub = c->nsub, nd = c->numdms, ds = c->downsamp;
+    if (nch <= 0 || nsub <= 0 || nsub > nch || nch % nsub || nd <= 0 || ds <= 0 || c->P <= 0 ||
+        c->P % ds || !(c->dt > 0.0))
+        return -1;
+    double avgdm = 0.0;
+    for (int i = 0; i < nd; i++) {
+        dms[i] = c->lodm + i * c->dmstep;
+        avgdm += dms[i];
+    }
+    avgdm /= nd;
+    const double maxdm = dms[nd - 1], dsdt = ds * c->dt;
+    const double bw = orc_delay_from_dm(maxdm, c->lofreq) -
+                      orc_delay_from_dm(maxdm, c->lofreq + (nch - 1) * c->chan_wid);
+    const int bpr = (int)(bw / c->dt) / c->P + 1;
+    *numout = c->numout > 0 ? c->numout : orc_choose_good_N(nspec / ds);
+    double *cd = (double *)malloc(sizeof(double) * (size_t)nch);
+    double *sd = (double *)malloc(sizeof(double) * (size_t)nsub);
+    const int cps = nch / nsub;
+    orc_subband_delays(nch, nsub, avgdm, c->lofreq, c->chan_wid, sd);
+    orc_chan_delays(nch, avgdm, c->lofreq, c->chan_wid, cd);
+    for (int s = 0; s < nsub; s++)
+        for (int j = 0; j < cps; j++) cd[s * cps + j] -= sd[s];
+    int bad = 0;
+    for (int ch = 0; ch < nch; ch++) {
+        idispdt[ch] = (int)orc_nearest_long(cd[ch] / c->dt);
+        /* the reference refuses more dispersion across a subband than one
+         * raw block (backend_common.c:1706-1712); past it, or below zero,
+         * its loops would index outside the two blocks */
+        if (idispdt[ch] < 0 || idispdt[ch] > c->P) bad = 1;
+    }
+    const int dsW = c->P * bpr / ds;
+    for (int i = 0; i < nd; i++) {
+        orc_subband_delays(nch, nsub, dms[i], c->lofreq, c->chan_wid, sd);
+        const double top = sd[nsub - 1];
+        for (int s = 0; s < nsub; s++) {
+            const int o = (int)orc_nearest_long((sd[s] - top) / dsdt);
+            offsets[i * nsub + s] = o;
+            if (o < 0 || o > dsW) bad = 1;
+        }
+    }
+    free(cd);
+    free(sd);
+    return bad ? -1 : bpr;
+}
+
+typedef struct {
+    const float *raw;
+    long long nblk, next;   /* whole raw blocks, the next one to read */
+    int first;
+    float *last, *cur;      /* raw blocks, [P][numchan] */
+} OrcReader;
+
+/* read_subbands + prep_subbands + dedisp_subbands into sub block `blk` of
+ * fdata ([nsub][W]); P on success, 0 when the data has ended */
+static int orc_read_subbands(OrcReader *r, const OrcDedispConfig *c, const int *idispdt,
+                             float *fdata, int W, int blk)
+{
+    const int P = c->P, nch = c->numchan, cps = nch / c->nsub;
+    const size_t bytes = sizeof(float) * (size_t)P * nch;
+    if (r->first) {
+        if (r->next >= r->nblk) return -1;
+        memcpy(r->last, r->raw + r->next * (long long)P * nch, bytes);
+        r->next++;
+        r->first = 0;
+    }
+    if (r->next >= r->nblk) return 0;
+    memcpy(r->cur, r->raw + r->next * (long long)P * nch, bytes);
+    r->next++;
+    for (int s = 0; s < c->nsub; s++)
+        for (int j = 0; j < P; j++) fdata[(long long)s * W + (long long)blk * P + j] = 0.0f;
+    for (int ch = 0; ch < nch; ch++) {
+        float *sub = fdata + (long long)(ch / cps) * W + (long long)blk * P;
+        const int d = idispdt[ch];
+        int j = 0;
+        for (; j < P - d; j++) sub[j] += r->last[(long long)(j + d) * nch + ch];
+        for (; j < P; j++) sub[j] += r->cur[(long long)(j + d - P) * nch + ch];
+    }
+    float *t = r->last;
+    r->last = r->cur;
+    r->cur = t;
+    return P;
+}
+
+long long orc_dedisp(const OrcDedispConfig *c, const float *raw, long long nspec, float *out,
+                     float *pad)
+{
+    const int nch = c->numchan, nsub = c->nsub, nd = c->numdms, ds = c->downsamp, P = c->P;
+    if (nch <= 0 || nsub <= 0 || P <= 0 || nspec % P) return -1;
+    double *dms = (double *)malloc(sizeof(double) * (size_t)nd);
+    int *idispdt = (int *)malloc(sizeof(int) * (size_t)nch);
+    int *offs = (int *)malloc(sizeof(int) * (size_t)nd * nsub);
+    long long numout = 0;
+    const int bpr = orc_dedisp_tables(c, nspec, dms, idispdt, offs, &numout);
+    if (bpr < 0) {
+        free(dms); free(idispdt); free(offs);
+        return -1;
+    }
+    const int W = P * bpr, dsW = W / ds;
+    OrcReader rd = {raw, nspec / P, 0, 1, NULL, NULL};
+    rd.last = (float *)malloc(sizeof(float) * (size_t)P * nch);
+    rd.cur = (float *)malloc(sizeof(float) * (size_t)P * nch);
+    float *data[2], *dsdata[2];
+    for (int i = 0; i < 2; i++) {
+        data[i] = (float *)calloc((size_t)nsub * W, sizeof(float));
+        dsdata[i] = ds > 1 ? (float *)calloc((size_t)nsub * dsW, sizeof(float)) : data[i];
+    }
+    float *res = (float *)malloc(sizeof(float) * (size_t)nd * dsW);
+    int cur = 0, firsttime = 1, failed = 0;
+    long long totwrote = 0, statnum = 0;
+    double avg = 0.0;
+    for (;;) {
+        /* get_data: one work block read ahead of the one de-dispersed */
+        int totnumread = 0;
+        for (;;) {
+            for (int b = 0; b < bpr; b++) {
+                const int n = orc_read_subbands(&rd, c, idispdt, data[cur], W, b);
+                if (n < 0) { failed = 1; break; }
+                if (n != P)
+                    for (int s = 0; s < nsub; s++)
+                        memset(data[cur] + (long long)s * W + (long long)b * P, 0,
+                               sizeof(float) * (size_t)P);
+                if (!firsttime) totnumread += n;
+            }
+            if (failed) break;
+            if (ds > 1)
+                for (int s = 0; s < nsub; s++)
+                    for (int u = 0; u < dsW; u++) {
+                        const float *x = data[cur] + (long long)s * W + (long long)u * ds;
+                        float acc = 0.0f;
+                        for (int k = 0; k < ds; k++) acc += x[k];
+                        dsdata[cur][(long long)s * dsW + u] = acc / ds;
+                    }
+            if (firsttime) {
+                cur ^= 1;
+                firsttime = 0;
+            } else {
+                break;
+            }
+        }
+        if (failed) break;
+        /* float_dedisp of (last, current) for every DM */
+        const float *lastd = dsdata[cur ^ 1], *curd = dsdata[cur];
+        for (int i = 0; i < nd; i++) {
+            float *r = res + (long long)i * dsW;
+            for (int u = 0; u < dsW; u++) r[u] = -0.0f;
+            for (int s = 0; s < nsub; s++) {
+                const int o = offs[i * nsub + s];
+                int u = 0;
+                for (; u < dsW - o; u++) r[u] += lastd[(long long)s * dsW + o + u];
+                for (; u < dsW; u++) r[u] += curd[(long long)s * dsW + u - (dsW - o)];
+            }
+        }
+        cur ^= 1;
+        if (totnumread != W) break;
+        long long nw = dsW;
+        if (totwrote + nw > numout) nw = numout - totwrote;
+        for (int i = 0; i < nd; i++)
+            memcpy(out + (long long)i * numout + totwrote, res + (long long)i * dsW,
+                   sizeof(float) * (size_t)nw);
+        for (long long u = 0; u < nw; u++) {
+            /* update_stats: the running mean of the first DM's samples */
+            const float x = res[u];
+            const double dev = x - avg;
+            avg += dev / ((double)(statnum + u) + 1.0);
+        }
+        statnum += nw;
+        totwrote += nw;
+        if (totwrote == numout) break;
+    }
+    const float padv = (float)avg;
+    for (int i = 0; i < nd && !failed; i++)
+        for (long long u = totwrote; u < numout; u++) out[(long long)i * numout + u] = padv;
+    if (pad) *pad = padv;
+    for (int i = 0; i < 2; i++) {
+        if (ds > 1) free(dsdata[i]);
+        free(data[i]);
+    }
+    free(res); free(rd.last); free(rd.cur); free(dms); free(idispdt); free(offs);
+    return failed ? -1 : totwrote;
+}
+
+int orc_dedisp_direct(const OrcDedispConfig *c, const float *raw, long long nspec,
+                      long long ntime, float *out)
+{
+    const int nch = c->numchan, nsub = c->nsub, nd = c->numdms, ds = c->downsamp;
+    double *dms = (double *)malloc(sizeof(double) * (size_t)nd);
+    int *idispdt = (int *)malloc(sizeof(int) * (size_t)nch);
+    int *offs = (int *)malloc(sizeof(int) * (size_t)nd * nsub);
+    long long numout = 0;
+    int rc = orc_dedisp_tables(c, nspec, dms, idispdt, offs, &numout) < 0 ? -1 : 0;
+    const int cps = rc == 0 ? nch / nsub : 1;
+    int maxo = 0, maxd = 0;
+    for (int i = 0; rc == 0 && i < nd * nsub; i++) maxo = offs[i] > maxo ? offs[i] : maxo;
+    for (int i = 0; rc == 0 && i < nch; i++) maxd = idispdt[i] > maxd ? idispdt[i] : maxd;
+    const long long nsubt = ntime + maxo;          /* downsampled subband samples needed */
+    if (rc == 0 && nsubt * ds + maxd > nspec) rc = -1;
+    float *sub = rc == 0 ? (float *)malloc(sizeof(float) * (size_t)nsub * nsubt) : NULL;
+    for (int s = 0; rc == 0 && s < nsub; s++)
+        for (long long u = 0; u < nsubt; u++) {
+            float acc = 0.0f;
+            for (int k = 0; k < ds; k++) {
+                const long long t = u * ds + k;
+                float v = 0.0f;
+                for (int j = 0; j < cps; j++) {
+                    const int ch = s * cps + j;
+                    v += raw[(t + idispdt[ch]) * nch + ch];
+                }
+                acc = ds > 1 ? acc + v : v;
+            }
+            sub[(long long)s * nsubt + u] = ds > 1 ? acc / ds : acc;
+        }
+    for (int i = 0; rc == 0 && i < nd; i++)
+        for (long long u = 0; u < ntime; u++) {
+            float acc = -0.0f;
+            for (int s = 0; s < nsub; s++) acc += sub[(long long)s * nsubt + u + offs[i * nsub + s]];
+            out[(long long)i * ntime + u] = acc;
+        }
+    free(sub); free(dms); free(idispdt); free(offs);
+    return rc;
+}

@@ -177,6 +177,38 @@ int orc_rednoise(const float *in, long long N, double dt, float *out);
 int orc_zap(float *amps, long long namps, long long N, double T, const double *freq_hz,
             const double *width_hz, const int *bary, int nbirds, double baryv);
 
+/* ---- PRESTO prepsubband de-dispersion (the prep front end's hot loop) ---- */
+typedef struct {
+    int numchan;          /* channels, lowest frequency first */
+    double lofreq;        /* centre of the lowest channel, MHz */
+    double chan_wid;      /* MHz */
+    double dt;            /* raw sample time, s */
+    int P;                /* spectra per raw block (spectra_per_subint) */
+    int nsub;
+    double lodm, dmstep;
+    int numdms;
+    int downsamp;
+    long long numout;     /* per-DM output length; 0: choose_good_N(nspec / downsamp) */
+} OrcDedispConfig;
+
+double    orc_delay_from_dm(double dm, double freq_mhz);
+long long orc_choose_good_N(long long n);
+/* Delay tables: dms[numdms], idispdt[numchan] (raw samples, within-subband),
+ * offsets[numdms * nsub] (downsampled samples); *numout resolved.  Returns
+ * blocksperread, or -1 for a configuration the reference rejects. */
+int orc_dedisp_tables(const OrcDedispConfig *c, long long nspec, double *dms, int *idispdt,
+                      int *offsets, long long *numout);
+/* The nobary write loop over raw float spectra [nspec][numchan] (time-major):
+ * out[numdms * numout] receives every DM series with its padding.  Returns the
+ * number of de-dispersed samples written per DM (the rest is padding), or -1;
+ * *pad receives the padding value. */
+long long orc_dedisp(const OrcDedispConfig *c, const float *raw, long long nspec, float *out,
+                     float *pad);
+/* The same series by direct (unblocked) indexing: the closed form the
+ * blocked loop must equal.  Writes ntime samples per DM, no padding. */
+int orc_dedisp_direct(const OrcDedispConfig *c, const float *raw, long long nspec,
+                      long long ntime, float *out);
+
 #ifdef __cplusplus
 }
 #endif

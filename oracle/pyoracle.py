@@ -107,6 +107,15 @@ def lib():
         L.orc_realfft.argtypes = [P, LL, P]; L.orc_realfft.restype = I
         L.orc_rednoise.argtypes = [P, LL, D, P]; L.orc_rednoise.restype = I
         L.orc_zap.argtypes = [P, LL, LL, D, P, P, P, I, D]; L.orc_zap.restype = I
+        L.orc_delay_from_dm.argtypes = [D, D]; L.orc_delay_from_dm.restype = D
+        L.orc_choose_good_N.argtypes = [LL]; L.orc_choose_good_N.restype = LL
+        L.orc_dedisp_tables.argtypes = [C.POINTER(OrcDedispConfig), LL, P, P, P,
+                                        C.POINTER(LL)]
+        L.orc_dedisp_tables.restype = I
+        L.orc_dedisp.argtypes = [C.POINTER(OrcDedispConfig), P, LL, P, C.POINTER(C.c_float)]
+        L.orc_dedisp.restype = LL
+        L.orc_dedisp_direct.argtypes = [C.POINTER(OrcDedispConfig), P, LL, LL, P]
+        L.orc_dedisp_direct.restype = I
         L.free = C.CDLL(None).free
         L.free.argtypes = [P]
         _lib = L
@@ -622,3 +631,62 @@ def format_metrichist(ms, nharms):
             L.append("%-4d %4d %4d %13.7f %13.7f %12.5f %14d\n"
                      % (h.k, h.Hk, h.win, h.flo, h.fhi, ms["lo"] + b * bw, h.counts[b]))
     return "".join(L)
+
+
+# ---- PRESTO prepsubband de-dispersion (src/prep_dedisp.c) ----------------------
+
+class OrcDedispConfig(C.Structure):
+    _fields_ = [("numchan", C.c_int), ("lofreq", C.c_double), ("chan_wid", C.c_double),
+                ("dt", C.c_double), ("P", C.c_int), ("nsub", C.c_int),
+                ("lodm", C.c_double), ("dmstep", C.c_double), ("numdms", C.c_int),
+                ("downsamp", C.c_int), ("numout", C.c_longlong)]
+
+
+def dedisp_config(numchan, lofreq, chan_wid, dt, P, nsub, lodm, dmstep, numdms, downsamp=1,
+                  numout=0):
+    return OrcDedispConfig(numchan, lofreq, chan_wid, dt, P, nsub, lodm, dmstep, numdms,
+                           downsamp, numout)
+
+
+def delay_from_dm(dm, f):
+    return lib().orc_delay_from_dm(dm, f)
+
+
+def choose_good_N(n):
+    return lib().orc_choose_good_N(n)
+
+
+def dedisp_tables(cfg, nspec):
+    """(blocksperread, dms, idispdt, offsets[numdms, nsub], numout); blocksperread
+    is -1 for a configuration the reference rejects"""
+    dms = np.zeros(cfg.numdms)
+    idisp = np.zeros(cfg.numchan, np.int32)
+    offs = np.zeros((cfg.numdms, cfg.nsub), np.int32)
+    nout = C.c_longlong(0)
+    bpr = lib().orc_dedisp_tables(C.byref(cfg), nspec, ptr(dms), ptr(idisp), ptr(offs),
+                                  C.byref(nout))
+    return bpr, dms, idisp, offs, nout.value
+
+
+def dedisp(cfg, raw):
+    """the prepsubband write loop over raw float spectra [nspec, numchan]:
+    (series[numdms, numout], samples written per DM, padding value)"""
+    raw = np.ascontiguousarray(raw, np.float32)
+    nspec = raw.shape[0]
+    bpr, _, _, _, numout = dedisp_tables(cfg, nspec)
+    if bpr < 0:
+        raise ValueError("configuration rejected")
+    out = np.zeros((cfg.numdms, numout), np.float32)
+    pad = C.c_float(0)
+    n = lib().orc_dedisp(C.byref(cfg), ptr(raw), nspec, ptr(out), C.byref(pad))
+    if n < 0:
+        raise ValueError("de-dispersion failed")
+    return out, n, pad.value
+
+
+def dedisp_direct(cfg, raw, ntime):
+    raw = np.ascontiguousarray(raw, np.float32)
+    out = np.zeros((cfg.numdms, ntime), np.float32)
+    if lib().orc_dedisp_direct(C.byref(cfg), ptr(raw), raw.shape[0], ntime, ptr(out)) != 0:
+        raise ValueError("direct de-dispersion out of range")
+    return out

@@ -1,0 +1,158 @@
+"""De-dispersion (the prep front end's prepsubband loop) on the CPU, no GPU.
+
+The oracle restates the reference's blocked CPU loop
+(src/prep_dedisp.c:485-530, 745-790, 1104-1290; PRESTO dispersion.c,
+backend_common.c, misc_utils.c).  It is pinned here by:
+- the direct closed form (no blocks) the loop must equal bit for bit, over
+  downsampling factors, subband counts and block lengths;
+- the written length derived from the loop's structure;
+- a dispersed pulse that lines up only at the injected DM;
+- the textbook dispersion constant;
+- choose_good_N as the smallest power of two or good factor times a power of
+  ten at or above N (brute force).
+The library's tables (tables-only plan, device -1) and helpers equal the
+oracle's.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import coherentsearch_c_amd as cs
+from coherentsearch_c_amd import synth
+
+CFGS = [  # numchan, lofreq, chan_wid, dt, P, nsub, lodm, dmstep, numdms, downsamp
+    (64, 1200.0, 2.0, 64e-6, 256, 16, 40.0, 2.0, 21, 2),
+    (32, 400.0, 0.25, 1e-4, 128, 32, 0.0, 1.0, 7, 1),
+    (48, 700.0, 1.0, 5e-5, 192, 4, 10.0, 3.0, 12, 4),
+    (16, 1400.0, 8.0, 1e-4, 256, 1, 0.0, 10.0, 5, 8),
+]
+
+
+def _raw(cfg, nblocks, seed, u8=False):
+    nch, P = cfg[0], cfg[4]
+    x = synth.filterbank_with_pulsar(P * nblocks, nch, cfg[1], cfg[2], cfg[3], 45.0, 0.031,
+                                     0.05, 30.0, seed, dtype=np.uint8 if u8 else np.float32)
+    return x.astype(np.float32)
+
+
+@pytest.mark.parametrize("cfg", CFGS)
+def test_blocked_loop_equals_closed_form(oracle, cfg):
+    c = oracle.dedisp_config(*cfg)
+    raw = _raw(cfg, 48, 3, u8=cfg[0] == 48)
+    out, n, pad = oracle.dedisp(c, raw)
+    bpr, dms, idisp, offs, numout = oracle.dedisp_tables(c, raw.shape[0])
+    W = cfg[4] * bpr
+    # written: work block m goes out once block m + 1 was read whole, and the
+    # first read takes two raw blocks
+    m = max(0, (raw.shape[0] - cfg[4]) // W - 1)
+    assert n == min(m * W // cfg[9], numout) and n > 0
+    direct = oracle.dedisp_direct(c, raw, n)
+    assert np.array_equal(out[:, :n].view(np.uint32), direct.view(np.uint32))
+    # padding: DM 0's mean, on every DM
+    assert np.all(out[:, n:] == np.float32(pad))
+    assert pad == pytest.approx(out[0, :n].astype(np.float64).mean(), rel=1e-6, abs=1e-6)
+
+
+def test_tables_closed_forms(oracle):
+    c = oracle.dedisp_config(*CFGS[0])
+    bpr, dms, idisp, offs, numout = oracle.dedisp_tables(c, 256 * 48)
+    assert np.allclose(dms, 40.0 + 2.0 * np.arange(21))
+    cps = 4
+    # within a subband the top channel has no delay and delays grow downwards;
+    # across DMs the subband offsets grow with DM and vanish for the top subband
+    assert np.all(idisp.reshape(16, cps)[:, -1] == 0)
+    assert np.all(np.diff(idisp.reshape(16, cps), axis=1) <= 0)
+    assert np.all(offs[:, -1] == 0) and np.all(np.diff(offs[:, 0]) >= 0)
+    # offsets are the subband top channels' delays in downsampled samples
+    f = 1200.0 + (np.arange(16) * cps + cps - 1) * 2.0
+    want = np.array([[math.floor((d / (0.000241 * fi * fi) - d / (0.000241 * f[-1] ** 2))
+                                 / 128e-6 + 0.5) for fi in f] for d in dms])
+    assert np.array_equal(offs, want)
+
+
+def test_dispersion_constant(oracle):
+    # PRESTO's 1/0.000241 s MHz^2 is the textbook 4.148808e3 to 1.4e-4
+    for f in (150.0, 1400.0):
+        k = oracle.delay_from_dm(1.0, f) * f * f
+        assert abs(k - 4148.808) / 4148.808 < 2e-4
+    assert oracle.delay_from_dm(10.0, 400.0) == pytest.approx(
+        10.0 * oracle.delay_from_dm(1.0, 400.0), rel=1e-15)
+    assert oracle.delay_from_dm(1.0, 0.0) == 0.0
+
+
+def test_pulse_lines_up_at_its_dm(oracle):
+    nch, lo, cw, dt, P = 64, 1200.0, 4.0, 64e-6, 512
+    c = oracle.dedisp_config(nch, lo, cw, dt, P, 16, 0.0, 20.0, 10, 1)
+    period = 0.02
+    raw = synth.filterbank_with_pulsar(P * 64, nch, lo, cw, dt, 100.0, period, 0.02, 40.0, 5)
+    out, n, _ = oracle.dedisp(c, raw)
+    nb = 64
+    ph = (np.arange(n) * dt / period % 1.0 * nb).astype(int)
+    peak = []
+    for i in range(10):
+        prof = np.bincount(ph, weights=out[i, :n].astype(np.float64), minlength=nb)
+        prof /= np.bincount(ph, minlength=nb)
+        peak.append((prof.max() - prof.mean()) / prof.std())
+    assert int(np.argmax(peak)) == 5, peak          # DM 100 of 0, 20, ..., 180
+
+
+def test_choose_good_N_is_the_smallest_good_length(oracle):
+    good = [1000] + [v for v in range(1001, 10001) if v % 16 == 0 and _smooth11(v)]
+    assert len(good) == 114
+    rng = np.random.default_rng(1)
+    ns = [1, 5, 100, 999, 1000, 1001, 1008, 12000, 12001, 2 ** 20, 2 ** 20 + 1, 851968,
+          6815744] + list(rng.integers(1, 10 ** 9, 300))
+    for n in ns:
+        n = int(n)
+        cands = [1 << k for k in range(1, 40) if (1 << k) >= n]
+        cands += [g * 10 ** k for g in good for k in range(0, 8) if g * 10 ** k >= n]
+        assert oracle.choose_good_N(n) == min(cands), n
+        assert cs.abi.host().cs_choose_good_N(n) == oracle.choose_good_N(n)
+
+
+def _smooth11(v):
+    for p in (2, 3, 5, 7, 11):
+        while v % p == 0:
+            v //= p
+    return v == 1
+
+
+@pytest.mark.parametrize("cfg", CFGS)
+def test_library_tables_equal_oracle(oracle, cfg):
+    c = oracle.dedisp_config(*cfg)
+    nspec = cfg[4] * 48
+    bpr, dms, idisp, offs, numout = oracle.dedisp_tables(c, nspec)
+    d = cs.Dedisp(cfg[0], cfg[1], cfg[2], cfg[3], nspec, cfg[5], cfg[6], cfg[7], cfg[8],
+                  spectra_per_subint=cfg[4], downsamp=cfg[9], device=-1)
+    gd, gi, go = d.tables()
+    info = d.info
+    assert info["blocksperread"] == bpr and info["numout"] == numout
+    assert np.array_equal(gd, dms) and np.array_equal(gi, idisp) and np.array_equal(go, offs)
+    raw = _raw(cfg, 48, 3)
+    _, n, _ = oracle.dedisp(c, raw)
+    assert info["written"] == n
+    for f in (150.0, 1400.0):
+        assert cs.abi.host().cs_delay_from_dm(30.0, f) == oracle.delay_from_dm(30.0, f)
+    d.close()
+
+
+def test_rejections(oracle):
+    base = dict(numchan=64, lofreq=1200.0, chan_wid=2.0, dt=64e-6, nspec=256 * 8, nsub=16,
+                lodm=0.0, dmstep=1.0, numdms=4, spectra_per_subint=256, downsamp=1)
+    for bad in (dict(nsub=24),                         # does not divide 64 channels
+                dict(downsamp=3),                      # not a factor of the block
+                dict(nspec=256 * 8 + 5),               # not whole blocks
+                dict(lodm=5000.0, spectra_per_subint=16, nspec=16 * 64)):  # subband smear > block
+        with pytest.raises(ValueError):
+            cs.Dedisp(**dict(base, **bad), device=-1)
+        args = dict(base, **bad)
+        c = oracle.dedisp_config(args["numchan"], args["lofreq"], args["chan_wid"], args["dt"],
+                                 args["spectra_per_subint"], args["nsub"], args["lodm"],
+                                 args["dmstep"], args["numdms"], args["downsamp"])
+        if args["nspec"] % args["spectra_per_subint"] == 0:
+            assert oracle.dedisp_tables(c, args["nspec"])[0] == -1
+    d = cs.Dedisp(**base, device=-1)
+    with pytest.raises(RuntimeError):
+        d.run(np.zeros((256 * 8, 64), np.float32))
+    d.close()
