@@ -277,6 +277,132 @@ def wide_harmonics(spec, steps=5):
     return out
 
 
+def _filterbank_on_device(nspec, nchan, lo, cw, dt, dm, period, duty, snr, seed):
+    """uint8 [nspec, nchan] spectra generated on the GPU in slabs: round(96 + 8
+    N(0, 1) + pulse) clipped to [0, 255], the pulse a von Mises pulsar delayed
+    per channel by the cold-plasma law (synth.filterbank_with_pulsar's model)"""
+    import torch
+    dev = "cuda:0"
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    out = torch.empty((nspec, nchan), dtype=torch.uint8, device=dev)
+    f = lo + torch.arange(nchan, device=dev, dtype=torch.float64) * cw
+    ftop = lo + (nchan - 1) * cw
+    delay = dm / (0.000241 * f * f) - dm / (0.000241 * ftop * ftop)
+    kappa = np.log(2.0) / (1.0 - np.cos(np.pi * duty))
+    nb = 4096
+    prof = np.exp(kappa * (np.cos(2 * np.pi * (np.arange(nb) + 0.5) / nb) - 1.0))
+    prof -= prof.mean()
+    prof /= np.sqrt(np.mean(prof ** 2))
+    amp = 8.0 * snr / (np.sqrt(nspec) * np.sqrt(nchan))
+    prof_d = torch.tensor(prof * amp, dtype=torch.float32, device=dev)
+    phase0 = np.random.default_rng(seed).uniform()
+    slab = 1 << 15
+    for r0 in range(0, nspec, slab):
+        n = min(slab, nspec - r0)
+        t = torch.arange(r0, r0 + n, device=dev, dtype=torch.float64)[:, None] * dt
+        ph = torch.remainder((t - delay[None, :]) / period + phase0, 1.0)
+        x = torch.randn((n, nchan), generator=g, device=dev, dtype=torch.float32)
+        x.mul_(8.0).add_(96.0).add_(prof_d[(ph * nb).long() % nb])
+        out[r0:r0 + n] = x.round_().clamp_(0, 255).to(torch.uint8)
+    return out
+
+
+def dedisp_timing(steps=5, ref_flow=True):
+    """The prep front end's de-dispersion (PRESTO prepsubband, in memory) of a
+    1024-channel 8-bit filterbank into 128 DM series, and the search of those
+    series: resident in HBM (this library) against the reference's flow
+    (series to the host, then host realfft + rednoise and search_file per DM,
+    src/prep_search.c:44-145)."""
+    import torch
+    import coherentsearch_c_amd as cs
+    from coherentsearch_c_amd.engine import Engine, run_plain
+    nch, lo, cw, dt, P = 1024, 1200.0, 300.0 / 1024, 64e-6, 2048
+    nspec = 1 << 22
+    nsub, lodm, dmstep, nd, ds = 64, 0.0, 0.5, 128, 4
+    period, dm0 = 0.00731, 20.0
+    t0 = time.perf_counter()
+    raw = _filterbank_on_device(nspec, nch, lo, cw, dt, dm0, period, 0.03, 40.0, 21)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    d = cs.Dedisp(nch, lo, cw, dt, nspec, nsub, lodm, dmstep, nd, spectra_per_subint=P,
+                  downsamp=ds)
+    info = d.info
+
+    class Resident:
+        ptr = raw.data_ptr()
+        uint8 = True
+
+    d.run(Resident())
+    walls, dev = [], []
+    for _ in range(steps):
+        w = time.perf_counter()
+        d.run(Resident())
+        walls.append(time.perf_counter() - w)
+        dev.append(d.last_ms())
+    dev = np.array(dev)
+    written, tsub = info["written"], info["written"] + info["max_offset"]
+    raw_bytes = (tsub * ds + P) * nch                      # read once (upper bound of rows)
+    sub_bytes = nsub * tsub * 4                            # written, then read once
+    out_bytes = nd * written * 4
+    sub_b, dm_b = raw_bytes + sub_bytes, sub_bytes + out_bytes
+    ms_total, ms_sub, ms_dm = (float(np.mean(dev[:, k])) for k in range(3))
+    res = {"workload": "%d channels x %d spectra (uint8, %.0f s at %g us), nsub %d, %d DMs "
+                       "(%g-%g), downsamp %d; spectra resident in HBM"
+                       % (nch, nspec, nspec * dt, dt * 1e6, nsub, nd, lodm,
+                          lodm + (nd - 1) * dmstep, ds),
+           "generate_s": gen_s, "written": written, "numout": info["numout"],
+           "device_ms": ms_total, "subband_ms": ms_sub, "dm_ms": ms_dm,
+           "wall_ms": 1e3 * float(np.mean(walls)),
+           "spectra_per_s": nspec / (ms_total * 1e-3),
+           "subband_gbs": sub_b / (ms_sub * 1e-3) / 1e9, "dm_gbs": dm_b / (ms_dm * 1e-3) / 1e9,
+           "roofline": {"bound": "hbm", "achieved": (sub_b + dm_b) / (ms_total * 1e-3) / 1e9,
+                        "peak": 8000.0, "unit": "GB/s",
+                        "frac": (sub_b + dm_b) / (ms_total * 1e-3) / 8e12,
+                        "bytes": sub_b + dm_b}}
+    log("dedisp: %s" % res["workload"])
+    log("  device %.3f ms (subband %.3f ms, %.0f GB/s; DM %.3f ms, %.0f GB/s), wall %.2f ms"
+        % (ms_total, ms_sub, res["subband_gbs"], ms_dm, res["dm_gbs"], res["wall_ms"]))
+    # from pinned host memory (the reference reads blocks on the host)
+    host = raw.cpu().pin_memory()
+    w = time.perf_counter()
+    d.run(host.numpy())
+    res["host_input_wall_ms"] = 1e3 * (time.perf_counter() - w)
+    log("  from pinned host memory: %.1f ms" % res["host_input_wall_ms"])
+    del host
+    # the search of every DM series on the device, then the reference's flow
+    p = cs.make_params(nharms=32)
+    with Engine(device=0, nslots=4) as eng:
+        w = time.perf_counter()
+        d.run(Resident())
+        finals = cs.prep_search(d, eng, p, ncands=100, threshold=6.0)
+        res["prep_search_resident_s"] = time.perf_counter() - w
+    dms, _, _ = d.tables()
+    best = [max([m for fr, m in zip(f["freq"], f["metric"])
+                 if abs(fr * period - round(fr * period)) < 2e-3 and round(fr * period) >= 1]
+                or [0.0]) for f in finals]
+    res["pulsar_dm_found"] = float(dms[int(np.argmax(best))])
+    res["pulsar_dm_injected"] = dm0
+    log("  de-dispersion + search of %d DMs, resident: %.3f s; pulsar at DM %g (injected %g)"
+        % (nd, res["prep_search_resident_s"], res["pulsar_dm_found"], dm0))
+    if ref_flow:
+        w = time.perf_counter()
+        series = d.copy_out()
+        jobs = [{"samples": series[i], "dt": d.dt_out, "params": p, "threshold": 6.0,
+                 "blocksize": 65536} for i in range(nd)]
+        _, wall = run_plain(jobs)
+        res["prep_search_reference_flow_s"] = time.perf_counter() - w
+        res["ratio_vs_reference_flow"] = (res["prep_search_reference_flow_s"]
+                                          / res["prep_search_resident_s"])
+        log("  reference flow (series to the host, host staging + search_file per DM): "
+            "%.3f s, %.1fx" % (res["prep_search_reference_flow_s"],
+                              res["ratio_vs_reference_flow"]))
+    d.close()
+    del raw
+    torch.cuda.empty_cache()
+    return res
+
+
 def stage_only(sizes, reps=10):
     """Device staging alone (realfft + rednoise of a resident series): each
     size staged `reps` times after two warm-up passes; prints one JSON line
@@ -327,6 +453,10 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--no-dedisp", action="store_true",
+                    help="skip the filterbank de-dispersion and its DM search")
+    ap.add_argument("--dedisp-only", action="store_true",
+                    help="only time the de-dispersion (prints one JSON line)")
     ap.add_argument("--stage-only", type=int, nargs="*", default=None, metavar="SAMPLES",
                     help="only time device staging (realfft + rednoise) of series of these "
                          "lengths (default 2^24 and 3*5*7*2^17)")
@@ -336,6 +466,10 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.dedisp_only:
+        print(json.dumps({"dedisp": dedisp_timing(steps=max(1, args.steps),
+                                                  ref_flow=False)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards by request or by "
@@ -432,6 +566,8 @@ def main():
                            "classes": tune}
     out["metric_stats"] = metric_stats_timing(spec, params)
     out["wide_harmonics"] = wide_harmonics(spec)
+    if not args.no_dedisp:
+        out["dedisp"] = dedisp_timing()
     if not args.no_sweep:
         out["dm_sweep"] = dm_sweep(args.sweep_files, args.slots)
         out["dm_sweep_one_observation"] = dm_sweep_resident(args.sweep_files, args.slots)

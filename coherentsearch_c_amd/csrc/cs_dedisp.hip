@@ -55,8 +55,9 @@
 namespace {
 
 constexpr int kWaves = 4;              /* waves per workgroup: one subband or DM each */
-constexpr long long kChunkTarget = 1 << 18;
-constexpr long long kRingBytes = 1ll << 30;
+constexpr long long kChunkTarget = 1 << 20;   /* raw spectra per pass */
+constexpr long long kRingBytes = 1ll << 30;    /* raw ring (host input) */
+constexpr long long kSubBytes = 1ll << 29;     /* subband ring */
 
 /* PRESTO's NEAREST_LONG: halves away from zero (src/prep_dedisp.c:37) */
 long nearest_long(double x)
@@ -239,6 +240,171 @@ dm_kernel(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, i
     out[(long long)i * numout + u0 + u] = acc;
 }
 
+/* ---- LDS-tiled kernels -------------------------------------------------- */
+
+constexpr int kRows = 256;            /* raw rows per subband tile: one per lane */
+constexpr int kTileBytes = 32 * 1024; /* LDS budget of a subband tile */
+constexpr int kTU = 512;              /* DM samples per DM tile: two per lane, 256 apart */
+constexpr int kDmThreads = 256;
+constexpr int kGMax = 32;             /* DMs per DM tile (register accumulators) */
+constexpr int kSpanFloats = 8192;     /* LDS floats of one subband batch of a DM tile */
+constexpr int kDmWorkgroups = 2048;   /* DM tiles per pass worth keeping the GPU busy with */
+
+template <int VEC>
+struct Vec;
+template <>
+struct Vec<16> {
+    using type = uint4;
+};
+template <>
+struct Vec<4> {
+    using type = uint32_t;
+};
+template <>
+struct Vec<1> {
+    using type = uint8_t;
+};
+
+template <int VEC>
+__device__ __forceinline__ void tile_store(unsigned char *dst, typename Vec<VEC>::type w)
+{
+    if constexpr (VEC == 16) {       /* rows are 4-byte aligned only: four dword stores */
+        uint32_t *d = (uint32_t *)dst;
+        d[0] = w.x;
+        d[1] = w.y;
+        d[2] = w.z;
+        d[3] = w.w;
+    } else {
+        *(typename Vec<VEC>::type *)dst = w;
+    }
+}
+
+/* Subbands from an LDS tile: the workgroup stages raw rows [t0, t0 + 256 +
+ * maxd) of one group of gs subbands (row pitch an odd number of dwords, so
+ * the lanes' rows fall in distinct banks), then lane r sums row t0 + r of
+ * each subband over its channels in ascending order, and the ds lanes of one
+ * downsampled sample add their sums in order (lane k = 0 writes).  rows:
+ * raw rows the pass reads (from the chunk start; at most its samples' rows
+ * plus the deepest delay, so at most two chunks).  VEC: bytes per
+ * global load (16 when every row segment is 16-byte aligned). */
+template <typename T, int VEC>
+__global__ void __launch_bounds__(kRows)
+subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chunk, int nchan,
+              int cps, int gs, int nsub, int ds, const int *__restrict__ idisp, int maxd,
+              int pitch, long long rows, int nu, float *__restrict__ out, int dsc)
+{
+    extern __shared__ __align__(16) unsigned char tile[];
+    using W = typename Vec<VEC>::type;
+    /* x: subband group (fast, so one row tile's groups run together and
+     * read whole rows), y: row tile */
+    const int s0 = (int)blockIdx.x * gs, ns = min(gs, nsub - s0);
+    const int c0 = s0 * cps;
+    const long long t0 = (long long)blockIdx.y * kRows;
+    const int nr = kRows + maxd;
+    const int nv = ns * cps * (int)sizeof(T) / VEC;       /* loads per row */
+    for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
+        const int r = idx / nv, v = idx - r * nv;
+        const long long t = t0 + r;
+        W w{};
+        if (t < rows) {
+            const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
+            w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+        }
+        tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
+    }
+    __syncthreads();
+    const int r = (int)threadIdx.x;
+    const int k = r % ds, base = (int)(threadIdx.x & 63) - k;
+    const long long u = (t0 + r) / ds;
+    for (int s = 0; s < ns; s++) {
+        float v = 0.0f;
+        for (int j = 0; j < cps; j++) {
+            const int ch = s * cps + j;
+            const int d = idisp[c0 + ch];
+            v += (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
+        }
+        float acc = v;
+        if (ds > 1) {
+            acc = 0.0f;
+            for (int q = 0; q < ds; q++) acc += __shfl(v, base + q, 64);
+            acc /= (float)ds;
+        }
+        if (k == 0 && u < nu) out[(long long)(s0 + s) * dsc + u] = acc;
+    }
+}
+
+/* DM samples from LDS spans: the workgroup takes 512 consecutive samples of
+ * a group of G consecutive DMs (lane l: samples l and l + 256, 2G
+ * accumulators; the two reads of one offset share an address and issue as
+ * one ds_read2st64).  Subbands go through LDS in batches of sb: subband s's span covers
+ * the samples every DM of the group reads, [u0 + omin_s, u0 + 256 + omax_s);
+ * each accumulator adds its subbands in ascending order across the batches.
+ * gmin[g][s]: omin_s; gbase[g][s]: the span's LDS offset within its batch;
+ * sofs[g][s][i] = gbase - omin + off[g * G + i][s] (the group's G offsets
+ * of one subband contiguous: one vector scalar load).  tlim: subband samples of the pass
+ * that exist (beyond: never read for a written sample). */
+template <int G>
+__global__ void __launch_bounds__(kDmThreads)
+dm_tiled(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, int nsub,
+         int sbatch, const int *__restrict__ gmin, const int *__restrict__ glen,
+         const int *__restrict__ gbase, const int *__restrict__ sofs, int numdms, int nu,
+         int tlim, float *__restrict__ out, long long numout, long long v0)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    float *span = (float *)smem;
+    /* x: DM group (fast: the groups of one sample tile read the same
+     * subband samples), y: sample tile */
+    const int g = (int)blockIdx.x, i0 = g * G, ng = min(G, numdms - i0);
+    const int u0 = (int)blockIdx.y * kTU, tid = (int)threadIdx.x;
+    const int *mn = gmin + (size_t)g * nsub, *ln = glen + (size_t)g * nsub,
+              *bs = gbase + (size_t)g * nsub;
+    float acc[G], acc2[G];
+#pragma unroll
+    for (int i = 0; i < G; i++) acc[i] = acc2[i] = -0.0f;
+    for (int sA = 0; sA < nsub; sA += sbatch) {
+        const int sZ = min(nsub, sA + sbatch);
+        if (sA) __syncthreads();
+        for (int s = sA; s < sZ; s++) {
+            const int len = ln[s], b = bs[s], tb = u0 + mn[s];
+            for (int q = tid; q < len; q += kDmThreads) {
+                const int t = tb + q;
+                float v = 0.0f;
+                if (t < tlim) v = t < dsc ? sa[(size_t)s * dsc + t] : sb[(size_t)s * dsc + (t - dsc)];
+                span[b + q] = v;
+            }
+        }
+        __syncthreads();
+        /* every accumulator, unguarded: a group's missing DMs have offset 0
+         * (a valid span index) and are never stored, so the G offsets of a
+         * subband come in one vector scalar load and the G reads issue
+         * back to back */
+        for (int s = sA; s < sZ; s++) {
+            const int *so = sofs + ((size_t)g * nsub + s) * G;
+            /* all 2G reads first (in flight together), then the adds */
+            float x[G], y[G];
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                const float *p = span + so[i] + tid;
+                x[i] = p[0];
+                y[i] = p[kDmThreads];
+            }
+#pragma unroll
+            for (int i = 0; i < G; i++) {
+                acc[i] += x[i];
+                acc2[i] += y[i];
+            }
+        }
+    }
+    const int u = u0 + tid;
+#pragma unroll
+    for (int i = 0; i < G; i++)
+        if (i < ng) {
+            float *o = out + (size_t)(i0 + i) * numout + v0;
+            if (u < nu) o[u] = acc[i];
+            if (u + kDmThreads < nu) o[u + kDmThreads] = acc2[i];
+        }
+}
+
 __global__ void pad_kernel(float *__restrict__ out, long long numout, long long written,
                            int numdms, float v)
 {
@@ -259,12 +425,18 @@ struct CsDedisp {
     long long nspec = 0, chunk = 0;
     int dsc = 0;                   /* downsampled samples per chunk */
     hipStream_t st = nullptr, cst = nullptr;
+    hipStream_t ost = nullptr;     /* DM 0's samples back to the host */
     int *d_idisp = nullptr, *d_offs = nullptr;
     float *d_sub = nullptr;        /* 2 x nsub x dsc */
     float *d_out = nullptr;        /* numdms x numout */
     void *d_ring = nullptr;        /* 3 chunks of raw spectra (host input) */
     size_t ring_bytes = 0;
     float *h_dm0 = nullptr;        /* pinned: DM 0's written samples */
+    bool simple = false;           /* the untiled kernels only */
+    /* DM tiles: G DMs per group, subband batches of sbatch, tables
+     * [ngroups][nsub] (gmin, glen, gbase) and [numdms][nsub] (sofs) */
+    int dm_g = 0, dm_ngroups = 0, dm_sbatch = 0, dm_lds = 0;
+    int *d_gtab = nullptr;
     float pad = 0.0f;
     double last_ms = 0.0, last_sub_ms = 0.0, last_dm_ms = 0.0;
 };
@@ -307,14 +479,17 @@ extern "C" CS_API void cs_gpu_dedisp_free(CsDedisp *d)
     hipSetDevice(d->device);
     if (d->st) hipStreamSynchronize(d->st);
     if (d->cst) hipStreamSynchronize(d->cst);
+    if (d->ost) hipStreamSynchronize(d->ost);
     hipFree(d->d_idisp);
     hipFree(d->d_offs);
     hipFree(d->d_sub);
     hipFree(d->d_out);
     hipFree(d->d_ring);
+    hipFree(d->d_gtab);
     if (d->h_dm0) hipHostFree(d->h_dm0);
     if (d->st) hipStreamDestroy(d->st);
     if (d->cst) hipStreamDestroy(d->cst);
+    if (d->ost) hipStreamDestroy(d->ost);
     delete d;
 }
 
@@ -333,17 +508,24 @@ extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long l
     {
         const Tables &t = d->t;
         const int P = c->spectra_per_subint, nsub = c->nsub;
-        /* a pass: whole raw blocks, at least one work block, about 2^18
-         * spectra, and a raw ring within 1 GiB at 4 bytes a sample */
-        long long want = std::max<long long>(t.worklen, kChunkTarget);
-        const long long cap = kRingBytes / (3ll * c->numchan * 4);
+        /* a pass over spectra in HBM: whole raw blocks, at least one work
+         * block, about 2^20 spectra, the subband ring within 512 MiB (passes
+         * over host spectra may be shorter: run_passes) */
+        /* CS_DEDISP_CHUNK: spectra per pass (tests of several passes) */
+        const char *ec = getenv("CS_DEDISP_CHUNK");
+        const long long target = ec && atoll(ec) > 0 ? atoll(ec) : kChunkTarget;
+        long long want = std::max<long long>(t.worklen, target);
+        const long long cap = kSubBytes * c->downsamp / (8ll * nsub);
         if (want > cap) want = std::max<long long>(t.worklen, cap);
         d->chunk = (want + P - 1) / P * P;
         d->dsc = (int)(d->chunk / c->downsamp);
         if (device < 0) return d;     /* tables only */
+        const char *env = getenv("CS_DEDISP_SIMPLE");
+        d->simple = env && *env && *env != '0';
         DCHK(hipSetDevice(device));
         DCHK(hipStreamCreateWithFlags(&d->st, hipStreamNonBlocking));
         DCHK(hipStreamCreateWithFlags(&d->cst, hipStreamNonBlocking));
+        DCHK(hipStreamCreateWithFlags(&d->ost, hipStreamNonBlocking));
         DCHK(hipMalloc(&d->d_idisp, sizeof(int) * t.idispdt.size()));
         DCHK(hipMalloc(&d->d_offs, sizeof(int) * t.offsets.size()));
         DCHK(hipMemcpy(d->d_idisp, t.idispdt.data(), sizeof(int) * t.idispdt.size(),
@@ -355,6 +537,68 @@ extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long l
         if (t.written > 0)
             DCHK(hipHostMalloc((void **)&d->h_dm0, sizeof(float) * (size_t)t.written,
                                hipHostMallocDefault));
+        /* DM tiles: the largest group (up to 32 DMs, a power of two) that
+         * still gives a pass about kDmWorkgroups tiles and whose widest span
+         * fits one batch */
+        const int nd = c->numdms;
+        const long long tiles = (d->dsc + kTU - 1) / kTU;
+        int G = 1;
+        while (G < kGMax && G < nd) G *= 2;
+        while (G > 4 && tiles * ((nd + G - 1) / G) < kDmWorkgroups) G /= 2;
+        std::vector<int> gmin, glen, gbase, sofs;
+        for (;; G /= 2) {
+            const int ng = (nd + G - 1) / G;
+            gmin.assign((size_t)ng * nsub, 0);
+            glen.assign((size_t)ng * nsub, 0);
+            int maxlen = 0;
+            for (int g = 0; g < ng; g++)
+                for (int s = 0; s < nsub; s++) {
+                    int lo = 1 << 30, hi = -1;
+                    for (int i = g * G; i < std::min(nd, (g + 1) * G); i++) {
+                        lo = std::min(lo, t.offsets[(size_t)i * nsub + s]);
+                        hi = std::max(hi, t.offsets[(size_t)i * nsub + s]);
+                    }
+                    gmin[(size_t)g * nsub + s] = lo;
+                    glen[(size_t)g * nsub + s] = kTU + hi - lo;
+                    maxlen = std::max(maxlen, kTU + hi - lo);
+                }
+            if (maxlen <= kSpanFloats || G == 1) {
+                d->dm_g = G;
+                d->dm_ngroups = ng;
+                d->dm_sbatch = std::max(1, std::min(nsub, kSpanFloats / maxlen));
+                d->dm_lds = d->dm_sbatch * maxlen * (int)sizeof(float);
+                break;
+            }
+        }
+        if (d->dm_lds > kSpanFloats * (int)sizeof(float)) d->dm_g = 0;   /* untiled */
+        if (d->dm_g) {
+            const int ng = d->dm_ngroups;
+            gbase.assign((size_t)ng * nsub, 0);
+            for (int g = 0; g < ng; g++)
+                for (int sA = 0; sA < nsub; sA += d->dm_sbatch) {
+                    int b = 0;
+                    for (int s = sA; s < std::min(nsub, sA + d->dm_sbatch); s++) {
+                        gbase[(size_t)g * nsub + s] = b;
+                        b += glen[(size_t)g * nsub + s];
+                    }
+                }
+            sofs.assign((size_t)ng * nsub * d->dm_g, 0);
+            for (int i = 0; i < nd; i++) {
+                const int g = i / d->dm_g, k = i % d->dm_g;
+                for (int s = 0; s < nsub; s++)
+                    sofs[((size_t)g * nsub + s) * d->dm_g + k] =
+                        gbase[(size_t)g * nsub + s] - gmin[(size_t)g * nsub + s] +
+                        t.offsets[(size_t)i * nsub + s];
+            }
+            std::vector<int> all;
+            all.insert(all.end(), gmin.begin(), gmin.end());
+            all.insert(all.end(), glen.begin(), glen.end());
+            all.insert(all.end(), gbase.begin(), gbase.end());
+            all.insert(all.end(), sofs.begin(), sofs.end());
+            DCHK(hipMalloc(&d->d_gtab, sizeof(int) * all.size()));
+            DCHK(hipMemcpy(d->d_gtab, all.data(), sizeof(int) * all.size(),
+                           hipMemcpyHostToDevice));
+        }
     }
     return d;
 fail:
@@ -420,7 +664,15 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     const Tables &t = d->t;
     const CsDedispConfig &c = d->c;
     const int nch = c.numchan, nsub = c.nsub, cps = nch / nsub, ds = c.downsamp;
-    const long long C = d->chunk, dsc = d->dsc;
+    /* passes over host spectra: a raw ring of three chunks within 1 GiB */
+    long long C = d->chunk;
+    if (!on_device) {
+        const int P = c.spectra_per_subint;
+        long long want = std::max<long long>(t.worklen, kRingBytes / (3ll * nch * (long long)sizeof(T)));
+        want = (want + P - 1) / P * P;
+        C = std::min(C, want);
+    }
+    const long long dsc = C / ds;
     const long long tsub = t.written > 0 ? t.written + t.maxoff : 0;
     const long long rows = tsub * ds + t.maxd;        /* raw spectra the passes read */
     const long long nk1 = (tsub + dsc - 1) / dsc, nk2 = (t.written + dsc - 1) / dsc;
@@ -433,7 +685,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         ev.push_back(*e);
         return true;
     };
-    std::vector<hipEvent_t> k1s(nk1), k1e(nk1), k2s(nk2), k2e(nk2);
+    std::vector<hipEvent_t> k1s(nk1), k1e(nk1), k2s(nk2), k2e(nk2), cp(nk2);
     const T *ring = nullptr;
     auto chunk_ptr = [&](long long j) -> const T * {
         return on_device ? spectra + j * C * nch : ring + (j % 3) * C * nch;
@@ -449,6 +701,73 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             return false;
         return hipEventRecord(raw_ready[j % 3], d->cst) == hipSuccess;
     };
+    int sb_gs = 0, sb_pitch = 0, sb_vec = 0;
+    /* the subband pass of chunk j: nu downsampled samples into `sub` */
+    auto launch_sub = [&](long long j, long long nu, float *sub) -> hipError_t {
+        const T *ra = chunk_ptr(j), *rb = chunk_ptr(j + 1);
+        /* grid y holds the row tiles: at most 65535 */
+        if (sb_gs && (nu * ds + kRows - 1) / kRows <= 65535) {
+            /* rows the tiles may load: those the pass reads (its samples'
+             * rows plus the deepest delay), never past the data */
+            const long long nrow = nu * ds, here = std::min(rows - j * C, nrow + t.maxd);
+            const dim3 grid((unsigned)((nsub + sb_gs - 1) / sb_gs),
+                            (unsigned)((nrow + kRows - 1) / kRows));
+            const size_t lds = (size_t)(kRows + t.maxd) * sb_pitch;
+#define CS_SUB(V)                                                                          \
+    subband_tiled<T, V><<<grid, kRows, lds, d->st>>>(ra, rb, C, nch, cps, sb_gs, nsub, ds,   \
+                                                     d->d_idisp, t.maxd, sb_pitch, here,    \
+                                                     (int)nu, sub, (int)dsc)
+            if (sb_vec == 16) CS_SUB(16);
+            else if (sb_vec == 4 || sizeof(T) != 1) CS_SUB(4);
+            else if constexpr (sizeof(T) == 1) CS_SUB(1);
+#undef CS_SUB
+        } else {
+            const dim3 grid((unsigned)((nu + 63) / 64), (unsigned)((nsub + kWaves - 1) / kWaves));
+            subband_kernel<T><<<grid, 64 * kWaves, 0, d->st>>>(ra, rb, C, nch, cps, nsub, ds,
+                                                               d->d_idisp, (int)nu, sub, (int)dsc);
+        }
+        return hipGetLastError();
+    };
+    /* DM 0's samples of DM pass m back to the pinned host buffer on their
+     * own stream, so the host's running mean follows the passes */
+    auto copy_dm0 = [&](long long m, long long nv, long long v0) -> hipError_t {
+        hipError_t e = hipStreamWaitEvent(d->ost, k2e[m], 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d->h_dm0 + v0, d->d_out + v0, sizeof(float) * (size_t)nv,
+                               hipMemcpyDeviceToHost, d->ost);
+        if (e == hipSuccess) e = hipEventRecord(cp[m], d->ost);
+        return e;
+    };
+    /* the DM pass of subband chunk m: nv samples from v0 on */
+    auto launch_dm = [&](long long m, long long nv, long long v0) -> hipError_t {
+        const float *sa = d->d_sub + (m & 1) * (size_t)nsub * dsc;
+        const float *sb = d->d_sub + ((m + 1) & 1) * (size_t)nsub * dsc;
+        if (d->dm_g && (nv + kTU - 1) / kTU <= 65535) {
+            const int ng = d->dm_ngroups;
+            const int *gmin = d->d_gtab, *glen = gmin + (size_t)ng * nsub,
+                      *gbase = glen + (size_t)ng * nsub, *sofs = gbase + (size_t)ng * nsub;
+            const int tlim = (int)(nv + t.maxoff);
+            const dim3 grid((unsigned)ng, (unsigned)((nv + kTU - 1) / kTU));
+#define CS_DM(G)                                                                           \
+    dm_tiled<G><<<grid, kDmThreads, (size_t)d->dm_lds, d->st>>>(sa, sb, (int)dsc, nsub, d->dm_sbatch, \
+                                                         gmin, glen, gbase, sofs, c.numdms,  \
+                                                         (int)nv, tlim, d->d_out, t.numout, v0)
+            switch (d->dm_g) {
+            case 32: CS_DM(32); break;
+            case 16: CS_DM(16); break;
+            case 8: CS_DM(8); break;
+            case 4: CS_DM(4); break;
+            case 2: CS_DM(2); break;
+            default: CS_DM(1); break;
+            }
+#undef CS_DM
+        } else {
+            const dim3 grid((unsigned)((nv + 63) / 64), (unsigned)((c.numdms + kWaves - 1) / kWaves));
+            dm_kernel<<<grid, 64 * kWaves, 0, d->st>>>(sa, sb, (int)dsc, nsub, d->d_offs, c.numdms,
+                                                       (int)nv, d->d_out, t.numout, v0);
+        }
+        return hipGetLastError();
+    };
     DCHK(hipSetDevice(d->device));
     for (int i = 0; i < 3; i++) {
         if (!new_event(&raw_ready[i], hipEventDisableTiming) ||
@@ -458,6 +777,8 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     for (auto *v : {&k1s, &k1e, &k2s, &k2e})
         for (auto &e : *v)
             if (!new_event(&e, hipEventDefault)) goto fail;
+    for (auto &e : cp)
+        if (!new_event(&e, hipEventDisableTiming)) goto fail;
     if (!on_device && nk1 > 0) {
         const size_t need = 3 * (size_t)C * row_bytes;
         if (d->ring_bytes < need) {
@@ -469,6 +790,26 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         }
         ring = (const T *)d->d_ring;
     }
+    /* subband tiles for this sample type: the widest group of subbands
+     * whose tile fits the LDS budget, loads as wide as the rows' alignment
+     * allows (64 % ds == 0: a downsampled sample's lanes share a wave) */
+    if (!d->simple && 64 % ds == 0) {
+        const int es = (int)sizeof(T);
+        const bool al16 = !on_device || (uintptr_t)spectra % 16 == 0;
+        sb_vec = (cps * es) % 16 == 0 && (nch * es) % 16 == 0 && al16 ? 16
+                 : (cps * es) % 4 == 0 && (nch * es) % 4 == 0 &&
+                           (!on_device || (uintptr_t)spectra % 4 == 0)
+                     ? 4
+                     : 1;
+        for (int gs = nsub; gs >= 1; gs--) {
+            const int pitch = (gs * cps * es + 15) / 16 * 16 + 4;    /* odd dwords */
+            if ((long long)(kRows + t.maxd) * pitch <= kTileBytes) {
+                sb_gs = gs;
+                sb_pitch = pitch;
+                break;
+            }
+        }
+    }
     if (!copy_chunk(0) || !copy_chunk(1)) goto fail;
     for (long long j = 0; j < nk1; j++) {
         const long long u0 = j * dsc, nu = std::min(dsc, tsub - u0);
@@ -478,24 +819,15 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             if ((j + 1) * C < rows) DCHK(hipStreamWaitEvent(d->st, raw_ready[(j + 1) % 3], 0));
         }
         DCHK(hipEventRecord(k1s[j], d->st));
-        {
-            const dim3 grid((unsigned)((nu + 63) / 64), (unsigned)((nsub + kWaves - 1) / kWaves));
-            subband_kernel<T><<<grid, 64 * kWaves, 0, d->st>>>(
-                chunk_ptr(j), chunk_ptr(j + 1), C, nch, cps, nsub, ds, d->d_idisp, (int)nu,
-                sub, (int)dsc);
-            DCHK(hipGetLastError());
-        }
+        DCHK(launch_sub(j, nu, sub));
         DCHK(hipEventRecord(k1e[j], d->st));
         DCHK(hipEventRecord(k1_done[j % 3], d->st));
         if (j >= 1 && j - 1 < nk2) {
             const long long m = j - 1, v0 = m * dsc, nv = std::min(dsc, t.written - v0);
             DCHK(hipEventRecord(k2s[m], d->st));
-            const dim3 grid((unsigned)((nv + 63) / 64), (unsigned)((c.numdms + kWaves - 1) / kWaves));
-            dm_kernel<<<grid, 64 * kWaves, 0, d->st>>>(
-                d->d_sub + (m & 1) * (size_t)nsub * dsc, d->d_sub + ((m + 1) & 1) * (size_t)nsub * dsc,
-                (int)dsc, nsub, d->d_offs, c.numdms, (int)nv, d->d_out, t.numout, v0);
-            DCHK(hipGetLastError());
+            DCHK(launch_dm(m, nv, v0));
             DCHK(hipEventRecord(k2e[m], d->st));
+            DCHK(copy_dm0(m, nv, v0));
         }
         if (!copy_chunk(j + 2)) goto fail;
     }
@@ -503,22 +835,18 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     for (long long m = std::max<long long>(nk1 - 1, 0); m < nk2; m++) {
         const long long v0 = m * dsc, nv = std::min(dsc, t.written - v0);
         DCHK(hipEventRecord(k2s[m], d->st));
-        const dim3 grid((unsigned)((nv + 63) / 64), (unsigned)((c.numdms + kWaves - 1) / kWaves));
-        dm_kernel<<<grid, 64 * kWaves, 0, d->st>>>(
-            d->d_sub + (m & 1) * (size_t)nsub * dsc, d->d_sub + ((m + 1) & 1) * (size_t)nsub * dsc,
-            (int)dsc, nsub, d->d_offs, c.numdms, (int)nv, d->d_out, t.numout, v0);
-        DCHK(hipGetLastError());
+        DCHK(launch_dm(m, nv, v0));
         DCHK(hipEventRecord(k2e[m], d->st));
+        DCHK(copy_dm0(m, nv, v0));
     }
     /* the padding value: the running mean of DM 0's written samples, in
      * order, as update_stats keeps it (misc_utils.h:114-127) */
     {
         double avg = 0.0;
-        if (t.written > 0) {
-            DCHK(hipMemcpyAsync(d->h_dm0, d->d_out, sizeof(float) * (size_t)t.written,
-                                hipMemcpyDeviceToHost, d->st));
-            DCHK(hipStreamSynchronize(d->st));
-            for (long long u = 0; u < t.written; u++) {
+        for (long long m = 0; m < nk2; m++) {        /* while later passes run */
+            DCHK(hipEventSynchronize(cp[m]));
+            const long long v1 = std::min(t.written, (m + 1) * dsc);
+            for (long long u = m * dsc; u < v1; u++) {
                 const double dev = (double)d->h_dm0[u] - avg;
                 avg += dev / ((double)u + 1.0);
             }
@@ -530,6 +858,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         }
         DCHK(hipStreamSynchronize(d->st));
         DCHK(hipStreamSynchronize(d->cst));
+        DCHK(hipStreamSynchronize(d->ost));
     }
     {
         double s1 = 0.0, s2 = 0.0;
@@ -550,6 +879,7 @@ fail:
     if (rc != 0) {
         hipStreamSynchronize(d->st);
         hipStreamSynchronize(d->cst);
+        hipStreamSynchronize(d->ost);
     }
     for (hipEvent_t e : ev) hipEventDestroy(e);
     return rc;

@@ -25,8 +25,12 @@ CASES = [
     (64, 1200.0, 2.0, 64e-6, 256, 16, 40.0, 2.0, 21, 2, 48, False),
     (48, 700.0, 1.0, 5e-5, 192, 4, 10.0, 3.0, 12, 4, 40, True),
     (16, 1400.0, 8.0, 1e-4, 256, 1, 0.0, 10.0, 5, 8, 30, False),
-    # several device passes (2^20 spectra, passes of 2^18)
+    # 2^20 spectra: one device pass, and 16 with passes of 2^16 (CHUNKED)
     (64, 1300.0, 3.0, 64e-6, 1024, 16, 0.0, 4.0, 32, 1, 1024, True),
+    # bytes loaded one by one (3-channel subbands), three DM tile groups
+    (30, 1100.0, 2.0, 64e-6, 256, 10, 5.0, 1.5, 70, 2, 64, True),
+    # a downsampling factor that does not divide a wave (untiled subbands)
+    (32, 900.0, 1.0, 64e-6, 192, 8, 0.0, 2.0, 9, 3, 48, False),
 ]
 
 
@@ -40,14 +44,24 @@ def _plan(c, nspec, numout=0):
                      spectra_per_subint=c[4], downsamp=c[9], numout=numout)
 
 
+CHUNKED = [(CASES[3], "tiled"), (CASES[3], "untiled"), (CASES[4], "tiled")]
+
+
+@pytest.mark.parametrize("chunk", [0, 1 << 16])
+@pytest.mark.parametrize("kernels", ["tiled", "untiled"])
 @pytest.mark.parametrize("c", CASES)
-def test_series_match_oracle(oracle, c):
+def test_series_match_oracle(oracle, c, kernels, chunk, monkeypatch):
+    if chunk and (c, kernels) not in CHUNKED:
+        pytest.skip("one pass at this size")
+    # the untiled kernels serve shapes the LDS tiles do not fit
+    monkeypatch.setenv("CS_DEDISP_SIMPLE", "1" if kernels == "untiled" else "0")
+    monkeypatch.setenv("CS_DEDISP_CHUNK", str(chunk))
     nspec = c[4] * c[10]
     u8 = c[11]
     raw = synth.filterbank_with_pulsar(nspec, c[0], c[1], c[2], c[3], 37.0, 0.0213, 0.04, 25.0,
                                        7, dtype=np.uint8 if u8 else np.float32)
-    say("dedisp %d chans x %d spectra (%s), nsub %d, %d DMs, downsamp %d"
-        % (c[0], nspec, raw.dtype, c[5], c[8], c[9]))
+    say("dedisp %s: %d chans x %d spectra (%s), nsub %d, %d DMs, downsamp %d"
+        % (kernels, c[0], nspec, raw.dtype, c[5], c[8], c[9]))
     want, n, pad = oracle.dedisp(oracle.dedisp_config(*c[:10]), raw.astype(np.float32))
     d = _plan(c, nspec)
     info = d.info
