@@ -163,7 +163,7 @@ class CsDedispConfig(C.Structure):
     _fields_ = [("numchan", C.c_int), ("lofreq", C.c_double), ("chan_wid", C.c_double),
                 ("dt", C.c_double), ("spectra_per_subint", C.c_int), ("nsub", C.c_int),
                 ("lodm", C.c_double), ("dmstep", C.c_double), ("numdms", C.c_int),
-                ("downsamp", C.c_int), ("numout", C.c_longlong)]
+                ("downsamp", C.c_int), ("numout", C.c_longlong), ("clip_sigma", C.c_float)]
 
 
 class CsDedispInfo(C.Structure):

@@ -38,6 +38,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "cs_dedisp.h"
@@ -193,6 +194,25 @@ __device__ __forceinline__ float sample(const T *p)
     return (float)*p;
 }
 
+/* Time-domain clipping as the subband kernels see it (clip_times replaces a
+ * raw row whose zero-DM sum is more than trig from the running mean by the
+ * block's channel levels): zdm == nullptr means no clipping. */
+struct ClipView {
+    const float *zdm;      /* zero-DM sum of every spectrum (absolute rows) */
+    const float *ravg;     /* per raw block: running mean of the zero-DM sums */
+    const float *trig;     /* per raw block: clip_sigma x running std */
+    const float *levels;   /* [block][numchan]: the channel levels */
+    long long row0;        /* absolute row of the pass's chunk start */
+    int P;                 /* raw block length */
+};
+
+/* the block of a clipped row, or -1 */
+__device__ __forceinline__ int clip_block(const ClipView &cv, long long row)
+{
+    const int b = (int)(row / cv.P);
+    return fabsf(cv.zdm[row] - cv.ravg[b]) > cv.trig[b] ? b : -1;
+}
+
 /* One downsampled subband sample per lane: wave w of the workgroup takes
  * subband blockIdx.y * kWaves + w, lanes take consecutive samples u.  ra/rb:
  * the pass's raw chunk and the next one (chunk spectra each, [t][numchan]). */
@@ -200,7 +220,7 @@ template <typename T>
 __global__ void __launch_bounds__(64 * kWaves)
 subband_kernel(const T *__restrict__ ra, const T *__restrict__ rb, long long chunk, int nchan,
                int cps, int nsub, int ds, const int *__restrict__ idisp, int nu,
-               float *__restrict__ out, int dsc)
+               float *__restrict__ out, int dsc, ClipView cv)
 {
     const int s = __builtin_amdgcn_readfirstlane((int)(blockIdx.y * kWaves + (threadIdx.x >> 6)));
     const int u = (int)(blockIdx.x * 64 + (threadIdx.x & 63));
@@ -214,7 +234,8 @@ subband_kernel(const T *__restrict__ ra, const T *__restrict__ rb, long long chu
             const int ch = c0 + j;
             const long long t = t0 + idisp[ch];
             const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
-            v += sample(row + ch);
+            const int b = cv.zdm ? clip_block(cv, cv.row0 + t) : -1;
+            v += b >= 0 ? cv.levels[(long long)b * nchan + ch] : sample(row + ch);
         }
         acc = ds > 1 ? acc + v : v;
     }
@@ -287,12 +308,15 @@ __device__ __forceinline__ void tile_store(unsigned char *dst, typename Vec<VEC>
  * raw rows the pass reads (from the chunk start; at most its samples' rows
  * plus the deepest delay, so at most two chunks).  VEC: bytes per
  * global load (16 when every row segment is 16-byte aligned). */
-template <typename T, int VEC>
+template <typename T, int VEC, bool CLIP>
 __global__ void __launch_bounds__(kRows)
 subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chunk, int nchan,
               int cps, int gs, int nsub, int ds, const int *__restrict__ idisp, int maxd,
-              int pitch, long long rows, int nu, float *__restrict__ out, int dsc)
+              int pitch, long long rows, int nu, float *__restrict__ out, int dsc, ClipView cv)
 {
+    /* with clipping the tile holds floats (a clipped row's channel levels
+     * are not samples of type T), loaded one element per lane */
+    using TT = typename std::conditional<CLIP, float, T>::type;
     extern __shared__ __align__(16) unsigned char tile[];
     using W = typename Vec<VEC>::type;
     /* x: subband group (fast, so one row tile's groups run together and
@@ -301,16 +325,31 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
     const int c0 = s0 * cps;
     const long long t0 = (long long)blockIdx.y * kRows;
     const int nr = kRows + maxd;
-    const int nv = ns * cps * (int)sizeof(T) / VEC;       /* loads per row */
-    for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
-        const int r = idx / nv, v = idx - r * nv;
-        const long long t = t0 + r;
-        W w{};
-        if (t < rows) {
-            const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
-            w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+    if constexpr (CLIP) {
+        const int ne = ns * cps;                          /* elements per row */
+        for (int idx = (int)threadIdx.x; idx < nr * ne; idx += kRows) {
+            const int r = idx / ne, e = idx - r * ne;
+            const long long t = t0 + r;
+            float x = 0.0f;
+            if (t < rows) {
+                const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
+                const int b = clip_block(cv, cv.row0 + t);
+                x = b >= 0 ? cv.levels[(long long)b * nchan + c0 + e] : (float)row[c0 + e];
+            }
+            *(float *)(tile + (size_t)r * pitch + (size_t)e * 4) = x;
         }
-        tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
+    } else {
+        const int nv = ns * cps * (int)sizeof(T) / VEC;   /* loads per row */
+        for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
+            const int r = idx / nv, v = idx - r * nv;
+            const long long t = t0 + r;
+            W w{};
+            if (t < rows) {
+                const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
+                w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+            }
+            tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
+        }
     }
     __syncthreads();
     const int r = (int)threadIdx.x;
@@ -321,7 +360,7 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
         for (int j = 0; j < cps; j++) {
             const int ch = s * cps + j;
             const int d = idisp[c0 + ch];
-            v += (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
+            v += (float)*(const TT *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(TT));
         }
         float acc = v;
         if (ds > 1) {
@@ -405,6 +444,202 @@ dm_tiled(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, in
         }
 }
 
+/* ---- time-domain clipping: clip_times of every raw block (clipping.c) ----
+ * A block's statistics depend on its own samples only; the running state
+ * (the 0.9/0.1 updates of the zero-DM mean, std and channel levels) is a
+ * scan over blocks.  So: zero-DM sums and the exact lower median per block
+ * (a workgroup each), the block's AS 52 statistics (one lane each, in
+ * sample order), the channel means of its good rows (one lane per channel,
+ * in row order), then the scan (one workgroup).  The subband kernels then
+ * read a clipped row's channel levels in place of its samples. */
+
+struct BlockStat {
+    double avg, std;       /* of the good rows' zero-DM sums */
+    float lo, hi;          /* the good range, median -/+ 3 std of all rows */
+    int ngood, pad_;
+};
+
+struct ClipState {
+    float running_avg, running_std;
+    int blocksread, pad_;
+};
+
+__device__ __forceinline__ unsigned order_key(float f)
+{
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float key_float(unsigned k)
+{
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+/* zero-DM sums (float, channel order) and their element of rank (P-1)/2 by
+ * an 8-bit radix select; one raw block per workgroup */
+template <typename T>
+__global__ void __launch_bounds__(256)
+clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int vec16,
+                float *__restrict__ zdm, float *__restrict__ med)
+{
+    __shared__ unsigned hist[256];
+    __shared__ unsigned s_prefix, s_k;
+    const int b = (int)blockIdx.x;
+    const T *blk = chunk + (long long)b * P * nch;
+    float *z = zdm + (blk0 + b) * P;
+    for (int i = (int)threadIdx.x; i < P; i += 256) {
+        const T *row = blk + (long long)i * nch;
+        float sum = 0.0f;
+        if (vec16) {
+            constexpr int E = 16 / (int)sizeof(T);
+            for (int c = 0; c < nch; c += E) {
+                const uint4 w = *(const uint4 *)(row + c);
+                const T *e = (const T *)&w;
+#pragma unroll
+                for (int q = 0; q < E; q++) sum += (float)e[q];
+            }
+        } else {
+            for (int c = 0; c < nch; c++) sum += (float)row[c];
+        }
+        z[i] = sum;
+    }
+    __syncthreads();
+    unsigned prefix = 0, mask = 0, k = (unsigned)(P - 1) / 2;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < P; i += 256) {
+            const unsigned key = order_key(z[i]);
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned acc = 0;
+            int d = 0;
+            for (; d < 255; d++) {
+                if (acc + hist[d] > k) break;
+                acc += hist[d];
+            }
+            s_prefix = prefix | ((unsigned)d << shift);
+            s_k = k - acc;
+        }
+        __syncthreads();
+        prefix = s_prefix;
+        k = s_k;
+        mask |= 255u << shift;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) med[blk0 + b] = key_float(prefix);
+}
+
+/* AS 52 mean and variance in double, in sample order (misc_utils.c:641) */
+__device__ __forceinline__ void as52_step(double x, int &n, double &m, double &v, double &an1)
+{
+    if (n == 0) {
+        m = x;
+    } else {
+        const double an = (double)(n + 1);
+        an1 = (double)n;
+        const double dx = (x - m) / an;
+        v += an * an1 * dx * dx;
+        m += dx;
+    }
+    n++;
+}
+
+/* one lane per block: the statistics of all rows, the good range, and the
+ * statistics of the good rows */
+__global__ void clip_seq_kernel(const float *__restrict__ zdm, const float *__restrict__ med,
+                                long long blk0, int nblk, int P, BlockStat *__restrict__ bs)
+{
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= nblk) return;
+    const float *z = zdm + (blk0 + b) * P;
+    int n = 0;
+    double m = 0.0, v = 0.0, an1 = 0.0;
+    for (int i = 0; i < P; i++) as52_step((double)z[i], n, m, v, an1);
+    if (n > 1) v /= an1;
+    const double sd = sqrt(v);
+    const float md = med[blk0 + b];
+    const float lo = md - 3.0 * sd, hi = md + 3.0 * sd;
+    int g = 0;
+    double gm = 0.0, gv = 0.0, gan1 = 0.0;
+    for (int i = 0; i < P; i++) {
+        const float x = z[i];
+        if (x > lo && x < hi) as52_step((double)x, g, gm, gv, gan1);
+    }
+    if (g > 1) gv /= gan1;
+    BlockStat r;
+    r.avg = gm;
+    r.std = sqrt(gv);
+    r.lo = lo;
+    r.hi = hi;
+    r.ngood = g;
+    r.pad_ = 0;
+    bs[blk0 + b] = r;
+}
+
+/* one lane per (channel, block): the channel's mean over the block's good
+ * rows, summed in double in row order */
+template <typename T>
+__global__ void clip_chavg_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch,
+                                  const float *__restrict__ zdm,
+                                  const BlockStat *__restrict__ bs, double *__restrict__ chavg)
+{
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x), b = (int)blockIdx.y;
+    if (c >= nch) return;
+    const BlockStat st = bs[blk0 + b];
+    const T *blk = chunk + (long long)b * P * nch;
+    const float *z = zdm + (blk0 + b) * P;
+    double acc = 0.0;
+    for (int i = 0; i < P; i++) {
+        const float x = z[i];
+        if (x > st.lo && x < st.hi) acc += (double)(float)blk[(long long)i * nch + c];
+    }
+    chavg[(blk0 + b) * nch + c] = st.ngood > 0 ? acc / st.ngood : 0.0;
+}
+
+/* the running state over the blocks [blk0, blk0 + nblk), continuing from
+ * `st`; one workgroup, lanes over channels */
+__global__ void __launch_bounds__(256)
+clip_scan_kernel(long long blk0, int nblk, int nch, float clip_sigma,
+                 const BlockStat *__restrict__ bs, const double *__restrict__ chavg,
+                 ClipState *__restrict__ st, float *__restrict__ crun, float *__restrict__ ravg,
+                 float *__restrict__ trig, float *__restrict__ levels)
+{
+    float ra = st->running_avg, rs = st->running_std;
+    int nread = st->blocksread;
+    for (int b = 0; b < nblk; b++) {
+        const long long B = blk0 + b;
+        const BlockStat s = bs[B];
+        const bool none = s.ngood < 1;
+        const double ca = none ? (double)ra : s.avg, cs = none ? (double)rs : s.std;
+        if (nread) {
+            ra = 0.9 * ra + 0.1 * ca;
+            rs = 0.9 * rs + 0.1 * cs;
+        } else {
+            ra = ca;
+            rs = cs;
+        }
+        for (int c = (int)threadIdx.x; c < nch; c += (int)blockDim.x) {
+            const double cv = none ? (double)crun[c] : chavg[B * nch + c];
+            const float nv = nread ? (float)(0.9 * crun[c] + 0.1 * cv) : (float)cv;
+            crun[c] = nv;
+            levels[B * nch + c] = nv;
+        }
+        if (threadIdx.x == 0) {
+            ravg[B] = ra;
+            trig[B] = clip_sigma * rs;
+        }
+        nread++;
+    }
+    if (threadIdx.x == 0) {
+        st->running_avg = ra;
+        st->running_std = rs;
+        st->blocksread = nread;
+    }
+}
+
 __global__ void pad_kernel(float *__restrict__ out, long long numout, long long written,
                            int numdms, float v)
 {
@@ -437,6 +672,12 @@ struct CsDedisp {
      * [ngroups][nsub] (gmin, glen, gbase) and [numdms][nsub] (sofs) */
     int dm_g = 0, dm_ngroups = 0, dm_sbatch = 0, dm_lds = 0;
     int *d_gtab = nullptr;
+    /* clipping (clip_sigma > 0): per spectrum, per raw block, state */
+    float *d_zdm = nullptr, *d_med = nullptr, *d_ravg = nullptr, *d_trig = nullptr;
+    float *d_levels = nullptr, *d_crun = nullptr;
+    double *d_chavg = nullptr;
+    BlockStat *d_bs = nullptr;
+    ClipState *d_cst = nullptr;
     float pad = 0.0f;
     double last_ms = 0.0, last_sub_ms = 0.0, last_dm_ms = 0.0;
 };
@@ -486,6 +727,10 @@ extern "C" CS_API void cs_gpu_dedisp_free(CsDedisp *d)
     hipFree(d->d_out);
     hipFree(d->d_ring);
     hipFree(d->d_gtab);
+    for (void *q : {(void *)d->d_zdm, (void *)d->d_med, (void *)d->d_ravg, (void *)d->d_trig,
+                    (void *)d->d_levels, (void *)d->d_crun, (void *)d->d_chavg, (void *)d->d_bs,
+                    (void *)d->d_cst})
+        hipFree(q);
     if (d->h_dm0) hipHostFree(d->h_dm0);
     if (d->st) hipStreamDestroy(d->st);
     if (d->cst) hipStreamDestroy(d->cst);
@@ -537,6 +782,18 @@ extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long l
         if (t.written > 0)
             DCHK(hipHostMalloc((void **)&d->h_dm0, sizeof(float) * (size_t)t.written,
                                hipHostMallocDefault));
+        if (c->clip_sigma > 0.0f) {
+            const size_t nb = (size_t)(nspec / P), nc = (size_t)c->numchan;
+            DCHK(hipMalloc(&d->d_zdm, sizeof(float) * (size_t)nspec));
+            DCHK(hipMalloc(&d->d_med, sizeof(float) * nb));
+            DCHK(hipMalloc(&d->d_ravg, sizeof(float) * nb));
+            DCHK(hipMalloc(&d->d_trig, sizeof(float) * nb));
+            DCHK(hipMalloc(&d->d_levels, sizeof(float) * nb * nc));
+            DCHK(hipMalloc(&d->d_crun, sizeof(float) * nc));
+            DCHK(hipMalloc(&d->d_chavg, sizeof(double) * nb * nc));
+            DCHK(hipMalloc(&d->d_bs, sizeof(BlockStat) * nb));
+            DCHK(hipMalloc(&d->d_cst, sizeof(ClipState)));
+        }
         /* DM tiles: the largest group (up to 32 DMs, a power of two) that
          * still gives a pass about kDmWorkgroups tiles and whose widest span
          * fits one batch */
@@ -664,17 +921,21 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     const Tables &t = d->t;
     const CsDedispConfig &c = d->c;
     const int nch = c.numchan, nsub = c.nsub, cps = nch / nsub, ds = c.downsamp;
+    const int P = c.spectra_per_subint;
+    const bool clip = c.clip_sigma > 0.0f;
     /* passes over host spectra: a raw ring of three chunks within 1 GiB */
     long long C = d->chunk;
     if (!on_device) {
-        const int P = c.spectra_per_subint;
         long long want = std::max<long long>(t.worklen, kRingBytes / (3ll * nch * (long long)sizeof(T)));
         want = (want + P - 1) / P * P;
         C = std::min(C, want);
     }
     const long long dsc = C / ds;
     const long long tsub = t.written > 0 ? t.written + t.maxoff : 0;
-    const long long rows = tsub * ds + t.maxd;        /* raw spectra the passes read */
+    /* raw spectra the passes read (whole raw blocks with clipping: a block's
+     * statistics take all of its rows) */
+    long long rows = tsub * ds + t.maxd;
+    if (clip && tsub > 0) rows = std::min(d->nspec, (rows + P - 1) / P * P);
     const long long nk1 = (tsub + dsc - 1) / dsc, nk2 = (t.written + dsc - 1) / dsc;
     const size_t row_bytes = sizeof(T) * (size_t)nch;
     std::vector<hipEvent_t> ev;
@@ -702,6 +963,35 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         return hipEventRecord(raw_ready[j % 3], d->cst) == hipSuccess;
     };
     int sb_gs = 0, sb_pitch = 0, sb_vec = 0;
+    auto clip_view = [&](long long j) {
+        ClipView v{};
+        if (clip) {
+            v.zdm = d->d_zdm;
+            v.ravg = d->d_ravg;
+            v.trig = d->d_trig;
+            v.levels = d->d_levels;
+            v.row0 = j * C;
+            v.P = P;
+        }
+        return v;
+    };
+    /* clip_times' statistics and state for the raw blocks of chunk j */
+    auto clip_chunk = [&](long long j) -> hipError_t {
+        const long long r0 = j * C, r1 = std::min((j + 1) * C, rows);
+        if (!clip || r1 <= r0) return hipSuccess;
+        const long long blk0 = r0 / P;
+        const int nb = (int)((r1 - r0) / P);
+        const T *src = chunk_ptr(j);
+        const int vec16 = (nch * (int)sizeof(T)) % 16 == 0 && (uintptr_t)src % 16 == 0;
+        clip_zdm_kernel<T><<<nb, 256, 0, d->st>>>(src, blk0, P, nch, vec16, d->d_zdm, d->d_med);
+        clip_seq_kernel<<<(nb + 63) / 64, 64, 0, d->st>>>(d->d_zdm, d->d_med, blk0, nb, P, d->d_bs);
+        clip_chavg_kernel<T><<<dim3((unsigned)((nch + 255) / 256), (unsigned)nb), 256, 0, d->st>>>(
+            src, blk0, P, nch, d->d_zdm, d->d_bs, d->d_chavg);
+        clip_scan_kernel<<<1, 256, 0, d->st>>>(blk0, nb, nch, c.clip_sigma, d->d_bs, d->d_chavg,
+                                               d->d_cst, d->d_crun, d->d_ravg, d->d_trig,
+                                               d->d_levels);
+        return hipGetLastError();
+    };
     /* the subband pass of chunk j: nu downsampled samples into `sub` */
     auto launch_sub = [&](long long j, long long nu, float *sub) -> hipError_t {
         const T *ra = chunk_ptr(j), *rb = chunk_ptr(j + 1);
@@ -713,18 +1003,20 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             const dim3 grid((unsigned)((nsub + sb_gs - 1) / sb_gs),
                             (unsigned)((nrow + kRows - 1) / kRows));
             const size_t lds = (size_t)(kRows + t.maxd) * sb_pitch;
-#define CS_SUB(V)                                                                          \
-    subband_tiled<T, V><<<grid, kRows, lds, d->st>>>(ra, rb, C, nch, cps, sb_gs, nsub, ds,   \
-                                                     d->d_idisp, t.maxd, sb_pitch, here,    \
-                                                     (int)nu, sub, (int)dsc)
-            if (sb_vec == 16) CS_SUB(16);
-            else if (sb_vec == 4 || sizeof(T) != 1) CS_SUB(4);
-            else if constexpr (sizeof(T) == 1) CS_SUB(1);
+#define CS_SUB(V, K)                                                                       \
+    subband_tiled<T, V, K><<<grid, kRows, lds, d->st>>>(ra, rb, C, nch, cps, sb_gs, nsub, ds, \
+                                                        d->d_idisp, t.maxd, sb_pitch, here, \
+                                                        (int)nu, sub, (int)dsc, clip_view(j))
+            if (clip) CS_SUB(4, true);
+            else if (sb_vec == 16) CS_SUB(16, false);
+            else if (sb_vec == 4 || sizeof(T) != 1) CS_SUB(4, false);
+            else if constexpr (sizeof(T) == 1) CS_SUB(1, false);
 #undef CS_SUB
         } else {
             const dim3 grid((unsigned)((nu + 63) / 64), (unsigned)((nsub + kWaves - 1) / kWaves));
             subband_kernel<T><<<grid, 64 * kWaves, 0, d->st>>>(ra, rb, C, nch, cps, nsub, ds,
-                                                               d->d_idisp, (int)nu, sub, (int)dsc);
+                                                               d->d_idisp, (int)nu, sub, (int)dsc,
+                                                               clip_view(j));
         }
         return hipGetLastError();
     };
@@ -794,7 +1086,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
      * whose tile fits the LDS budget, loads as wide as the rows' alignment
      * allows (64 % ds == 0: a downsampled sample's lanes share a wave) */
     if (!d->simple && 64 % ds == 0) {
-        const int es = (int)sizeof(T);
+        const int es = clip ? 4 : (int)sizeof(T);      /* tile element: float with clipping */
         const bool al16 = !on_device || (uintptr_t)spectra % 16 == 0;
         sb_vec = (cps * es) % 16 == 0 && (nch * es) % 16 == 0 && al16 ? 16
                  : (cps * es) % 4 == 0 && (nch * es) % 4 == 0 &&
@@ -810,6 +1102,10 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             }
         }
     }
+    if (clip) {
+        DCHK(hipMemsetAsync(d->d_cst, 0, sizeof(ClipState), d->st));
+        DCHK(hipMemsetAsync(d->d_crun, 0, sizeof(float) * (size_t)nch, d->st));
+    }
     if (!copy_chunk(0) || !copy_chunk(1)) goto fail;
     for (long long j = 0; j < nk1; j++) {
         const long long u0 = j * dsc, nu = std::min(dsc, tsub - u0);
@@ -818,6 +1114,10 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             DCHK(hipStreamWaitEvent(d->st, raw_ready[j % 3], 0));
             if ((j + 1) * C < rows) DCHK(hipStreamWaitEvent(d->st, raw_ready[(j + 1) % 3], 0));
         }
+        /* the pass reads the first rows of chunk j + 1: its blocks' clip
+         * levels first (chunk 0's before the first pass) */
+        if (j == 0) DCHK(clip_chunk(0));
+        DCHK(clip_chunk(j + 1));
         DCHK(hipEventRecord(k1s[j], d->st));
         DCHK(launch_sub(j, nu, sub));
         DCHK(hipEventRecord(k1e[j], d->st));

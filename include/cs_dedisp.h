@@ -10,8 +10,8 @@
  * request of the engine (cs_gpu.h, CsSearchRequest.samples) takes a pointer
  * into the plan's series directly.
  *
- * Scope: the nobary path without clipping, masks, zero-DM or ignored
- * channels (prepsubband -nobary -noclip), on spectra already read into
+ * Scope: the nobary path with or without clipping (-clip / -noclip), no
+ * masks, zero-DM or ignored channels (prepsubband -nobary), on spectra already read into
  * memory: nspec x numchan, time-major, lowest frequency first, float32 (what
  * PRESTO's get_rawblock returns) or uint8 (8-bit filterbank samples as their
  * integer values).  The series are bit-identical to the reference's CPU and
@@ -44,6 +44,8 @@ typedef struct {
     int numdms;             /* -numdms */
     int downsamp;           /* -downsamp (a factor of spectra_per_subint) */
     long long numout;       /* -numout; 0: choose_good_N(nspec / downsamp) */
+    float clip_sigma;       /* -clip: time-domain clipping of every raw block
+                               (clip_times, clipping.c:41-215); 0: -noclip */
 } CsDedispConfig;
 
 typedef struct {

@@ -2386,11 +2386,122 @@ int orc_dedisp_tables(const OrcDedispConfig *c, long long nspec, double *dms, in
     return bad ? -1 : bpr;
 }
 
+/* AS 52 running mean and variance, in double over float samples
+ * (misc_utils.c:641-675) */
+void orc_avg_var(const float *x, int n, double *mean, double *var)
+{
+    double m = (double)x[0], v = 0.0, an1 = 0.0;
+    for (int i = 1; i < n; i++) {
+        const double an = (double)(i + 1);
+        an1 = (double)i;
+        const double dx = (x[i] - m) / an;
+        v += an * an1 * dx * dx;
+        m += dx;
+    }
+    if (n > 1) v /= an1;
+    *mean = m;
+    *var = v;
+}
+
+static int orc_fcmp(const void *a, const void *b)
+{
+    const float x = *(const float *)a, y = *(const float *)b;
+    return x < y ? -1 : x > y;
+}
+
+/* the element of rank (n - 1) / 2, which PRESTO's quickselect median
+ * returns (median.c:12-75) */
+float orc_lower_median(const float *x, int n)
+{
+    float *t = (float *)malloc(sizeof(float) * (size_t)n);
+    memcpy(t, x, sizeof(float) * (size_t)n);
+    qsort(t, (size_t)n, sizeof(float), orc_fcmp);
+    const float m = t[(n - 1) / 2];
+    free(t);
+    return m;
+}
+
+typedef struct {
+    float running_avg, running_std;
+    float *chan_running_avg;
+    int blocksread;
+} OrcClip;
+
+/* clip_times on one raw block [P][nch] in place (clipping.c:41-215, without
+ * the CLIPBINSFILE list); levels receives the block's channel levels */
+static int orc_clip_times(OrcClip *st, float *raw, int P, int nch, float clip_sigma,
+                          float *levels)
+{
+    float *zdm = (float *)malloc(sizeof(float) * (size_t)P);
+    float *good = (float *)malloc(sizeof(float) * (size_t)P);
+    double *cavg = (double *)calloc((size_t)nch, sizeof(double));
+    for (int i = 0; i < P; i++) {
+        float z = 0.0f;
+        for (int c = 0; c < nch; c++) z += raw[(long long)i * nch + c];
+        zdm[i] = z;
+    }
+    double cur_avg, cur_std;
+    orc_avg_var(zdm, P, &cur_avg, &cur_std);
+    cur_std = sqrt(cur_std);
+    const float med = orc_lower_median(zdm, P);
+    const float lo = med - 3.0 * cur_std, hi = med + 3.0 * cur_std;
+    int ngood = 0;
+    for (int i = 0; i < P; i++)
+        if (zdm[i] > lo && zdm[i] < hi) {
+            good[ngood++] = zdm[i];
+            for (int c = 0; c < nch; c++) cavg[c] += raw[(long long)i * nch + c];
+        }
+    if (ngood < 1) {
+        cur_avg = st->running_avg;
+        cur_std = st->running_std;
+        for (int c = 0; c < nch; c++) cavg[c] = st->chan_running_avg[c];
+    } else {
+        orc_avg_var(good, ngood, &cur_avg, &cur_std);
+        cur_std = sqrt(cur_std);
+        for (int c = 0; c < nch; c++) cavg[c] /= ngood;
+    }
+    if (st->blocksread) {
+        st->running_avg = 0.9 * st->running_avg + 0.1 * cur_avg;
+        st->running_std = 0.9 * st->running_std + 0.1 * cur_std;
+        for (int c = 0; c < nch; c++)
+            st->chan_running_avg[c] = 0.9 * st->chan_running_avg[c] + 0.1 * cavg[c];
+    } else {
+        st->running_avg = cur_avg;
+        st->running_std = cur_std;
+        for (int c = 0; c < nch; c++) st->chan_running_avg[c] = cavg[c];
+    }
+    const float trigger = clip_sigma * st->running_std;
+    if (levels) memcpy(levels, st->chan_running_avg, sizeof(float) * (size_t)nch);
+    int clipped = 0;
+    for (int i = 0; i < P; i++)
+        if (fabsf(zdm[i] - st->running_avg) > trigger) {
+            memcpy(raw + (long long)i * nch, st->chan_running_avg, sizeof(float) * (size_t)nch);
+            clipped++;
+        }
+    st->blocksread++;
+    free(zdm); free(good); free(cavg);
+    return clipped;
+}
+
+long long orc_clip_all(const OrcDedispConfig *c, float *raw, long long nspec, float *levels)
+{
+    const int P = c->P, nch = c->numchan;
+    if (P <= 0 || nch <= 0 || nspec % P) return -1;
+    OrcClip st = {0.0f, 0.0f, (float *)calloc((size_t)nch, sizeof(float)), 0};
+    long long n = 0;
+    for (long long b = 0; b < nspec / P; b++)
+        n += orc_clip_times(&st, raw + b * (long long)P * nch, P, nch, c->clip_sigma,
+                            levels ? levels + b * (long long)nch : NULL);
+    free(st.chan_running_avg);
+    return n;
+}
+
 typedef struct {
     const float *raw;
     long long nblk, next;   /* whole raw blocks, the next one to read */
     int first;
     float *last, *cur;      /* raw blocks, [P][numchan] */
+    OrcClip clip;           /* clip_times' state across blocks */
 } OrcReader;
 
 /* read_subbands + prep_subbands + dedisp_subbands into sub block `blk` of
@@ -2400,14 +2511,18 @@ static int orc_read_subbands(OrcReader *r, const OrcDedispConfig *c, const int *
 {
     const int P = c->P, nch = c->numchan, cps = nch / c->nsub;
     const size_t bytes = sizeof(float) * (size_t)P * nch;
+    /* prep_subbands clips every block it copies in, the first included
+     * (backend_common.c:936-1000) */
     if (r->first) {
         if (r->next >= r->nblk) return -1;
         memcpy(r->last, r->raw + r->next * (long long)P * nch, bytes);
+        if (c->clip_sigma > 0.0f) orc_clip_times(&r->clip, r->last, P, nch, c->clip_sigma, NULL);
         r->next++;
         r->first = 0;
     }
     if (r->next >= r->nblk) return 0;
     memcpy(r->cur, r->raw + r->next * (long long)P * nch, bytes);
+    if (c->clip_sigma > 0.0f) orc_clip_times(&r->clip, r->cur, P, nch, c->clip_sigma, NULL);
     r->next++;
     for (int s = 0; s < c->nsub; s++)
         for (int j = 0; j < P; j++) fdata[(long long)s * W + (long long)blk * P + j] = 0.0f;
@@ -2439,7 +2554,8 @@ long long orc_dedisp(const OrcDedispConfig *c, const float *raw, long long nspec
         return -1;
     }
     const int W = P * bpr, dsW = W / ds;
-    OrcReader rd = {raw, nspec / P, 0, 1, NULL, NULL};
+    OrcReader rd = {raw, nspec / P, 0, 1, NULL, NULL,
+                    {0.0f, 0.0f, (float *)calloc((size_t)nch, sizeof(float)), 0}};
     rd.last = (float *)malloc(sizeof(float) * (size_t)P * nch);
     rd.cur = (float *)malloc(sizeof(float) * (size_t)P * nch);
     float *data[2], *dsdata[2];
@@ -2518,7 +2634,8 @@ long long orc_dedisp(const OrcDedispConfig *c, const float *raw, long long nspec
         if (ds > 1) free(dsdata[i]);
         free(data[i]);
     }
-    free(res); free(rd.last); free(rd.cur); free(dms); free(idispdt); free(offs);
+    free(res); free(rd.last); free(rd.cur); free(rd.clip.chan_running_avg);
+    free(dms); free(idispdt); free(offs);
     return failed ? -1 : totwrote;
 }
 

@@ -189,6 +189,7 @@ typedef struct {
     int numdms;
     int downsamp;
     long long numout;     /* per-DM output length; 0: choose_good_N(nspec / downsamp) */
+    float clip_sigma;     /* time-domain clipping (clip_times) of every raw block; 0: off */
 } OrcDedispConfig;
 
 double    orc_delay_from_dm(double dm, double freq_mhz);
@@ -204,6 +205,14 @@ int orc_dedisp_tables(const OrcDedispConfig *c, long long nspec, double *dms, in
  * *pad receives the padding value. */
 long long orc_dedisp(const OrcDedispConfig *c, const float *raw, long long nspec, float *out,
                      float *pad);
+/* clip_times (clipping.c:41-215) of every raw block of raw[nspec][numchan],
+ * in place, in block order, from a fresh state; levels (optional,
+ * [nspec / P][numchan]) receive each block's channel levels.  Returns the
+ * number of rows replaced. */
+long long orc_clip_all(const OrcDedispConfig *c, float *raw, long long nspec, float *levels);
+/* PRESTO avg_var (AS 52) and the lower median of n floats */
+void  orc_avg_var(const float *x, int n, double *mean, double *var);
+float orc_lower_median(const float *x, int n);
 /* The same series by direct (unblocked) indexing: the closed form the
  * blocked loop must equal.  Writes ntime samples per DM, no padding. */
 int orc_dedisp_direct(const OrcDedispConfig *c, const float *raw, long long nspec,

@@ -116,6 +116,10 @@ def lib():
         L.orc_dedisp.restype = LL
         L.orc_dedisp_direct.argtypes = [C.POINTER(OrcDedispConfig), P, LL, LL, P]
         L.orc_dedisp_direct.restype = I
+        L.orc_clip_all.argtypes = [C.POINTER(OrcDedispConfig), P, LL, P]
+        L.orc_clip_all.restype = LL
+        L.orc_avg_var.argtypes = [P, I, C.POINTER(D), C.POINTER(D)]
+        L.orc_lower_median.argtypes = [P, I]; L.orc_lower_median.restype = C.c_float
         L.free = C.CDLL(None).free
         L.free.argtypes = [P]
         _lib = L
@@ -639,13 +643,36 @@ class OrcDedispConfig(C.Structure):
     _fields_ = [("numchan", C.c_int), ("lofreq", C.c_double), ("chan_wid", C.c_double),
                 ("dt", C.c_double), ("P", C.c_int), ("nsub", C.c_int),
                 ("lodm", C.c_double), ("dmstep", C.c_double), ("numdms", C.c_int),
-                ("downsamp", C.c_int), ("numout", C.c_longlong)]
+                ("downsamp", C.c_int), ("numout", C.c_longlong), ("clip_sigma", C.c_float)]
 
 
 def dedisp_config(numchan, lofreq, chan_wid, dt, P, nsub, lodm, dmstep, numdms, downsamp=1,
-                  numout=0):
+                  numout=0, clip_sigma=0.0):
     return OrcDedispConfig(numchan, lofreq, chan_wid, dt, P, nsub, lodm, dmstep, numdms,
-                           downsamp, numout)
+                           downsamp, numout, clip_sigma)
+
+
+def clip_all(cfg, raw):
+    """clip_times over every raw block, in order: (clipped copy, levels
+    [nblocks, numchan], rows replaced)"""
+    x = np.array(raw, dtype=np.float32, order="C")
+    lv = np.zeros((x.shape[0] // cfg.P, cfg.numchan), np.float32)
+    n = lib().orc_clip_all(C.byref(cfg), ptr(x), x.shape[0], ptr(lv))
+    if n < 0:
+        raise ValueError("not whole blocks")
+    return x, lv, n
+
+
+def avg_var(x):
+    x = np.ascontiguousarray(x, np.float32)
+    m, v = C.c_double(), C.c_double()
+    lib().orc_avg_var(ptr(x), x.size, C.byref(m), C.byref(v))
+    return m.value, v.value
+
+
+def lower_median(x):
+    x = np.ascontiguousarray(x, np.float32)
+    return lib().orc_lower_median(ptr(x), x.size)
 
 
 def delay_from_dm(dm, f):

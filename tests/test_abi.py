@@ -58,7 +58,8 @@ def test_struct_layouts():
     assert C.sizeof(abi.CsSearchInfo) == 96 and abi.CsSearchInfo.kernel_ms.offset == 56
     assert abi.CsSearchInfo.search_ms.offset == 80 and abi.CsSearchInfo.post_ms.offset == 88
     assert C.sizeof(abi.CsEngineStats) == 72 and abi.CsEngineStats.completed.offset == 48
-    assert C.sizeof(abi.CsDedispConfig) == 72 and abi.CsDedispConfig.numout.offset == 64
+    assert C.sizeof(abi.CsDedispConfig) == 80 and abi.CsDedispConfig.numout.offset == 64
+    assert abi.CsDedispConfig.clip_sigma.offset == 72
     assert abi.CsDedispConfig.lodm.offset == 40 and abi.CsDedispConfig.downsamp.offset == 60
     assert C.sizeof(abi.CsDedispInfo) == 56 and abi.CsDedispInfo.pad.offset == 48
     assert abi.CsEngineStats.plans_resident.offset == 64 and abi.CsSearchInfo.reused.offset == 4

@@ -156,3 +156,57 @@ def test_rejections(oracle):
     with pytest.raises(RuntimeError):
         d.run(np.zeros((256 * 8, 64), np.float32))
     d.close()
+
+
+# ---- clip_times (clipping.c:41-215) ------------------------------------------
+
+def _rfi(raw, nrows, seed, level=40.0):
+    rng = np.random.default_rng(seed)
+    rows = np.sort(rng.choice(raw.shape[0], nrows, replace=False))
+    raw = raw.copy()
+    raw[rows] += level
+    return raw, rows
+
+
+def test_clip_statistics_closed_forms(oracle):
+    rng = np.random.default_rng(4)
+    x = rng.normal(3.0, 2.0, 4097).astype(np.float32)
+    m, v = oracle.avg_var(x)
+    assert m == pytest.approx(x.astype(np.float64).mean(), rel=1e-13)
+    assert v == pytest.approx(x.astype(np.float64).var(ddof=1), rel=1e-12)
+    for n in (1, 2, 3, 4, 2048, 2049):
+        y = rng.normal(size=n).astype(np.float32)
+        assert oracle.lower_median(y) == np.sort(y)[(n - 1) // 2]
+
+
+def test_clip_replaces_exactly_the_rfi_rows(oracle):
+    cfg = CFGS[0]
+    c = oracle.dedisp_config(*cfg, clip_sigma=6.0)
+    raw = _raw(cfg, 48, 3)
+    dirty, rows = _rfi(raw, 40, 1)
+    x, levels, n = oracle.clip_all(c, dirty)
+    changed = np.where(np.any(x != dirty, axis=1))[0]
+    assert n == len(rows) and np.array_equal(changed, rows)
+    # a replaced row holds its block's channel levels; the first block's
+    # levels are the channel means of its good (unflagged) rows
+    P = cfg[4]
+    for r in rows:
+        assert np.array_equal(x[r], levels[r // P])
+    b0 = dirty[:P].astype(np.float64)
+    good = np.setdiff1d(np.arange(P), rows)
+    assert np.allclose(levels[0], b0[good].mean(axis=0), rtol=1e-6)
+    # clip_sigma 0 leaves the data alone
+    c0 = oracle.dedisp_config(*cfg, clip_sigma=0.0)
+    out0, _, _ = oracle.dedisp(c0, dirty)
+    outc, nw, _ = oracle.dedisp(c, dirty)
+    assert not np.array_equal(out0, outc)
+
+
+@pytest.mark.parametrize("cfg", CFGS[:3])
+def test_clipped_blocked_loop_equals_closed_form(oracle, cfg):
+    c = oracle.dedisp_config(*cfg, clip_sigma=6.0)
+    dirty, _ = _rfi(_raw(cfg, 48, 3), 60, 2)
+    out, n, pad = oracle.dedisp(c, dirty)
+    x, _, _ = oracle.clip_all(c, dirty)
+    direct = oracle.dedisp_direct(c, x, n)
+    assert np.array_equal(out[:, :n].view(np.uint32), direct.view(np.uint32))

@@ -87,6 +87,52 @@ def test_series_match_oracle(oracle, c, kernels, chunk, monkeypatch):
     d.close()
 
 
+CLIPPED = [  # (case, passes of 2^16 spectra or one pass)
+    (CASES[0], 0), (CASES[1], 0), (CASES[3], 1 << 16), (CASES[5], 0)]
+
+
+@pytest.mark.parametrize("kernels", ["tiled", "untiled"])
+@pytest.mark.parametrize("case", range(len(CLIPPED)))
+def test_clipped_series_match_oracle(oracle, case, kernels, monkeypatch):
+    """prepsubband's default time-domain clipping (clip_times at 6 sigma) of
+    every raw block, on spectra with broadband RFI bursts"""
+    c, chunk = CLIPPED[case]
+    monkeypatch.setenv("CS_DEDISP_SIMPLE", "1" if kernels == "untiled" else "0")
+    monkeypatch.setenv("CS_DEDISP_CHUNK", str(chunk))
+    nspec = c[4] * c[10]
+    u8 = c[11]
+    raw = synth.filterbank_with_pulsar(nspec, c[0], c[1], c[2], c[3], 37.0, 0.0213, 0.04, 25.0,
+                                       13, dtype=np.uint8 if u8 else np.float32)
+    rng = np.random.default_rng(5)
+    rows = rng.choice(nspec, max(8, nspec // 500), replace=False)
+    if u8:
+        raw[rows] = np.minimum(raw[rows].astype(np.int32) + 60, 255).astype(np.uint8)
+    else:
+        raw[rows] += 40.0
+    cfg = oracle.dedisp_config(*c[:10], clip_sigma=6.0)
+    want, n, pad = oracle.dedisp(cfg, raw.astype(np.float32))
+    _, _, nclip = oracle.clip_all(cfg, raw.astype(np.float32))
+    say("clip %s: %d chans x %d spectra (%s), %d rows clipped, written %d"
+        % (kernels, c[0], nspec, raw.dtype, nclip, n))
+    assert nclip > 0
+    d = cs.Dedisp(c[0], c[1], c[2], c[3], nspec, c[5], c[6], c[7], c[8],
+                  spectra_per_subint=c[4], downsamp=c[9], clip_sigma=6.0)
+    d.run(raw)
+    assert np.array_equal(d.copy_out().view(np.uint32), want.view(np.uint32))
+    assert d.info["pad"] == pad
+    import torch
+    t = torch.from_numpy(raw).to("cuda:0")
+
+    class Resident:
+        ptr = t.data_ptr()
+        uint8 = u8
+
+    d.run(Resident())
+    torch.cuda.synchronize()
+    assert np.array_equal(d.copy_out().view(np.uint32), want.view(np.uint32))
+    d.close()
+
+
 def test_numout_shorter_and_longer(oracle):
     c = CASES[0]
     nspec = c[4] * c[10]
