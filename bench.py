@@ -363,6 +363,23 @@ def dedisp_timing(steps=5, ref_flow=True):
     log("dedisp: %s" % res["workload"])
     log("  device %.3f ms (subband %.3f ms, %.0f GB/s; DM %.3f ms, %.0f GB/s), wall %.2f ms"
         % (ms_total, ms_sub, res["subband_gbs"], ms_dm, res["dm_gbs"], res["wall_ms"]))
+    # prepsubband's default time-domain clipping (-clip 6) on the same spectra
+    dc = cs.Dedisp(nch, lo, cw, dt, nspec, nsub, lodm, dmstep, nd, spectra_per_subint=P,
+                   downsamp=ds, clip_sigma=6.0)
+    dc.run(Resident())
+    cdev, cw_ = [], []
+    for _ in range(steps):
+        w = time.perf_counter()
+        dc.run(Resident())
+        cw_.append(time.perf_counter() - w)
+        cdev.append(dc.last_ms())
+    cdev = np.array(cdev)
+    res["clip6_device_ms"] = float(np.mean(cdev[:, 0]))
+    res["clip6_clip_ms"] = float(np.mean(cdev[:, 0] - cdev[:, 1] - cdev[:, 2]))
+    res["clip6_wall_ms"] = 1e3 * float(np.mean(cw_))
+    log("  with clipping (-clip 6): device %.3f ms (clip statistics %.3f ms), wall %.2f ms"
+        % (res["clip6_device_ms"], res["clip6_clip_ms"], res["clip6_wall_ms"]))
+    dc.close()
     # from pinned host memory (the reference reads blocks on the host)
     host = raw.cpu().pin_memory()
     w = time.perf_counter()

@@ -270,6 +270,7 @@ constexpr int kDmThreads = 256;
 constexpr int kGMax = 32;             /* DMs per DM tile (register accumulators) */
 constexpr int kSpanFloats = 8192;     /* LDS floats of one subband batch of a DM tile */
 constexpr int kDmWorkgroups = 2048;   /* DM tiles per pass worth keeping the GPU busy with */
+constexpr int kMaxClipDelay = 2048;   /* deepest subband delay the clipped tiles take */
 
 template <int VEC>
 struct Vec;
@@ -314,9 +315,7 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
               int cps, int gs, int nsub, int ds, const int *__restrict__ idisp, int maxd,
               int pitch, long long rows, int nu, float *__restrict__ out, int dsc, ClipView cv)
 {
-    /* with clipping the tile holds floats (a clipped row's channel levels
-     * are not samples of type T), loaded one element per lane */
-    using TT = typename std::conditional<CLIP, float, T>::type;
+
     extern __shared__ __align__(16) unsigned char tile[];
     using W = typename Vec<VEC>::type;
     /* x: subband group (fast, so one row tile's groups run together and
@@ -325,20 +324,17 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
     const int c0 = s0 * cps;
     const long long t0 = (long long)blockIdx.y * kRows;
     const int nr = kRows + maxd;
+    /* with clipping: each row's clip test once (its block, or -1), kept in
+     * LDS; a clipped row's channel levels replace its samples where the sums
+     * read them */
+    __shared__ int rblk[CLIP ? kRows + kMaxClipDelay : 1];
     if constexpr (CLIP) {
-        const int ne = ns * cps;                          /* elements per row */
-        for (int idx = (int)threadIdx.x; idx < nr * ne; idx += kRows) {
-            const int r = idx / ne, e = idx - r * ne;
+        for (int r = (int)threadIdx.x; r < nr; r += kRows) {
             const long long t = t0 + r;
-            float x = 0.0f;
-            if (t < rows) {
-                const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
-                const int b = clip_block(cv, cv.row0 + t);
-                x = b >= 0 ? cv.levels[(long long)b * nchan + c0 + e] : (float)row[c0 + e];
-            }
-            *(float *)(tile + (size_t)r * pitch + (size_t)e * 4) = x;
+            rblk[r] = t < rows ? clip_block(cv, cv.row0 + t) : -1;
         }
-    } else {
+    }
+    {
         const int nv = ns * cps * (int)sizeof(T) / VEC;   /* loads per row */
         for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
             const int r = idx / nv, v = idx - r * nv;
@@ -360,7 +356,12 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
         for (int j = 0; j < cps; j++) {
             const int ch = s * cps + j;
             const int d = idisp[c0 + ch];
-            v += (float)*(const TT *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(TT));
+            float x = (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
+            if constexpr (CLIP) {
+                const int b = rblk[r + d];
+                if (b >= 0) x = cv.levels[(long long)b * nchan + c0 + ch];
+            }
+            v += x;
         }
         float acc = v;
         if (ds > 1) {
@@ -580,63 +581,97 @@ __global__ void clip_seq_kernel(const float *__restrict__ zdm, const float *__re
 }
 
 /* one lane per (channel, block): the channel's mean over the block's good
- * rows, summed in double in row order */
+ * rows, summed in double in row order; the rows' good flags go through LDS
+ * 4096 at a time */
 template <typename T>
-__global__ void clip_chavg_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch,
-                                  const float *__restrict__ zdm,
-                                  const BlockStat *__restrict__ bs, double *__restrict__ chavg)
+__global__ void __launch_bounds__(256)
+clip_chavg_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch,
+                  const float *__restrict__ zdm, const BlockStat *__restrict__ bs,
+                  double *__restrict__ chavg)
 {
+    __shared__ unsigned char good[4096];
     const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x), b = (int)blockIdx.y;
-    if (c >= nch) return;
     const BlockStat st = bs[blk0 + b];
     const T *blk = chunk + (long long)b * P * nch;
     const float *z = zdm + (blk0 + b) * P;
     double acc = 0.0;
-    for (int i = 0; i < P; i++) {
-        const float x = z[i];
-        if (x > st.lo && x < st.hi) acc += (double)(float)blk[(long long)i * nch + c];
+    for (int i0 = 0; i0 < P; i0 += 4096) {
+        const int n = min(4096, P - i0);
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < n; i += 256) {
+            const float x = z[i0 + i];
+            good[i] = x > st.lo && x < st.hi;
+        }
+        __syncthreads();
+        /* loads unconditional (in flight together), adds of 0.0 for bad
+         * rows: acc starts at +0.0 and never becomes -0.0, so adding +0.0
+         * leaves it unchanged, as skipping the row does */
+        if (c < nch) {
+            const T *col = blk + (long long)i0 * nch + c;
+#pragma unroll 8
+            for (int i = 0; i < n; i++) {
+                const double x = (double)(float)col[(long long)i * nch];
+                acc += good[i] ? x : 0.0;
+            }
+        }
     }
-    chavg[(blk0 + b) * nch + c] = st.ngood > 0 ? acc / st.ngood : 0.0;
+    if (c < nch) chavg[(blk0 + b) * nch + c] = st.ngood > 0 ? acc / st.ngood : 0.0;
 }
 
 /* the running state over the blocks [blk0, blk0 + nblk), continuing from
- * `st`; one workgroup, lanes over channels */
+ * `in` into `out` (two buffers: every workgroup reads the state the previous
+ * launch left).  One lane per channel walks the blocks: the channel levels'
+ * recurrences are independent of each other, and the zero-DM mean and std
+ * (uniform) are recomputed by every lane from the block statistics, which go
+ * through LDS 512 blocks at a time. */
 __global__ void __launch_bounds__(256)
 clip_scan_kernel(long long blk0, int nblk, int nch, float clip_sigma,
                  const BlockStat *__restrict__ bs, const double *__restrict__ chavg,
-                 ClipState *__restrict__ st, float *__restrict__ crun, float *__restrict__ ravg,
-                 float *__restrict__ trig, float *__restrict__ levels)
+                 const ClipState *__restrict__ in, ClipState *__restrict__ out,
+                 float *__restrict__ crun, float *__restrict__ ravg, float *__restrict__ trig,
+                 float *__restrict__ levels)
 {
-    float ra = st->running_avg, rs = st->running_std;
-    int nread = st->blocksread;
-    for (int b = 0; b < nblk; b++) {
-        const long long B = blk0 + b;
-        const BlockStat s = bs[B];
-        const bool none = s.ngood < 1;
-        const double ca = none ? (double)ra : s.avg, cs = none ? (double)rs : s.std;
-        if (nread) {
-            ra = 0.9 * ra + 0.1 * ca;
-            rs = 0.9 * rs + 0.1 * cs;
-        } else {
-            ra = ca;
-            rs = cs;
+    __shared__ BlockStat sb[512];
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    float ra = in->running_avg, rs = in->running_std;
+    int nread = in->blocksread;
+    float cr = c < nch ? crun[c] : 0.0f;
+    for (int b0 = 0; b0 < nblk; b0 += 512) {
+        const int n = min(512, nblk - b0);
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < n; i += 256) sb[i] = bs[blk0 + b0 + i];
+        __syncthreads();
+#pragma unroll 8
+        for (int i = 0; i < n; i++) {
+            const long long B = blk0 + b0 + i;
+            const BlockStat s = sb[i];
+            const bool none = s.ngood < 1;
+            const double ca = none ? (double)ra : s.avg, cs = none ? (double)rs : s.std;
+            if (nread) {
+                ra = 0.9 * ra + 0.1 * ca;
+                rs = 0.9 * rs + 0.1 * cs;
+            } else {
+                ra = ca;
+                rs = cs;
+            }
+            if (c < nch) {
+                const double cv = none ? (double)cr : chavg[B * nch + c];
+                cr = nread ? (float)(0.9 * cr + 0.1 * cv) : (float)cv;
+                levels[B * nch + c] = cr;
+            }
+            if (lead) {
+                ravg[B] = ra;
+                trig[B] = clip_sigma * rs;
+            }
+            nread++;
         }
-        for (int c = (int)threadIdx.x; c < nch; c += (int)blockDim.x) {
-            const double cv = none ? (double)crun[c] : chavg[B * nch + c];
-            const float nv = nread ? (float)(0.9 * crun[c] + 0.1 * cv) : (float)cv;
-            crun[c] = nv;
-            levels[B * nch + c] = nv;
-        }
-        if (threadIdx.x == 0) {
-            ravg[B] = ra;
-            trig[B] = clip_sigma * rs;
-        }
-        nread++;
     }
-    if (threadIdx.x == 0) {
-        st->running_avg = ra;
-        st->running_std = rs;
-        st->blocksread = nread;
+    if (c < nch) crun[c] = cr;
+    if (lead) {
+        out->running_avg = ra;
+        out->running_std = rs;
+        out->blocksread = nread;
     }
 }
 
@@ -792,7 +827,7 @@ extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long l
             DCHK(hipMalloc(&d->d_crun, sizeof(float) * nc));
             DCHK(hipMalloc(&d->d_chavg, sizeof(double) * nb * nc));
             DCHK(hipMalloc(&d->d_bs, sizeof(BlockStat) * nb));
-            DCHK(hipMalloc(&d->d_cst, sizeof(ClipState)));
+            DCHK(hipMalloc(&d->d_cst, 2 * sizeof(ClipState)));
         }
         /* DM tiles: the largest group (up to 32 DMs, a power of two) that
          * still gives a pass about kDmWorkgroups tiles and whose widest span
@@ -939,7 +974,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     const long long nk1 = (tsub + dsc - 1) / dsc, nk2 = (t.written + dsc - 1) / dsc;
     const size_t row_bytes = sizeof(T) * (size_t)nch;
     std::vector<hipEvent_t> ev;
-    hipEvent_t raw_ready[3] = {}, k1_done[3] = {};
+    hipEvent_t raw_ready[3] = {}, k1_done[3] = {}, t_start = nullptr;
     int rc = -1;
     auto new_event = [&](hipEvent_t *e, unsigned flags) {
         if (hipEventCreateWithFlags(e, flags) != hipSuccess) return false;
@@ -975,21 +1010,30 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         }
         return v;
     };
+    int nscan = 0;              /* scans so far: the state's buffer */
     /* clip_times' statistics and state for the raw blocks of chunk j */
     auto clip_chunk = [&](long long j) -> hipError_t {
-        const long long r0 = j * C, r1 = std::min((j + 1) * C, rows);
+        /* j = -1: every block the passes read (spectra in HBM) */
+        const long long r0 = j < 0 ? 0 : j * C, r1 = j < 0 ? rows : std::min((j + 1) * C, rows);
         if (!clip || r1 <= r0) return hipSuccess;
         const long long blk0 = r0 / P;
         const int nb = (int)((r1 - r0) / P);
-        const T *src = chunk_ptr(j);
+        const T *src = j < 0 ? spectra : chunk_ptr(j);
         const int vec16 = (nch * (int)sizeof(T)) % 16 == 0 && (uintptr_t)src % 16 == 0;
         clip_zdm_kernel<T><<<nb, 256, 0, d->st>>>(src, blk0, P, nch, vec16, d->d_zdm, d->d_med);
         clip_seq_kernel<<<(nb + 63) / 64, 64, 0, d->st>>>(d->d_zdm, d->d_med, blk0, nb, P, d->d_bs);
-        clip_chavg_kernel<T><<<dim3((unsigned)((nch + 255) / 256), (unsigned)nb), 256, 0, d->st>>>(
-            src, blk0, P, nch, d->d_zdm, d->d_bs, d->d_chavg);
-        clip_scan_kernel<<<1, 256, 0, d->st>>>(blk0, nb, nch, c.clip_sigma, d->d_bs, d->d_chavg,
-                                               d->d_cst, d->d_crun, d->d_ravg, d->d_trig,
-                                               d->d_levels);
+        for (int b0 = 0; b0 < nb; b0 += 65535) {           /* grid y: at most 65535 */
+            const int n = std::min(65535, nb - b0);
+            clip_chavg_kernel<T><<<dim3((unsigned)((nch + 255) / 256), (unsigned)n), 256, 0,
+                                   d->st>>>(src + (long long)b0 * P * nch, blk0 + b0, P, nch,
+                                            d->d_zdm, d->d_bs, d->d_chavg);
+        }
+        const ClipState *sin = d->d_cst + (nscan & 1);
+        ClipState *sout = d->d_cst + ((nscan + 1) & 1);
+        nscan++;
+        clip_scan_kernel<<<(nch + 255) / 256, 256, 0, d->st>>>(
+            blk0, nb, nch, c.clip_sigma, d->d_bs, d->d_chavg, sin, sout, d->d_crun, d->d_ravg,
+            d->d_trig, d->d_levels);
         return hipGetLastError();
     };
     /* the subband pass of chunk j: nu downsampled samples into `sub` */
@@ -1007,8 +1051,11 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     subband_tiled<T, V, K><<<grid, kRows, lds, d->st>>>(ra, rb, C, nch, cps, sb_gs, nsub, ds, \
                                                         d->d_idisp, t.maxd, sb_pitch, here, \
                                                         (int)nu, sub, (int)dsc, clip_view(j))
-            if (clip) CS_SUB(4, true);
-            else if (sb_vec == 16) CS_SUB(16, false);
+            if (clip) {
+                if (sb_vec == 16) CS_SUB(16, true);
+                else if (sb_vec == 4 || sizeof(T) != 1) CS_SUB(4, true);
+                else if constexpr (sizeof(T) == 1) CS_SUB(1, true);
+            } else if (sb_vec == 16) CS_SUB(16, false);
             else if (sb_vec == 4 || sizeof(T) != 1) CS_SUB(4, false);
             else if constexpr (sizeof(T) == 1) CS_SUB(1, false);
 #undef CS_SUB
@@ -1071,6 +1118,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             if (!new_event(&e, hipEventDefault)) goto fail;
     for (auto &e : cp)
         if (!new_event(&e, hipEventDisableTiming)) goto fail;
+    if (!new_event(&t_start, hipEventDefault)) goto fail;
     if (!on_device && nk1 > 0) {
         const size_t need = 3 * (size_t)C * row_bytes;
         if (d->ring_bytes < need) {
@@ -1086,14 +1134,16 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
      * whose tile fits the LDS budget, loads as wide as the rows' alignment
      * allows (64 % ds == 0: a downsampled sample's lanes share a wave) */
     if (!d->simple && 64 % ds == 0) {
-        const int es = clip ? 4 : (int)sizeof(T);      /* tile element: float with clipping */
+        /* loads: VEC bytes of samples (alignment of the raw rows); tile
+         * elements: floats with clipping, samples otherwise */
+        const int rs = (int)sizeof(T), es = rs;
         const bool al16 = !on_device || (uintptr_t)spectra % 16 == 0;
-        sb_vec = (cps * es) % 16 == 0 && (nch * es) % 16 == 0 && al16 ? 16
-                 : (cps * es) % 4 == 0 && (nch * es) % 4 == 0 &&
+        sb_vec = (cps * rs) % 16 == 0 && (nch * rs) % 16 == 0 && al16 ? 16
+                 : (cps * rs) % 4 == 0 && (nch * rs) % 4 == 0 &&
                            (!on_device || (uintptr_t)spectra % 4 == 0)
                      ? 4
                      : 1;
-        for (int gs = nsub; gs >= 1; gs--) {
+        for (int gs = nsub; gs >= 1 && !(clip && t.maxd > kMaxClipDelay); gs--) {
             const int pitch = (gs * cps * es + 15) / 16 * 16 + 4;    /* odd dwords */
             if ((long long)(kRows + t.maxd) * pitch <= kTileBytes) {
                 sb_gs = gs;
@@ -1102,8 +1152,9 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             }
         }
     }
+    DCHK(hipEventRecord(t_start, d->st));
     if (clip) {
-        DCHK(hipMemsetAsync(d->d_cst, 0, sizeof(ClipState), d->st));
+        DCHK(hipMemsetAsync(d->d_cst, 0, 2 * sizeof(ClipState), d->st));
         DCHK(hipMemsetAsync(d->d_crun, 0, sizeof(float) * (size_t)nch, d->st));
     }
     if (!copy_chunk(0) || !copy_chunk(1)) goto fail;
@@ -1115,9 +1166,15 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             if ((j + 1) * C < rows) DCHK(hipStreamWaitEvent(d->st, raw_ready[(j + 1) % 3], 0));
         }
         /* the pass reads the first rows of chunk j + 1: its blocks' clip
-         * levels first (chunk 0's before the first pass) */
-        if (j == 0) DCHK(clip_chunk(0));
-        DCHK(clip_chunk(j + 1));
+         * levels first (chunk 0's before the first pass).  Spectra in HBM:
+         * every block's at once, so the per-block statistics run on all
+         * blocks in parallel */
+        if (on_device) {
+            if (j == 0) DCHK(clip_chunk(-1));
+        } else {
+            if (j == 0) DCHK(clip_chunk(0));
+            DCHK(clip_chunk(j + 1));
+        }
         DCHK(hipEventRecord(k1s[j], d->st));
         DCHK(launch_sub(j, nu, sub));
         DCHK(hipEventRecord(k1e[j], d->st));
@@ -1170,7 +1227,8 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         d->last_sub_ms = s1;
         d->last_dm_ms = s2;
         d->last_ms = 0.0;
-        if (nk1 > 0 && hipEventElapsedTime(&ms, k1s[0], nk2 > 0 ? k2e[nk2 - 1] : k1e[nk1 - 1]) ==
+        /* from the start of the run (clipping included) to the last DM pass */
+        if (nk1 > 0 && hipEventElapsedTime(&ms, t_start, nk2 > 0 ? k2e[nk2 - 1] : k1e[nk1 - 1]) ==
                            hipSuccess)
             d->last_ms = ms;
     }

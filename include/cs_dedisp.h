@@ -85,8 +85,9 @@ CS_API int cs_gpu_dedisp_run(CsDedisp *d, const void *spectra, int is_uint8);
 CS_API const float *cs_gpu_dedisp_series(const CsDedisp *d, int i);
 /* copy DMs [i0, i0 + n) to host memory (n * numout floats) */
 CS_API int cs_gpu_dedisp_copy_out(const CsDedisp *d, int i0, int n, float *host);
-/* milliseconds of the last run's device work (HIP events around the
- * subband and DM kernels), and its launches */
+/* milliseconds of the last run's device work (HIP events on the plan's
+ * stream, from the run's start, clipping included, to its last DM pass), and
+ * the subband and DM kernels' shares */
 CS_API double cs_gpu_dedisp_last_ms(const CsDedisp *d, double *subband_ms, double *dm_ms);
 
 /* PRESTO helpers the tables use (third_party/presto/src/misc_utils.c:334-373,
