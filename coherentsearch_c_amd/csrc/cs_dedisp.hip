@@ -477,7 +477,10 @@ __device__ __forceinline__ float key_float(unsigned k)
 }
 
 /* zero-DM sums (float, channel order) and their element of rank (P-1)/2 by
- * an 8-bit radix select; one raw block per workgroup */
+ * an 8-bit radix select; one raw block per workgroup.  The sums go through
+ * LDS: 256 rows x 64 (float: 32) channels at a time are loaded coalesced (rows
+ * padded to an odd number of dwords), then lane r adds row r's channels in order,
+ * its sum carried from one channel chunk to the next. */
 template <typename T>
 __global__ void __launch_bounds__(256)
 clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int vec16,
@@ -485,24 +488,63 @@ clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int
 {
     __shared__ unsigned hist[256];
     __shared__ unsigned s_prefix, s_k;
+    constexpr int CW = sizeof(T) == 1 ? 64 : 32;             /* channels per chunk */
+    constexpr int PITCH = CW * (int)sizeof(T) + 4;           /* bytes: 17 or 33 dwords */
+    constexpr int NSORT = 8192;                              /* blocks sorted in LDS */
+    constexpr int SMEM = 256 * PITCH > NSORT * 4 ? 256 * PITCH : NSORT * 4;
+    /* the row tile of the sums, then the keys of the sort */
+    __shared__ __align__(16) unsigned char smem[SMEM];
+    unsigned char *rows_ = smem;
     const int b = (int)blockIdx.x;
     const T *blk = chunk + (long long)b * P * nch;
     float *z = zdm + (blk0 + b) * P;
-    for (int i = (int)threadIdx.x; i < P; i += 256) {
-        const T *row = blk + (long long)i * nch;
+    (void)vec16;
+    for (int i0 = 0; i0 < P; i0 += 256) {
+        const int nrow = min(256, P - i0);
         float sum = 0.0f;
-        if (vec16) {
-            constexpr int E = 16 / (int)sizeof(T);
-            for (int c = 0; c < nch; c += E) {
-                const uint4 w = *(const uint4 *)(row + c);
-                const T *e = (const T *)&w;
-#pragma unroll
-                for (int q = 0; q < E; q++) sum += (float)e[q];
+        for (int c0 = 0; c0 < nch; c0 += CW) {
+            const int cw = min(CW, nch - c0);
+            __syncthreads();
+            for (int idx = (int)threadIdx.x; idx < nrow * cw; idx += 256) {
+                const int r = idx / cw, e = idx - r * cw;
+                *(T *)(rows_ + r * PITCH + e * (int)sizeof(T)) =
+                    blk[(long long)(i0 + r) * nch + c0 + e];
             }
-        } else {
-            for (int c = 0; c < nch; c++) sum += (float)row[c];
+            __syncthreads();
+            if ((int)threadIdx.x < nrow) {
+                const unsigned char *row = rows_ + threadIdx.x * PITCH;
+                for (int e = 0; e < cw; e++) sum += (float)*(const T *)(row + e * (int)sizeof(T));
+            }
         }
-        z[i] = sum;
+        if ((int)threadIdx.x < nrow) z[i0 + threadIdx.x] = sum;
+    }
+    /* the median: blocks of up to 8192 spectra by a bitonic sort of their
+     * order-preserving keys in LDS (sums of nearby values would pile the
+     * radix select's histogram atomics onto one bin) */
+    if (P <= NSORT) {
+        unsigned *keys = (unsigned *)smem;
+        int n2 = 1;
+        while (n2 < P) n2 <<= 1;
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < n2; i += 256)
+            keys[i] = i < P ? order_key(z[i]) : 0xFFFFFFFFu;
+        __syncthreads();
+        for (int kk = 2; kk <= n2; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int i = (int)threadIdx.x; i < n2; i += 256) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const unsigned a = keys[i], c = keys[ixj];
+                        if ((a > c) == ((i & kk) == 0)) {
+                            keys[i] = c;
+                            keys[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        if (threadIdx.x == 0) med[blk0 + b] = key_float(keys[(P - 1) / 2]);
+        return;
     }
     __syncthreads();
     unsigned prefix = 0, mask = 0, k = (unsigned)(P - 1) / 2;
