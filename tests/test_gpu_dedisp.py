@@ -31,6 +31,8 @@ CASES = [
     (30, 1100.0, 2.0, 64e-6, 256, 10, 5.0, 1.5, 70, 2, 64, True),
     # a downsampling factor that does not divide a wave (untiled subbands)
     (32, 900.0, 1.0, 64e-6, 192, 8, 0.0, 2.0, 9, 3, 48, False),
+    # raw blocks of 16384 spectra (clipping: the radix-select median)
+    (16, 1400.0, 8.0, 1e-4, 16384, 4, 0.0, 10.0, 5, 1, 8, True),
 ]
 
 
@@ -88,7 +90,7 @@ def test_series_match_oracle(oracle, c, kernels, chunk, monkeypatch):
 
 
 CLIPPED = [  # (case, passes of 2^16 spectra or one pass)
-    (CASES[0], 0), (CASES[1], 0), (CASES[3], 1 << 16), (CASES[5], 0)]
+    (CASES[0], 0), (CASES[1], 0), (CASES[3], 1 << 16), (CASES[5], 0), (CASES[6], 0)]
 
 
 @pytest.mark.parametrize("kernels", ["tiled", "untiled"])
