@@ -335,16 +335,33 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
         }
     }
     {
-        const int nv = ns * cps * (int)sizeof(T) / VEC;   /* loads per row */
-        for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
-            const int r = idx / nv, v = idx - r * nv;
-            const long long t = t0 + r;
-            W w{};
-            if (t < rows) {
-                const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
-                w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+        /* loads per row nv; the lanes take rpi rows per step, the division by
+         * nv done once (per load it would cost some 30 VALU instructions) */
+        const int nv = ns * cps * (int)sizeof(T) / VEC;
+        const int rpi = nv <= kRows ? kRows / nv : 0;
+        if (rpi) {
+            const int lr = (int)threadIdx.x / nv, v = (int)threadIdx.x - lr * nv;
+            if (lr < rpi)
+                for (int r = lr; r < nr; r += rpi) {
+                    const long long t = t0 + r;
+                    W w{};
+                    if (t < rows) {
+                        const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
+                        w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+                    }
+                    tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
+                }
+        } else {
+            for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
+                const int r = idx / nv, v = idx - r * nv;
+                const long long t = t0 + r;
+                W w{};
+                if (t < rows) {
+                    const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
+                    w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+                }
+                tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
             }
-            tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
         }
     }
     __syncthreads();
@@ -498,17 +515,35 @@ clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int
     const int b = (int)blockIdx.x;
     const T *blk = chunk + (long long)b * P * nch;
     float *z = zdm + (blk0 + b) * P;
-    (void)vec16;
     for (int i0 = 0; i0 < P; i0 += 256) {
         const int nrow = min(256, P - i0);
         float sum = 0.0f;
         for (int c0 = 0; c0 < nch; c0 += CW) {
             const int cw = min(CW, nch - c0);
             __syncthreads();
-            for (int idx = (int)threadIdx.x; idx < nrow * cw; idx += 256) {
-                const int r = idx / cw, e = idx - r * cw;
-                *(T *)(rows_ + r * PITCH + e * (int)sizeof(T)) =
-                    blk[(long long)(i0 + r) * nch + c0 + e];
+            if (vec16 && cw == CW) {
+                /* 16 bytes a lane: the address arithmetic of one load serves
+                 * 16 (float: 4) samples; four dword stores, as the padded
+                 * rows are 4-byte aligned */
+                constexpr int NV = CW * (int)sizeof(T) / 16, RPI = 256 / NV;
+                const int lr = (int)threadIdx.x / NV, lv = (int)threadIdx.x % NV;
+                for (int r = lr; r < nrow; r += RPI) {
+                    const uint4 w = *(const uint4 *)((const unsigned char *)(
+                                                         blk + (long long)(i0 + r) * nch + c0) +
+                                                     lv * 16);
+                    uint32_t *dst = (uint32_t *)(rows_ + r * PITCH + lv * 16);
+                    dst[0] = w.x;
+                    dst[1] = w.y;
+                    dst[2] = w.z;
+                    dst[3] = w.w;
+                }
+            } else {
+                for (int idx = (int)threadIdx.x; idx < nrow * CW; idx += 256) {
+                    const int r = idx / CW, e = idx % CW;
+                    if (e < cw)
+                        *(T *)(rows_ + r * PITCH + e * (int)sizeof(T)) =
+                            blk[(long long)(i0 + r) * nch + c0 + e];
+                }
             }
             __syncthreads();
             if ((int)threadIdx.x < nrow) {
