@@ -229,7 +229,10 @@ def test_resident_plans_in_the_mix():
                 assert _same(a, b)
         st = E.stats()
     say("mix with plans:", st)
-    assert st["plan_hits"] >= 8 and st["plans_resident"] <= 4
+    # how many requests reuse a plan depends on which resident plans are idle
+    # when each is admitted; at least the second round's first four (all of
+    # the first round's plans idle by then, two per geometry) do
+    assert st["plan_hits"] >= 4 and st["plans_resident"] <= 4
 
 
 def _submit(E, j):
