@@ -84,18 +84,16 @@ static int parse_i(const char *s, int *v)
     return 0;
 }
 
+/* The path without its extension: the extension is the basename's last '.'
+ * and what follows, unless that '.' starts the basename (Julia's splitext,
+ * the reference's output naming, src/cli.c:74-96). */
 static char *path_stem(const char *path)
 {
-    const size_t n = strlen(path);
-    size_t slash = 0, dot = n;
-    for (size_t i = 0; i < n; i++) if (path[i] == '/') slash = i + 1;
-    for (size_t i = slash; i < n; i++) if (path[i] == '.') dot = i;
-    if (dot == n || dot == slash) dot = n;
-    char *r = (char *)malloc(dot + 1);
-    if (!r) return NULL;
-    memcpy(r, path, dot);
-    r[dot] = '\0';
-    return r;
+    const char *base = strrchr(path, '/');
+    base = base ? base + 1 : path;
+    const char *dot = strrchr(base, '.');
+    const size_t keep = dot && dot > base ? (size_t)(dot - path) : strlen(path);
+    return strndup(path, keep);
 }
 
 static char *output_path(const char *in, const char *outfilenm, const char *outdir, int nfiles)
@@ -144,54 +142,64 @@ static int collisions(char **files, int n, const char *outdir)
     return bad;
 }
 
-static char **read_list(const char *listpath, int *nout)
+/* A .dat.list (src/cli.c:514-554 interface): one input per line, blank
+ * lines and '#' comments skipped, space, tab, CR and LF trimmed at both ends;
+ * a relative entry is taken relative to the list file's directory. */
+struct list_acc {
+    const char *listpath;
+    int dirlen;           /* length of the list's directory, -1 for none */
+    char **v;
+    int n, cap;
+};
+
+static int list_blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n'; }
+
+static int list_line(const char *s, size_t len, void *ctx)
 {
-    FILE *f = fopen(listpath, "r");
-    if (!f) {
-        fprintf(stderr, "Error: cannot open list file '%s'\n", listpath);
-        return NULL;
+    struct list_acc *L = (struct list_acc *)ctx;
+    while (len && list_blank(*s)) { s++; len--; }
+    while (len && list_blank(s[len - 1])) len--;
+    if (!len || *s == '#') return 0;
+    char *full = NULL;
+    if (*s == '/' || L->dirlen < 0) full = strndup(s, len);
+    else if (asprintf(&full, "%.*s/%.*s", L->dirlen, L->listpath, (int)len, s) < 0) full = NULL;
+    if (!full) return -1;
+    if (L->n == L->cap) {
+        const int cap = L->cap ? 2 * L->cap : 64;
+        char **g = (char **)realloc(L->v, (size_t)cap * sizeof(char *));
+        if (!g) { free(full); return -1; }
+        L->v = g;
+        L->cap = cap;
     }
-    char *dir = strdup(listpath);
-    char *sl = strrchr(dir, '/');
-    if (sl) *sl = '\0';
-    else dir[0] = '\0';
-    char **v = NULL, *line = NULL;
-    int n = 0, cap = 0;
-    size_t lcap = 0;
-    while (getline(&line, &lcap, f) >= 0) {
-        char *s = line;
-        while (*s == ' ' || *s == '\t' || *s == '\r' || *s == '\n') s++;
-        char *e = s + strlen(s);
-        while (e > s && (e[-1] == ' ' || e[-1] == '\t' || e[-1] == '\r' || e[-1] == '\n'))
-            *--e = '\0';
-        if (!*s || *s == '#') continue;
-        char *full;
-        if (s[0] == '/' || dir[0] == '\0') full = strdup(s);
-        else {
-            const size_t need = strlen(dir) + strlen(s) + 2;
-            full = (char *)malloc(need);
-            snprintf(full, need, "%s/%s", dir, s);
-        }
-        if (n == cap) {
-            cap = cap ? 2 * cap : 64;
-            v = (char **)realloc(v, (size_t)cap * sizeof(char *));
-        }
-        v[n++] = full;
-    }
-    free(line);
-    free(dir);
-    fclose(f);
-    *nout = n;
-    return v;
+    L->v[L->n++] = full;
+    return 0;
 }
 
+static char **read_list(const char *listpath, int *nout)
+{
+    const char *sl = strrchr(listpath, '/');
+    struct list_acc L = {listpath, sl && sl > listpath ? (int)(sl - listpath) : -1, NULL, 0, 0};
+    if (cs_for_each_line(listpath, list_line, &L) != 0) {
+        fprintf(stderr, "Error: cannot open list file '%s'\n", listpath);
+        for (int i = 0; i < L.n; i++) free(L.v[i]);
+        free(L.v);
+        return NULL;
+    }
+    *nout = L.n;
+    return L.v;
+}
+
+/* Every directory of path, outermost first (existing ones are left alone). */
 static void mkdir_p(const char *path)
 {
-    char *p = strdup(path);
-    for (char *q = p + 1; *q; q++)
-        if (*q == '/') { *q = '\0'; mkdir(p, 0777); *q = '/'; }
-    mkdir(p, 0777);
-    free(p);
+    if (!path || !*path) return;
+    for (const char *sep = strchr(path + 1, '/'); sep; sep = strchr(sep + 1, '/')) {
+        char *prefix = strndup(path, (size_t)(sep - path));
+        if (!prefix) return;
+        mkdir(prefix, 0777);
+        free(prefix);
+    }
+    mkdir(path, 0777);
 }
 
 

@@ -1,8 +1,21 @@
 /*
- * PRESTO .inf / .fft readers (src/fileio.c:38-164 semantics: the value after
- * the last '=', keys matched at line start including the leading space; the
- * .fft is mapped MAP_PRIVATE so a zaplist could edit it in memory; namps =
- * bytes / 8; T = N * dt) and small writers for synthetic inputs.
+ * PRESTO .inf / .fft input (the reference's FFTFile interface,
+ * include/coherent_search.h:48-81) and small writers for synthetic inputs.
+ *
+ * Interface kept from src/fileio.c:38-154 (it is what a caller observes):
+ *   - an .inf field is recognised when the raw line starts with its key,
+ *     leading space included; its value is the text after the line's last
+ *     '=' (the whole line without one), surrounding white space removed; a
+ *     later line of the same key wins;
+ *   - name.fft pairs with the name whose last four characters are ".inf";
+ *   - the .fft is mapped copy-on-write (MAP_PRIVATE), so a zaplist edits the
+ *     spectrum in memory only; namps = bytes / 8, T = N dt, df = 1/T, DC and
+ *     Nyquist are the first two floats, "_red.fft" marks a dereddened file;
+ *   - the error messages and their order of checks.
+ *
+ * The reader itself is this file's own: the .inf is read whole into memory
+ * and scanned once against a table of fields, and the mapping of the
+ * spectrum is a helper of its own.
  */
 #define _GNU_SOURCE
 #include "host_internal.h"
@@ -10,57 +23,140 @@
 #include <ctype.h>
 #include <errno.h>
 #include <fcntl.h>
+#include <stddef.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
-static char *dupstr(const char *s)
+static char *copy_span(const char *s, size_t n)
 {
-    if (!s) return NULL;
-    const size_t n = strlen(s) + 1;
-    char *r = (char *)malloc(n);
-    if (r) memcpy(r, s, n);
+    char *r = (char *)malloc(n + 1);
+    if (!r) return NULL;
+    memcpy(r, s, n);
+    r[n] = '\0';
     return r;
 }
 
-static char *value_of(char *line)
+/* the whole file as a NUL-terminated buffer (its length in *len) */
+static char *slurp(const char *path, size_t *len)
 {
-    char *eq = strrchr(line, '=');
-    char *s = eq ? eq + 1 : line;
-    while (*s && isspace((unsigned char)*s)) s++;
-    char *e = s + strlen(s);
-    while (e > s && isspace((unsigned char)e[-1])) *--e = '\0';
-    return s;
+    FILE *f = fopen(path, "rb");
+    if (!f) return NULL;
+    size_t cap = 4096, n = 0;
+    char *buf = (char *)malloc(cap);
+    while (buf) {
+        n += fread(buf + n, 1, cap - n - 1, f);
+        if (n < cap - 1) break;
+        char *g = (char *)realloc(buf, cap * 2);
+        if (!g) {
+            free(buf);
+            buf = NULL;
+            break;
+        }
+        buf = g;
+        cap *= 2;
+    }
+    const int bad = ferror(f);
+    fclose(f);
+    if (!buf || bad) {
+        free(buf);
+        return NULL;
+    }
+    buf[n] = '\0';
+    *len = n;
+    return buf;
+}
+
+int cs_for_each_line(const char *path, int (*fn)(const char *, size_t, void *), void *ctx)
+{
+    size_t n = 0;
+    char *text = slurp(path, &n);
+    if (!text) return -1;
+    int rc = 0;
+    for (const char *s = text, *end = text + n; s < end && rc == 0;) {
+        const char *nl = memchr(s, '\n', (size_t)(end - s));
+        const char *e = nl ? nl : end;
+        rc = fn(s, (size_t)(e - s), ctx) != 0 ? -1 : 0;
+        s = nl ? nl + 1 : end;
+    }
+    free(text);
+    return rc;
+}
+
+enum inf_kind { INF_TEXT, INF_REAL, INF_COUNT };
+
+/* The fields the search reads, by the key their line starts with. */
+static const struct {
+    const char *key;
+    enum inf_kind kind;
+    size_t value;         /* offset of the value in SimpleInf */
+    ptrdiff_t seen;       /* offset of its has_ flag, -1 for none */
+} inf_fields[] = {
+    {" Object being observed", INF_TEXT, offsetof(SimpleInf, object), -1},
+    {" Epoch", INF_REAL, offsetof(SimpleInf, epoch), offsetof(SimpleInf, has_epoch)},
+    {" Number of bins", INF_COUNT, offsetof(SimpleInf, N), offsetof(SimpleInf, has_N)},
+    {" Width of each time series bin", INF_REAL, offsetof(SimpleInf, dt),
+     offsetof(SimpleInf, has_dt)},
+    {" Dispersion measure", INF_REAL, offsetof(SimpleInf, DM), offsetof(SimpleInf, has_DM)},
+};
+
+/* one line [s, e): its value span, after the last '=' and trimmed */
+static void line_value(const char *s, const char *e, const char **vs, const char **ve)
+{
+    const char *v = s;
+    for (const char *c = s; c < e; c++)
+        if (*c == '=') v = c + 1;
+    while (v < e && isspace((unsigned char)*v)) v++;
+    while (e > v && isspace((unsigned char)e[-1])) e--;
+    *vs = v;
+    *ve = e;
+}
+
+static void inf_take(SimpleInf *inf, const char *s, const char *e)
+{
+    const size_t len = (size_t)(e - s);
+    for (size_t k = 0; k < sizeof(inf_fields) / sizeof(inf_fields[0]); k++) {
+        const size_t kl = strlen(inf_fields[k].key);
+        if (len < kl || memcmp(s, inf_fields[k].key, kl) != 0) continue;
+        const char *vs, *ve;
+        line_value(s, e, &vs, &ve);
+        char *val = copy_span(vs, (size_t)(ve - vs));
+        if (!val) return;
+        char *field = (char *)inf + inf_fields[k].value;
+        switch (inf_fields[k].kind) {
+        case INF_TEXT:
+            free(*(char **)field);
+            *(char **)field = val;
+            val = NULL;
+            break;
+        case INF_REAL:
+            *(double *)field = strtod(val, NULL);
+            break;
+        case INF_COUNT:
+            *(long long *)field = strtoll(val, NULL, 10);
+            break;
+        }
+        free(val);
+        if (inf_fields[k].seen >= 0) *(int *)((char *)inf + inf_fields[k].seen) = 1;
+        return;    /* the first key that matches owns the line */
+    }
 }
 
 int inf_read(const char *path, SimpleInf *inf)
 {
     memset(inf, 0, sizeof(*inf));
-    inf->path = dupstr(path);
-    FILE *f = fopen(path, "r");
-    if (!f) return -1;
-    char *line = NULL;
-    size_t cap = 0;
-    while (getline(&line, &cap, f) >= 0) {
-        if (strncmp(line, " Object being observed", 22) == 0) {
-            free(inf->object);
-            inf->object = dupstr(value_of(line));
-        } else if (strncmp(line, " Epoch", 6) == 0) {
-            inf->epoch = strtod(value_of(line), NULL);
-            inf->has_epoch = 1;
-        } else if (strncmp(line, " Number of bins", 15) == 0) {
-            inf->N = strtoll(value_of(line), NULL, 10);
-            inf->has_N = 1;
-        } else if (strncmp(line, " Width of each time series bin", 30) == 0) {
-            inf->dt = strtod(value_of(line), NULL);
-            inf->has_dt = 1;
-        } else if (strncmp(line, " Dispersion measure", 19) == 0) {
-            inf->DM = strtod(value_of(line), NULL);
-            inf->has_DM = 1;
-        }
+    inf->path = path ? copy_span(path, strlen(path)) : NULL;
+    size_t n = 0;
+    char *text = path ? slurp(path, &n) : NULL;
+    if (!text) return -1;
+    const char *end = text + n;
+    for (const char *s = text; s < end;) {
+        const char *nl = memchr(s, '\n', (size_t)(end - s));
+        const char *e = nl ? nl + 1 : end;     /* the line keeps its newline, as read */
+        inf_take(inf, s, e);
+        s = e;
     }
-    free(line);
-    fclose(f);
+    free(text);
     return 0;
 }
 
@@ -72,69 +168,65 @@ void inf_free(SimpleInf *inf)
     memset(inf, 0, sizeof(*inf));
 }
 
-/* name.fft -> name.inf (the last four characters replaced) */
-static char *inf_path_for(const char *path)
+/* Map a file copy-on-write.  0 with *map / *bytes set (NULL, 0 for an empty
+ * file); -1 after printing the failing call's message. */
+static int map_private(const char *path, void **map, size_t *bytes)
 {
-    const size_t plen = strlen(path);
-    const size_t keep = plen > 4 ? plen - 4 : 0;
-    char *p = (char *)malloc(keep + 5);
-    if (!p) return NULL;
-    memcpy(p, path, keep);
-    memcpy(p + keep, ".inf", 5);
-    return p;
-}
-
-int fftfile_open(FFTFile *ft, const char *path)
-{
-    memset(ft, 0, sizeof(*ft));
-    ft->path = dupstr(path);
-    char *ip = inf_path_for(path);
-    if (!ip) return -1;
-    if (inf_read(ip, &ft->inf) != 0) {
-        fprintf(stderr, "Error: The .inf file '%s' was not found.\n", ip);
-        free(ip);
-        return -1;
-    }
-    free(ip);
-    if (!ft->inf.has_N) {
-        fprintf(stderr, "Error: Missing 'Number of bins' in %s\n", ft->inf.path);
-        return -1;
-    }
-    if (!ft->inf.has_dt) {
-        fprintf(stderr, "Error: Missing 'Width of each time series bin' in %s\n",
-                ft->inf.path);
-        return -1;
-    }
+    *map = NULL;
+    *bytes = 0;
     const int fd = open(path, O_RDONLY);
     if (fd < 0) {
         fprintf(stderr, "Error: cannot open '%s': %s\n", path, strerror(errno));
         return -1;
     }
-    struct stat st;
-    if (fstat(fd, &st) != 0) {
+    struct stat sb;
+    int rc = 0;
+    if (fstat(fd, &sb) != 0) {
         fprintf(stderr, "Error: cannot stat '%s': %s\n", path, strerror(errno));
-        close(fd);
-        return -1;
-    }
-    const size_t bytes = (size_t)st.st_size;
-    ft->mapsize = bytes;
-    if (bytes > 0) {
-        void *m = mmap(NULL, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE, fd, 0);
+        rc = -1;
+    } else if (sb.st_size > 0) {
+        void *m = mmap(NULL, (size_t)sb.st_size, PROT_READ | PROT_WRITE, MAP_PRIVATE, fd, 0);
         if (m == MAP_FAILED) {
             fprintf(stderr, "Error: cannot mmap '%s': %s\n", path, strerror(errno));
-            close(fd);
-            return -1;
+            rc = -1;
+        } else {
+            *map = m;
         }
-        ft->map = m;
-        ft->amps = (float *)m;
-        ft->namps = (long long)(bytes / sizeof(float) / 2);
     }
+    if (rc == 0) *bytes = (size_t)sb.st_size;
     close(fd);
+    return rc;
+}
+
+int fftfile_open(FFTFile *ft, const char *path)
+{
+    memset(ft, 0, sizeof(*ft));
+    ft->path = copy_span(path, strlen(path));
+    /* the companion .inf: the path's last four characters become ".inf" */
+    const int stem = (int)(strlen(path) > 4 ? strlen(path) - 4 : 0);
+    char *infpath = NULL;
+    if (asprintf(&infpath, "%.*s.inf", stem, path) < 0) return -1;
+    const int found = inf_read(infpath, &ft->inf) == 0;
+    if (!found) fprintf(stderr, "Error: The .inf file '%s' was not found.\n", infpath);
+    free(infpath);
+    if (!found) return -1;
+    const char *missing = !ft->inf.has_N ? "Number of bins"
+                          : !ft->inf.has_dt ? "Width of each time series bin" : NULL;
+    if (missing) {
+        fprintf(stderr, "Error: Missing '%s' in %s\n", missing, ft->inf.path);
+        return -1;
+    }
+    void *map;
+    size_t bytes;
+    if (map_private(path, &map, &bytes) != 0) return -1;
+    ft->map = map;
+    ft->mapsize = bytes;
+    ft->amps = (float *)map;
+    ft->namps = (long long)(bytes / (2 * sizeof(float)));
     ft->N = ft->inf.N;
-    ft->T = (double)ft->N * ft->inf.dt;
+    ft->T = ft->inf.dt * (double)ft->N;
     ft->df = 1.0 / ft->T;
-    ft->dereddened = strstr(path, "_red.fft") != NULL;
-    ft->detrended = ft->dereddened;
+    ft->dereddened = ft->detrended = strstr(path, "_red.fft") != NULL;
     if (ft->namps > 0) {
         ft->DC = ft->amps[0];
         ft->Nyquist = ft->amps[1];

@@ -20,4 +20,9 @@ int cs_fft_any(double *z, long long n, int sign);
  * OpenMP threads (0: the OpenMP default). */
 int cs_fft_work(double *z, double *scratch, long long n, int sign, int nthreads);
 
+/* Calls fn(line, len, ctx) for every line of a text file (the newline not
+ * included; a last line without one counts).  0 when every call returned 0,
+ * -1 if the file cannot be read or a call failed. */
+int cs_for_each_line(const char *path, int (*fn)(const char *, size_t, void *), void *ctx);
+
 #endif

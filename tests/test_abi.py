@@ -219,3 +219,45 @@ def test_zaplist_read(tmp_path):
     assert z.tuples() == [(60.0, 0.5, 0), (100.5, 0.25, 1), (0.5, 0.1, 1), (120.25, 0.01, 0)]
     with pytest.raises(OSError):
         cs.Zaplist.read(tmp_path / "missing")
+
+
+def test_inf_fields_and_errors(tmp_path, capfd):
+    """.inf rules of the reference's reader (src/fileio.c:38-72): a key matches
+    at the start of the raw line (leading space included), the value is what
+    follows the last '=' trimmed, a later line wins, a line without '=' gives
+    the whole trimmed line, other keys and unindented keys are ignored; a
+    missing 'Number of bins' or 'Width' fails fftfile_open with its message."""
+    L = abi.host()
+    L.inf_read.argtypes = [C.c_char_p, C.POINTER(abi.SimpleInf)]
+    L.inf_read.restype = C.c_int
+    L.inf_free.argtypes = [C.POINTER(abi.SimpleInf)]
+    p = tmp_path / "x.inf"
+    p.write_bytes(b" Object being observed   =  J0000+00 = B1937 \r\n"
+                  b" Number of bins in the time series = 1000\n"
+                  b"Number of bins in the time series = 7\n"
+                  b" Number of bins in the time series = 4096\n"
+                  b" Epoch of observation (MJD) = 59000.5\n"
+                  b" Width of each time series bin (sec)    =   6.4e-05   \n"
+                  b" Dispersion measure  12.5")
+    inf = abi.SimpleInf()
+    assert L.inf_read(str(p).encode(), C.byref(inf)) == 0
+    assert inf.object == b"B1937" and inf.N == 4096 and inf.has_N == 1
+    assert inf.dt == 6.4e-05 and inf.has_dt == 1 and inf.epoch == 59000.5
+    assert inf.has_DM == 1 and inf.DM == 0.0     # no '=': strtod of the whole line
+    L.inf_free(C.byref(inf))
+    assert L.inf_read(str(tmp_path / "none.inf").encode(), C.byref(inf)) == -1
+    L.inf_free(C.byref(inf))
+    for text, msg in ((b" Width of each time series bin = 1e-4\n", "Missing 'Number of bins'"),
+                      (b" Number of bins = 8\n", "Missing 'Width of each time series bin'")):
+        (tmp_path / "y.inf").write_bytes(text)
+        (tmp_path / "y.fft").write_bytes(np.zeros(8, np.float32).tobytes())
+        ft = abi.FFTFile()
+        capfd.readouterr()
+        assert L.fftfile_open(C.byref(ft), str(tmp_path / "y.fft").encode()) == -1
+        assert msg in capfd.readouterr().err
+        L.fftfile_close(C.byref(ft))
+    # a .fft path shorter than its suffix: the .inf is ".inf" itself
+    ft = abi.FFTFile()
+    assert L.fftfile_open(C.byref(ft), b"abc") == -1
+    assert "The .inf file '.inf' was not found." in capfd.readouterr().err
+    L.fftfile_close(C.byref(ft))
