@@ -718,17 +718,22 @@ extern "C" CS_API int cs_gpu_session_matches(const GpuSession *S, const FFTFile 
 
 __global__ void cs_zero_u32(unsigned int *p) { p[0] = 0u; }
 
-static int enqueue_search(GpuSession *S, double threshold, float2 *stack_out)
+/* The launch sequence of one search into a record buffer of `cap` records
+ * and its counter: the counter cleared by a kernel (also inside a captured
+ * graph), the measured-sigma passes, the search kernel. */
+static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_out,
+                               CandRec *recs, unsigned int *counter, unsigned int cap)
 {
     S->args.stack_out = stack_out;
     S->args.threshold = threshold;
-    S->args.recs = S->d_recs;
-    S->args.cap = S->cap;
+    S->args.recs = recs;
+    S->args.counter = counter;
+    S->args.cap = cap;
     const int ntiles = S->args.ntiles;
     const int unit = 8 * std::max(S->args.xcd_stripe, 1);
     const int nblk = ((ntiles + unit - 1) / unit) * unit;
     fast_launch_fn fn = S->f64 ? S->fast->fn64 : S->fast->fn;
-    hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, S->d_counter);
+    hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, counter);
     if (hipGetLastError() != hipSuccess) return -1;
     if (S->measured && ntiles > 0) {
         /* sample pass, then sigma and the per-width factors of every chunk */
@@ -757,6 +762,11 @@ static int enqueue_search(GpuSession *S, double threshold, float2 *stack_out)
     }
     if (ntiles > 0 && fn(&S->args, nblk, S->st) != hipSuccess) return -1;
     return 0;
+}
+
+static int enqueue_search(GpuSession *S, double threshold, float2 *stack_out)
+{
+    return enqueue_search_into(S, threshold, stack_out, S->d_recs, S->d_counter, S->cap);
 }
 
 /* One search: the counter reset, the measured-sigma passes and the search
@@ -859,34 +869,26 @@ extern "C" CS_API CandidateList cs_gpu_session_run(GpuSession *S, const FFTFile 
     return cs_gpu_session_search_resident(S, threshold);
 }
 
-/* The records of a pass, ordered by key (chunk, slot, trial: the reference's
- * harvest order, src/cuda.cu:833-881), as candidates above threshold. */
-static CandidateList harvest(GpuSession *S, unsigned int count, double threshold)
+/* Candidates of `count` records sorted by key (chunk, slot, trial: the
+ * reference's harvest order, src/cuda.cu:833-881), above threshold and, for
+ * decimations, below the Nyquist frequency, written to out[0..].  Record i is
+ * read before candidate i is written, so out may overlap the records when
+ * they sit at the end of out's count candidate slots.  Returns the number
+ * written, -1 for a record no search could have produced. */
+static long long records_to_candidates(const GpuSession *S, const CandRec *recs,
+                                       unsigned int count, double threshold, Candidate *out)
 {
-    CandidateList out;
-    out.v = NULL; out.n = 0; out.cap = 0;
-    if (count == 0) return out;
-    std::vector<CandRec> recs(count);
-    if (hipMemcpyAsync(recs.data(), S->d_recs, (size_t)count * sizeof(CandRec),
-                       hipMemcpyDeviceToHost, S->st) != hipSuccess ||
-        hipStreamSynchronize(S->st) != hipSuccess)
-        return out;
-    std::sort(recs.begin(), recs.end(),
-              [](const CandRec &a, const CandRec &b) { return a.key < b.key; });
-    out.v = (Candidate *)malloc((size_t)count * sizeof(Candidate));
-    if (!out.v) return out;
-    out.cap = count;
-    for (const CandRec &r : recs) {
+    long long n = 0;
+    for (unsigned int i = 0; i < count; i++) {
+        CandRec r;
+        memcpy(&r, recs + i, sizeof(r));
         const long long c = (long long)(r.key >> 40);
         const int slot = (int)((r.key >> 32) & 0xff);
         const long long j = (long long)(r.key & 0xffffffffull);
         if (c >= S->nchunks || slot >= S->nslot || j >= (long long)S->chunks[(size_t)c].n) {
             fprintf(stderr, "Error: GPU search returned an invalid record (chunk %lld, "
                             "slot %d, trial %lld)\n", c, slot, j);
-            free(out.v);
-            out.v = NULL;
-            out.n = out.cap = 0;
-            return out;
+            return -1;
         }
         const double metric = (double)r.metric;
         if (!(metric > threshold)) continue;
@@ -902,16 +904,45 @@ static CandidateList harvest(GpuSession *S, unsigned int count, double threshold
             if (!(rr < S->nyq)) continue;
         }
         Candidate cd;
+        memset(&cd, 0, sizeof(cd));
         cd.freq = rr / S->T;
         cd.metric = metric;
         cd.r = rr;
         cd.nharm = nharm;
         cd.ducy = NAN;
-        out.v[out.n++] = cd;
+        memcpy(out + n, &cd, sizeof(cd));
+        n++;
     }
-    return out;
+    return n;
 }
 
+static bool key_less(const CandRec &a, const CandRec &b) { return a.key < b.key; }
+
+/* The records of a pass, ordered by key, as candidates above threshold. */
+static CandidateList harvest(GpuSession *S, unsigned int count, double threshold)
+{
+    CandidateList out;
+    out.v = NULL; out.n = 0; out.cap = 0;
+    if (count == 0) return out;
+    std::vector<CandRec> recs(count);
+    if (hipMemcpyAsync(recs.data(), S->d_recs, (size_t)count * sizeof(CandRec),
+                       hipMemcpyDeviceToHost, S->st) != hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess)
+        return out;
+    std::sort(recs.begin(), recs.end(), key_less);
+    out.v = (Candidate *)malloc((size_t)count * sizeof(Candidate));
+    if (!out.v) return out;
+    out.cap = count;
+    const long long n = records_to_candidates(S, recs.data(), count, threshold, out.v);
+    if (n < 0) {
+        free(out.v);
+        out.v = NULL;
+        out.cap = 0;
+        return out;
+    }
+    out.n = (size_t)n;
+    return out;
+}
 
 extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
                                                                double threshold)
@@ -936,6 +967,13 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
         hipEventElapsedTime(&ms, S->ev0, S->ev1);
         S->last_ms = ms;
         if (count <= S->cap) break;
+        if (attempt == 1) {
+            /* the grown buffer was searched and still overflowed: the count
+             * changed between two searches of the same spectrum */
+            fprintf(stderr, "Error: GPU search found %u records after a search that found "
+                            "fewer; not harvesting a partly written buffer\n", count);
+            return out;
+        }
         /* never truncate: grow to the exact count and search again */
         if (S->recs_own) dfree(S, S->d_recs);
         S->d_recs = NULL;
@@ -1068,8 +1106,11 @@ extern "C" CS_API int cs_gpu_session_work(const GpuSession *S, long long *ht,
 namespace {
 
 constexpr int kDucyM = 64;        /* exact interpolation taps */
-constexpr int kDucyHmax = 64;     /* harmonics per candidate */
+constexpr int kDucyHmax = 1024;   /* harmonics per candidate */
 constexpr int kDucyWcap = 256;    /* boxcar widths per profile */
+constexpr int kDucyThreads = 256;
+/* prefix sums over n + wmax bins, wmax <= 0.3 n (boxcar_widths' maxfrac cap) */
+constexpr int kDucyPs = 2 * kDucyHmax + (2 * kDucyHmax * 3) / 10 + 2;
 
 struct DucyCand {
     double r;
@@ -1079,14 +1120,15 @@ struct DucyCand {
     int woff;                     /* their offset in the width/shape tables */
 };
 
-/* One wave per candidate.  Lane h-1 interpolates harmonic h exactly (m = 64
- * taps of sinc(off) e^{-i pi off}, double); lanes over bins form the profile by
- * a direct real inverse transform of the H harmonics (harmonic H taken as the
- * real Nyquist term) scaled by 1/2H; lane 0 takes the prefix sums of the
- * mean-removed profile over n + wmax wrapped bins; lanes over widths take each
- * width's best window, (max - duty*total) * shape_w; lane 0 keeps the first
- * width with the largest value. */
-__global__ __launch_bounds__(64) void ducy_kernel(const float2 *__restrict__ amps,
+/* One workgroup per candidate.  Threads stride over the harmonics, each
+ * interpolating harmonic h exactly (m = 64 taps of sinc(off) e^{-i pi off},
+ * double); over the bins, each forming the profile by a direct real inverse
+ * transform of the H harmonics (harmonic H taken as the real Nyquist term)
+ * scaled by 1/2H; thread 0 takes the prefix sums of the mean-removed profile
+ * over n + wmax wrapped bins; threads over widths take each width's best
+ * window, (max - duty*total) * shape_w; thread 0 keeps the first width with
+ * the largest value. */
+__global__ __launch_bounds__(kDucyThreads) void ducy_kernel(const float2 *__restrict__ amps,
                                                   long long namps,
                                                   const DucyCand *__restrict__ cands,
                                                   const int *__restrict__ widths,
@@ -1095,13 +1137,12 @@ __global__ __launch_bounds__(64) void ducy_kernel(const float2 *__restrict__ amp
 {
     __shared__ double Xr[kDucyHmax + 1], Xi[kDucyHmax + 1];
     __shared__ double prof[2 * kDucyHmax];
-    __shared__ double ps[4 * kDucyHmax + 1];
+    __shared__ double ps[kDucyPs];
     __shared__ double cval[kDucyWcap];
     const DucyCand c = cands[blockIdx.x];
     const int lane = threadIdx.x;
     const int H = c.H, n = c.n;
-    if (lane < H) {
-        const int h = lane + 1;
+    for (int h = lane + 1; h <= H; h += kDucyThreads) {
         const double rh = c.r * (double)h;
         double dr = fmod(rh, 1.0);
         if (dr < 0) dr += 1.0;
@@ -1126,7 +1167,7 @@ __global__ __launch_bounds__(64) void ducy_kernel(const float2 *__restrict__ amp
     }
     __syncthreads();
     const double inv = 1.0 / (double)n;
-    for (int i = lane; i < n; i += 64) {
+    for (int i = lane; i < n; i += kDucyThreads) {
         double acc = 0.0;
         for (int h = 1; h < H; h++) {
             double sn, cs;
@@ -1150,7 +1191,7 @@ __global__ __launch_bounds__(64) void ducy_kernel(const float2 *__restrict__ amp
     }
     __syncthreads();
     const double stot = ps[n] - ps[0];
-    for (int k = lane; k < c.nw; k += 64) {
+    for (int k = lane; k < c.nw; k += kDucyThreads) {
         const int w = widths[c.woff + k];
         const double duty = (double)w / (double)n;
         double mx = ps[w] - ps[0];
@@ -1238,6 +1279,13 @@ cs_session_finalize(GpuSession *S, const CandidateList *in, int ncands)
             c->ducy = (double)wt[(size_t)woff_of_n[n]] / (double)n;
             continue;
         }
+        if (n + wt[(size_t)(woff_of_n[n] + nw_of_n[n] - 1)] + 1 > kDucyPs) {
+            fprintf(stderr, "Error: duty cycle of a %d-bin profile with widths up to %d "
+                            "exceeds the device pass (%d prefix sums)\n", n,
+                    wt[(size_t)(woff_of_n[n] + nw_of_n[n] - 1)], kDucyPs);
+            csx_candlist_free(&out);
+            return out;
+        }
         DucyCand d;
         d.r = c->r;
         d.H = H;
@@ -1265,7 +1313,7 @@ cs_session_finalize(GpuSession *S, const CandidateList *in, int ncands)
     int ok = 0;
     if (dalloc(S, (void **)&d, total) == hipSuccess) {
         if (hipMemcpyAsync(d, h_in.data(), o_o, hipMemcpyHostToDevice, S->st) == hipSuccess) {
-            hipLaunchKernelGGL(ducy_kernel, dim3((unsigned)dc.size()), dim3(64), 0, S->st,
+            hipLaunchKernelGGL(ducy_kernel, dim3((unsigned)dc.size()), dim3(kDucyThreads), 0, S->st,
                                (const float2 *)S->d_amps, S->namps, (const DucyCand *)d,
                                (const int *)(d + o_w), (const double *)(d + o_s),
                                (int *)(d + o_o));
@@ -1831,6 +1879,11 @@ extern "C" CS_API CandidateList cs_gpu_session_metric_candidates(GpuSession *S, 
             goto done;
         }
         if (count <= S->cap) break;
+        if (attempt == 1) {
+            fprintf(stderr, "Error: GPU metric normalisation found %u records after a pass "
+                            "that found fewer; not harvesting a partly written buffer\n", count);
+            goto done;
+        }
         if (S->recs_own) dfree(S, S->d_recs);
         S->d_recs = NULL;
         S->recs_own = true;

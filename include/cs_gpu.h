@@ -110,7 +110,7 @@ CS_API int cs_gpu_session_zap(GpuSession *s, const ZapList *z, double baryv, flo
  * measure_ducy of src/candidate.c:305-329: exact 64-tap interpolation of each
  * harmonic that fits, a 2H-bin profile, boxcar_best_width).  The spectrum does
  * not come back to the host.  cands is left as it is; the result is a new list
- * (empty, with a message on stderr, on failure or for nharm > 64). */
+ * (empty, with a message on stderr, on failure or for nharm > 1024). */
 CS_API CandidateList cs_gpu_session_finalize(GpuSession *s, const CandidateList *cands,
                                              int ncands);
 

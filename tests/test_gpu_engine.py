@@ -129,7 +129,9 @@ def _final_vs_host(spec, p, final, raw, ncands):
 @pytest.mark.parametrize("nh,r0,nbins,hifreq", [(60, 271.3, 1 << 15, 125.0),
                                                  (16, 3333.7, 1 << 14, 2000.0),
                                                  (64, 1021.9, 1 << 15, 300.0),
-                                                 (8, 16340.2, 1 << 14, 7800.0)])
+                                                 (8, 16340.2, 1 << 14, 7800.0),
+                                                 (120, 211.7, 1 << 15, 150.0),
+                                                 (100, 431.3, 1 << 15, 150.0)])
 def test_session_finalize_duty_cycles(oracle, nh, r0, nbins, hifreq):
     """cs_gpu_session_finalize: the S/N order and cut of the host's finalize
     and its duty cycles, measured on the device from the resident spectrum.
@@ -169,6 +171,30 @@ def test_engine_final_lists_match_host_post():
             assert info_r["status"] == 0 and info_f["status"] == 0 and none is None
             say("final list", final.size, "post_ms %.3f" % info_f["post_ms"])
             _final_vs_host(spec, j["params"], final, raw, 50)
+
+
+def test_engine_final_lists_wide_harmonics():
+    """Requests of more than 64 harmonics that ask for the final list: duty
+    cycles measured on the device over every fitting harmonic, equal to the
+    host's collapse + finalize of the request's raw list."""
+    jobs = []
+    for i, nh in enumerate((120, 96)):
+        x = synth.series_with_pulsar(1 << 19, 9.1e-3, 0.05, 15.0, 70 + i)
+        jobs.append(dict(samples=x, dt=synth.DT, threshold=4.5, blocksize=16384,
+                         params=cs.make_params(nharms=nh, m=16, maxdecim=6)))
+    with eng.Engine(nslots=2) as E:
+        for j in jobs:
+            raw, info_r, spec = E.wait(E.submit(j["samples"], j["dt"], j["params"],
+                                                threshold=j["threshold"],
+                                                blocksize=j["blocksize"]))
+            final, info_f, _ = E.wait(E.submit(j["samples"], j["dt"], j["params"],
+                                               threshold=j["threshold"],
+                                               blocksize=j["blocksize"], ncands=5000))
+            assert info_r["status"] == 0 and info_f["status"] == 0
+            say("nh", j["params"].nharms, "final list", final.size,
+                "nharm", final["nharm"][:6], "ducy", final["ducy"][:6])
+            assert final.size > 0 and np.any(final["nharm"] > 64)
+            _final_vs_host(spec, j["params"], final, raw, 5000)
 
 
 def _dm_trials(n, N=1 << 19, seed=31):
@@ -217,22 +243,29 @@ def test_resident_plans_reused_with_graphs(oracle):
 
 def test_resident_plans_in_the_mix():
     """Eight requests of two geometries on four slots: several plans of one
-    geometry may exist at once; every list equals the one-slot, no-cache run."""
+    geometry may exist at once; every list equals the one-slot, no-cache run.
+    Then the same requests one at a time: after the first request of each
+    geometry an idle plan of it is always resident (the cache holds four, and
+    only a miss creates a plan or evicts one), so at least the last six reuse
+    a plan, whatever the timing of the concurrent round."""
     jobs = _dm_trials(8, seed=41)
     with eng.Engine(nslots=1, plan_cache=0) as E:
         alone = [E.wait(_submit(E, j))[0] for j in jobs]
     with eng.Engine(nslots=4, plan_cache=4) as E:
-        for _ in range(2):
-            tk = [_submit(E, j) for j in jobs]
-            mix = [E.wait(t)[0] for t in tk]
-            for a, b in zip(alone, mix):
-                assert _same(a, b)
+        tk = [_submit(E, j) for j in jobs]
+        mix = [E.wait(t)[0] for t in tk]
+        for a, b in zip(alone, mix):
+            assert _same(a, b)
+        st1 = E.stats()
+        serial = []
+        for j in jobs:
+            serial.append(E.wait(_submit(E, j)))
         st = E.stats()
-    say("mix with plans:", st)
-    # how many requests reuse a plan depends on which resident plans are idle
-    # when each is admitted; at least the second round's first four (all of
-    # the first round's plans idle by then, two per geometry) do
-    assert st["plan_hits"] >= 4 and st["plans_resident"] <= 4
+    say("mix with plans:", st1, "then serial:", st, "reused", [o[1]["reused"] for o in serial])
+    for a, (b, info, _) in zip(alone, serial):
+        assert _same(a, b)
+    assert all(o[1]["reused"] == 1 for o in serial[2:])
+    assert st["plan_hits"] - st1["plan_hits"] >= 6 and st["plans_resident"] <= 4
 
 
 def _submit(E, j):
