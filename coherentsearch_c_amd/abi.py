@@ -178,6 +178,7 @@ HIP_SYMBOLS = ["cs_gpu_available", "cs_gpu_session_create", "cs_gpu_session_matc
                "cs_gpu_session_run", "cs_gpu_session_free", "cs_gpu_search_region",
                "cs_gpu_session_dump_stack", "cs_gpu_session_last_kernel_ms",
                "cs_gpu_session_upload", "cs_gpu_session_search_resident",
+               "cs_gpu_session_search_noalloc",
                "cs_gpu_session_work", "cs_gpu_stage_create", "cs_gpu_stage_free",
                "cs_gpu_stage_run_host", "cs_gpu_session_stage", "cs_gpu_session_set_graphs",
                "cs_gpu_session_finalize", "cs_gpu_session_zap", "cs_gpu_autotune",
@@ -332,6 +333,8 @@ def _declare(L):
     L.cs_gpu_session_upload.restype = I
     L.cs_gpu_session_search_resident.argtypes = [P, D]
     L.cs_gpu_session_search_resident.restype = CandidateList
+    L.cs_gpu_session_search_noalloc.argtypes = [P, D, P, LL]
+    L.cs_gpu_session_search_noalloc.restype = LL
     L.cs_gpu_session_last_kernel_ms.argtypes = [P]
     L.cs_gpu_session_last_kernel_ms.restype = D
     L.cs_gpu_session_work.argtypes = [P, Ps(LL), Ps(LL), Ps(D)]

@@ -269,6 +269,8 @@ static double2 *shared_twiddles64(int device)
     return tab[device];
 }
 
+static unsigned char *scratch_on(int device);
+
 static hipError_t dalloc(GpuSession *S, void **p, size_t bytes)
 {
     return S->async_alloc ? hipMallocAsync(p, bytes, S->st) : hipMalloc(p, bytes);
@@ -600,6 +602,7 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     }
 
     HIPCHK(hipSetDevice(device));
+    (void)scratch_on(device);     /* set-up of the no-allocation search's scratch */
     if (ext) {
         S->st = ext;
         S->async_alloc = true;
@@ -986,6 +989,92 @@ extern "C" CS_API CandidateList cs_gpu_session_search_resident(GpuSession *S,
         }
     }
     return harvest(S, count, threshold);
+}
+
+/* ---- a search that allocates nothing -------------------------------------
+ * The record buffer and its counter of cs_gpu_session_search_noalloc live in
+ * one file-scope device array per GPU, held until the process ends:
+ * CS_NOALLOC_MAXREC records of 16 bytes after a 256-byte counter slot.  In
+ * host code `scratch` names a host-side stand-in; its device address on each
+ * GPU comes from hipGetSymbolAddress, called once per device with that device
+ * current, and only that address is handed to kernels and copies. */
+#define CS_NOALLOC_MAXREC (1u << 18)
+static __device__ __align__(256) unsigned char scratch[256 + (size_t)CS_NOALLOC_MAXREC * sizeof(CandRec)];
+
+/* the device address of `scratch` on the current device `device`, NULL if the
+ * runtime cannot give it (message on stderr) */
+static unsigned char *scratch_on(int device)
+{
+    static std::mutex mu;
+    static unsigned char *addr[64] = {};
+    if (device < 0 || device >= 64) return NULL;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void *ptr = NULL;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "Error: device address of the search scratch on device %d: %s\n",
+                    device, hipGetErrorString(e));
+            return NULL;
+        }
+        addr[device] = (unsigned char *)ptr;
+    }
+    return addr[device];
+}
+
+extern "C" CS_API long long cs_gpu_session_search_noalloc(GpuSession *S, double threshold,
+                                                         Candidate *out, long long cap)
+{
+    if (!S || cap < 0 || (!out && cap > 0)) {
+        fprintf(stderr, "Error: cs_gpu_session_search_noalloc: bad arguments\n");
+        return -1;
+    }
+    if (hipSetDevice(S->device) != hipSuccess) return -1;
+    unsigned char *base = scratch_on(S->device);
+    if (!base) return -1;
+    unsigned int *counter = (unsigned int *)base;
+    CandRec *recs = (CandRec *)(base + 256);
+    hipEventRecord(S->ev0, S->st);
+    if (enqueue_search_into(S, threshold, NULL, recs, counter, CS_NOALLOC_MAXREC) != 0) {
+        fprintf(stderr, "Error: GPU search launch failed\n");
+        return -1;
+    }
+    hipEventRecord(S->ev1, S->st);
+    unsigned int count = 0;
+    if (hipMemcpyAsync(&count, counter, sizeof(count), hipMemcpyDeviceToHost, S->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess) {
+        fprintf(stderr, "Error: GPU search failed: %s\n", hipGetErrorString(hipGetLastError()));
+        return -1;
+    }
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, S->ev0, S->ev1);
+    S->last_ms = ms;
+    if (count > CS_NOALLOC_MAXREC) {
+        fprintf(stderr, "Error: the search found %u records; the search that allocates "
+                        "nothing holds at most %u (raise the threshold or use "
+                        "cs_gpu_session_search_resident)\n", count, CS_NOALLOC_MAXREC);
+        return -1;
+    }
+    if ((long long)count > cap) {
+        fprintf(stderr, "Error: the search found %u records but the output holds %lld "
+                        "candidates\n", count, cap);
+        return -1;
+    }
+    if (count == 0) return 0;
+    /* the records go to the end of the caller's array, are sorted there and
+     * turned into candidates from its start (a candidate never overtakes the
+     * record it is made from, since count <= cap) */
+    CandRec *tail = (CandRec *)((char *)out + (size_t)cap * sizeof(Candidate) -
+                                (size_t)count * sizeof(CandRec));
+    if (hipMemcpyAsync(tail, recs, (size_t)count * sizeof(CandRec), hipMemcpyDeviceToHost,
+                       S->st) != hipSuccess ||
+        hipStreamSynchronize(S->st) != hipSuccess) {
+        fprintf(stderr, "Error: record readback failed\n");
+        return -1;
+    }
+    std::sort(tail, tail + count, key_less);
+    return records_to_candidates(S, tail, count, threshold, out);
 }
 
 extern "C" CS_API void cs_gpu_session_free(GpuSession *S) { session_release(S); }

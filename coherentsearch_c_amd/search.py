@@ -260,6 +260,17 @@ class Session:
     def search_resident(self, threshold):
         return take_list(self.L.cs_gpu_session_search_resident(self.h, threshold))
 
+    def search_noalloc(self, threshold, cap):
+        """cs_gpu_session_search_noalloc into an array of `cap` candidates:
+        the candidates, or RuntimeError when the search finds more records
+        than cap or the library's fixed record array holds."""
+        buf = np.zeros(max(int(cap), 0), dtype=CAND_DTYPE)
+        n = self.L.cs_gpu_session_search_noalloc(self.h, threshold,
+                                                 buf.ctypes.data if buf.size else None, int(cap))
+        if n < 0:
+            raise RuntimeError("cs_gpu_session_search_noalloc failed (see stderr)")
+        return buf[:n].copy()
+
     def zap(self, zaplist, baryv=0.0, host_copy=True):
         """Apply a zaplist to the resident spectrum (cs_gpu_session_zap); with
         host_copy the session's Spectrum receives the zapped amplitudes.

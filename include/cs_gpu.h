@@ -88,6 +88,20 @@ CS_API int cs_gpu_session_upload(GpuSession *s, const FFTFile *ft);
  * the same candidate list without the host-to-device copy. */
 CS_API CandidateList cs_gpu_session_search_resident(GpuSession *s, double threshold);
 
+/* No counterpart in the reference: a search of the resident spectrum that
+ * allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same candidates as cs_gpu_session_search_resident, written
+ * to out[0..return value) in its order; the records and their counter live in
+ * a fixed per-GPU device array of the library (at most 262144 records), so
+ * the call returns -1 with a message when the search finds more records than
+ * that or than `cap`, and never allocates, frees or builds a graph.  Calls on
+ * one GPU share that array and must not overlap (they would race); each call
+ * clears its counter first.  Returns the number of candidates, -1 on
+ * error. */
+CS_API long long cs_gpu_session_search_noalloc(GpuSession *s, double threshold,
+                                               Candidate *out, long long cap);
+
 /* Replay the search's launch sequence (counter reset, measured-sigma passes,
  * search kernel) from a captured hipGraph (enable = 1, the default for
  * sessions with their own stream) or launch it eagerly (0).  The graph is
