@@ -326,7 +326,7 @@ def dedisp_timing(steps=5, ref_flow=True):
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     d = cs.Dedisp(nch, lo, cw, dt, nspec, nsub, lodm, dmstep, nd, spectra_per_subint=P,
-                  downsamp=ds)
+                  downsamp=ds, clip_sigma=0.0)
     info = d.info
 
     class Resident:

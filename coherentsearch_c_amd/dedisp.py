@@ -34,12 +34,12 @@ class Dedisp:
     channels (lowest frequency first): the reference's delay tables, device
     rings and the resident series [numdms][numout].  Arguments follow
     prepsubband's options (-nsub, -lodm, -dmstep, -numdms, -downsamp,
-    -numout, -clip: 0 is -noclip, prepsubband's default is 6) and the raw
+    -numout, -clip: 0 is -noclip; the default 6 is prepsubband's) and the raw
     header (lofreq = centre of the lowest channel in MHz, chan_wid in MHz, dt
     in s, the reader's block length)."""
 
     def __init__(self, numchan, lofreq, chan_wid, dt, nspec, nsub, lodm, dmstep, numdms,
-                 spectra_per_subint=2048, downsamp=1, numout=0, clip_sigma=0.0, device=0):
+                 spectra_per_subint=2048, downsamp=1, numout=0, clip_sigma=6.0, device=0):
         self.L = abi.host()
         self.cfg = abi.CsDedispConfig(int(numchan), float(lofreq), float(chan_wid), float(dt),
                                       int(spectra_per_subint), int(nsub), float(lodm),

@@ -124,7 +124,7 @@ def test_library_tables_equal_oracle(oracle, cfg):
     nspec = cfg[4] * 48
     bpr, dms, idisp, offs, numout = oracle.dedisp_tables(c, nspec)
     d = cs.Dedisp(cfg[0], cfg[1], cfg[2], cfg[3], nspec, cfg[5], cfg[6], cfg[7], cfg[8],
-                  spectra_per_subint=cfg[4], downsamp=cfg[9], device=-1)
+                  spectra_per_subint=cfg[4], downsamp=cfg[9], clip_sigma=0.0, device=-1)
     gd, gi, go = d.tables()
     info = d.info
     assert info["blocksperread"] == bpr and info["numout"] == numout
@@ -139,7 +139,8 @@ def test_library_tables_equal_oracle(oracle, cfg):
 
 def test_rejections(oracle):
     base = dict(numchan=64, lofreq=1200.0, chan_wid=2.0, dt=64e-6, nspec=256 * 8, nsub=16,
-                lodm=0.0, dmstep=1.0, numdms=4, spectra_per_subint=256, downsamp=1)
+                lodm=0.0, dmstep=1.0, numdms=4, spectra_per_subint=256, downsamp=1,
+                clip_sigma=0.0)
     for bad in (dict(nsub=24),                         # does not divide 64 channels
                 dict(downsamp=3),                      # not a factor of the block
                 dict(nspec=256 * 8 + 5),               # not whole blocks

@@ -43,7 +43,7 @@ def say(*a):
 
 def _plan(c, nspec, numout=0):
     return cs.Dedisp(c[0], c[1], c[2], c[3], nspec, c[5], c[6], c[7], c[8],
-                     spectra_per_subint=c[4], downsamp=c[9], numout=numout)
+                     spectra_per_subint=c[4], downsamp=c[9], numout=numout, clip_sigma=0.0)
 
 
 CHUNKED = [(CASES[3], "tiled"), (CASES[3], "untiled"), (CASES[4], "tiled")]
