@@ -34,15 +34,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(spec, params_kw, budget_s=12.0):
+def cpu_baseline(spec, params_kw, budget_s=12.0, threads=None):
     """Oracle (reference CPU path restated, f32, blocksize 2048, OpenMP
-    round-robin chunks, every host thread) on the same spectrum: passes over
-    the chunks from a quarter of the band to its end (all harmonics filled,
-    like most of the search) until the time budget is spent; harmonic-trials/s
-    over the timed searches only."""
+    round-robin chunks, `threads` host threads, default every one) on the same
+    spectrum: passes over the chunks from a quarter of the band to its end
+    (all harmonics filled, like most of the search) until the time budget is
+    spent; harmonic-trials/s over the timed searches only."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
-    threads = len(os.sched_getaffinity(0))
+    threads = threads or len(os.sched_getaffinity(0))
     cfg = po.config(blocksize=2048, **params_kw)
     o = po.Oracle(spec.amps, spec.N, spec.T, cfg)
     nchunks = o.nchunks
@@ -590,9 +590,22 @@ def main():
         out["dm_sweep_one_observation"] = dm_sweep_resident(args.sweep_files, args.slots)
     if not args.no_serve:
         out["serving"] = serving(args.serve_requests, args.slots)
+    for key, sub, ratio in (("dm_sweep", "ratio_vs_plain_device", "batched_vs_plain_device"),
+                            ("dm_sweep", "ratio_vs_plain_reference_flow",
+                             "batched_vs_plain_reference_list_mode"),
+                            ("dm_sweep_one_observation", "ratio_vs_plain_resident",
+                             "batched_vs_plain_resident_one_observation")):
+        if key in out:
+            out[ratio] = out[key][sub]
+    if "serving" in out:
+        out["serving_p50_ms"] = out["serving"]["p50_latency_ms"]
+        out["serving_p99_ms"] = out["serving"]["p99_latency_ms"]
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(spec, pkw, args.cpu_budget)
+            one = cpu_baseline(spec, pkw, args.cpu_budget / 2, threads=1)
+            out["cpu_baseline"]["value_1_thread"] = one["value"]
+            out["cpu_baseline"]["sample_1_thread"] = one["sample"]
         except Exception as e:  # the baseline is reported, never required
             out["cpu_baseline"] = {"error": repr(e)}
     sess.close()
