@@ -451,8 +451,10 @@ int csx_zap_ops(const ZapList *z, double T, long long N, long long namps, double
     }
     for (int i = 0; i < z->n; i++) {
         const double scale = z->v[i].bary ? 1.0 : 1.0 + baryv;
-        double lo = (z->v[i].freq - 0.5 * z->v[i].width) * T * scale;
-        double hi = (z->v[i].freq + 0.5 * z->v[i].width) * T * scale;
+        /* fused as the reference's -ffp-contract=fast build of zap.c fuses
+         * (f -/+ 0.5 w), src/Makefile:135-138 */
+        double lo = fma(-0.5, z->v[i].width, z->v[i].freq) * T * scale;
+        double hi = fma(0.5, z->v[i].width, z->v[i].freq) * T * scale;
         if (hi - lo < 1.0) {
             const double mid = 0.5 * (hi + lo);
             lo = mid - 0.5;

@@ -1447,11 +1447,11 @@ __global__ __launch_bounds__(256) void zap_kernel(float2 *amps, const CsZapOp *_
         for (int i = tid; i < CS_ZAP_MEDIANBINS; i += blockDim.x) {
             if (i < o.n_lo) {
                 const float2 a = amps[o.s_lo + i];
-                pw[0][i] = a.x * a.x + a.y * a.y;
+                pw[0][i] = __builtin_fmaf(a.x, a.x, a.y * a.y);
             }
             if (i < o.n_hi) {
                 const float2 a = amps[o.s_hi + i];
-                pw[1][i] = a.x * a.x + a.y * a.y;
+                pw[1][i] = __builtin_fmaf(a.x, a.x, a.y * a.y);
             }
         }
         __syncthreads();

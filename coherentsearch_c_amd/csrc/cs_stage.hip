@@ -20,7 +20,11 @@
  *      block, by ranking) and mean = (float)(median / ln 2); one workgroup per
  *      output segment applies norm = invsqrtf(mean_old + dslope*(float)ii) with
  *      PRESTO's bit-trick inverse square root, in the reference's float
- *      operation order (no contraction).
+ *      operation order.  The reference builds datpipe.c with
+ *      -ffp-contract=fast (src/Makefile:129-133); the operations gcc fuses
+ *      there are fused here explicitly: the powers fmaf(r, r, im*im), the
+ *      level fmaf(dslope, ii, mean_old), the Newton step
+ *      y*fmaf(-(0.5x*y), y, 1.5), the unpack's outputs.
  */
 #include <hip/hip_runtime.h>
 
@@ -327,11 +331,11 @@ __global__ void unpack_kernel(const double2 *__restrict__ W, float2 *__restrict_
     const double h2i = cc * (double)(lr - hr);
     double wi, wr;
     sincospi(-2.0 * (double)il / (double)N, &wi, &wr);
-    const double h2rwr = h2r * wr, h2rwi = h2r * wi;
-    const double h2iwr = h2i * wr, h2iwi = h2i * wi;
     if (il != ih)
-        d[il] = make_float2((float)(h1r + h2rwr - h2iwi), (float)(h1i + h2iwr + h2rwi));
-    d[ih] = make_float2((float)(h1r - h2rwr + h2iwi), (float)(-h1i + h2iwr + h2rwi));
+        d[il] = make_float2((float)__builtin_fma(-h2i, wi, __builtin_fma(h2r, wr, h1r)),
+                            (float)__builtin_fma(h2r, wi, __builtin_fma(h2i, wr, h1i)));
+    d[ih] = make_float2((float)__builtin_fma(h2i, wi, __builtin_fma(-h2r, wr, h1r)),
+                        (float)__builtin_fma(h2r, wi, __builtin_fma(h2i, wr, -h1i)));
 }
 
 /* lower median of each block's powers, mean = (float)(median / ln 2) */
@@ -407,13 +411,11 @@ __global__ __launch_bounds__(64) void block_mean_kernel(const float2 *__restrict
         unsigned k0 = 0xffffffffu, k1 = 0xffffffffu;
         if (lane < n) {
             const float2 v = d[s + lane];
-            const float a = v.x * v.x, c = v.y * v.y;
-            k0 = __float_as_uint(a + c);
+            k0 = __float_as_uint(__builtin_fmaf(v.x, v.x, v.y * v.y));
         }
         if (lane + 64 < n) {
             const float2 v = d[s + lane + 64];
-            const float a = v.x * v.x, c = v.y * v.y;
-            k1 = __float_as_uint(a + c);
+            k1 = __float_as_uint(__builtin_fmaf(v.x, v.x, v.y * v.y));
         }
         bitonic128(k0, k1, lane);
         if (lane == (t & 63)) {
@@ -424,8 +426,7 @@ __global__ __launch_bounds__(64) void block_mean_kernel(const float2 *__restrict
     }
     for (int e = lane; e < n; e += 64) {
         const float2 v = d[s + e];
-        const float a = v.x * v.x, c = v.y * v.y;
-        p[e] = a + c;
+        p[e] = __builtin_fmaf(v.x, v.x, v.y * v.y);
     }
     __syncthreads();
     for (int e = lane; e < n; e += 64) {
@@ -446,8 +447,7 @@ __device__ __forceinline__ float presto_invsqrtf(float x)
     const float y = __int_as_float(i);
     const float hx = 0.5f * x;
     const float hxy = hx * y;
-    const float hxyy = hxy * y;
-    return y * (1.5f - hxyy);
+    return y * __builtin_fmaf(-hxy, y, 1.5f);
 }
 
 /* Segment sg of nblocks + 1: segment 0 is the first half of block 0,
@@ -505,8 +505,7 @@ __global__ __launch_bounds__(128) void rednoise_kernel(const float2 *__restrict_
         if (first) {
             norm = presto_invsqrtf(m0);
         } else {
-            const float sl = slope * (float)e;
-            norm = presto_invsqrtf(m0 + sl);
+            norm = presto_invsqrtf(__builtin_fmaf(slope, (float)e, m0));
         }
         const float2 v = d[s0 + e];
         out[s0 + e] = make_float2(v.x * norm, v.y * norm);

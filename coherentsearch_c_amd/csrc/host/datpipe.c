@@ -9,6 +9,13 @@
  * reference uses FFTW in single precision.  With datpipe_set_device the same
  * staging runs on the GPU (cs_gpu_stage_*), for lengths whose half has no
  * prime factor above 7; other lengths stay on the host.
+ *
+ * The reference compiles datpipe.c with -ffp-contract=fast
+ * (src/Makefile:129-133); the products gcc fuses there are fused here with
+ * explicit fma/fmaf (this file is built without contraction): the powers
+ * fmaf(r, r, im*im), the rednoise level fmaf(dslope, ii, mean_old), the
+ * inverse square root's Newton step y*fmaf(-(0.5x*y), y, 1.5), the unpack's
+ * outputs and trigonometric recurrence.
  */
 #include "host_internal.h"
 
@@ -42,7 +49,7 @@ static float fast_invsqrtf(float x)
     t.f = x;
     t.i = 0x5f3759df - (t.i >> 1);
     const float y = t.f;
-    return y * (1.5f - 0.5f * x * y * y);
+    return y * fmaf(-(0.5f * x * y), y, 1.5f);
 }
 
 /* Numerical Recipes quickselect; returns element (n-1)/2; reorders arr. */
@@ -94,14 +101,13 @@ static int realfft_forward(const float *x, long long N, double *work, double *sc
         const double h1i = 0.5 * (d[2 * il + 1] - d[2 * ih + 1]);
         const double h2r = -cc * (d[2 * il + 1] + d[2 * ih + 1]);
         const double h2i = cc * (d[2 * il] - d[2 * ih]);
-        const double a = h2r * wr, b = h2r * wi, e = h2i * wr, f = h2i * wi;
-        d[2 * il] = (float)(h1r + a - f);
-        d[2 * il + 1] = (float)(h1i + e + b);
-        d[2 * ih] = (float)(h1r - a + f);
-        d[2 * ih + 1] = (float)(-h1i + e + b);
+        d[2 * il] = (float)fma(-h2i, wi, fma(h2r, wr, h1r));
+        d[2 * il + 1] = (float)fma(h2r, wi, fma(h2i, wr, h1i));
+        d[2 * ih] = (float)fma(h2i, wi, fma(-h2r, wr, h1r));
+        d[2 * ih + 1] = (float)fma(h2r, wi, fma(h2i, wr, -h1i));
         tmp = wr;
-        wr = tmp * wpr - wi * wpi + wr;
-        wi = wi * wpr + tmp * wpi + wi;
+        wr = fma(tmp, wpr, -(wi * wpi)) + wr;
+        wi = fma(wi, wpr, tmp * wpi) + wi;
     }
     tmp = d[0];
     d[0] = (float)(tmp + d[1]);
@@ -174,7 +180,7 @@ static int rednoise(const float *in, float *out, long long nc, long long N, doub
         float powbuf[1024];
         const float *v = in + 2 * bs[b];
         for (int ii = 0; ii < bl[b]; ii++)
-            powbuf[ii] = v[2 * ii] * v[2 * ii] + v[2 * ii + 1] * v[2 * ii + 1];
+            powbuf[ii] = fmaf(v[2 * ii], v[2 * ii], v[2 * ii + 1] * v[2 * ii + 1]);
         mean[b] = (float)(nr_median(powbuf, bl[b]) / log(2.0));
     }
     out[0] = 1.0f;
@@ -199,7 +205,7 @@ static int rednoise(const float *in, float *out, long long nc, long long N, doub
         const long long s0 = bs[o] + bl[o] / 2;
         const int n = bl[o] - bl[o] / 2 + (k < nb ? bl[k] / 2 : 0);
         for (int ii = 0; ii < n; ii++) {
-            const float norm = fast_invsqrtf(mean[o] + dslope * (float)ii);
+            const float norm = fast_invsqrtf(fmaf(dslope, (float)ii, mean[o]));
             out[2 * (s0 + ii)] = in[2 * (s0 + ii)] * norm;
             out[2 * (s0 + ii) + 1] = in[2 * (s0 + ii) + 1] * norm;
         }

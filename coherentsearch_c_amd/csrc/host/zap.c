@@ -59,7 +59,7 @@ static float median_power(const float *amps, long long s, int n)
     float v[CS_ZAP_MEDIANBINS];
     for (int i = 0; i < n; i++) {
         const float re = amps[2 * (s + i)], im = amps[2 * (s + i) + 1];
-        v[i] = re * re + im * im;
+        v[i] = fmaf(re, re, im * im);     /* the reference's contracted r*r + im*im */
     }
     for (int i = 1; i < n; i++) {
         const float x = v[i];

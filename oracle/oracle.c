@@ -2022,6 +2022,43 @@ static int fft_any_d(double *re, double *im, long n)
     return 0;
 }
 
+/* FP contraction of the staging and zap code.  The reference compiles
+ * datpipe.c and zap.c with -ffp-contract=fast (src/Makefile:129-138), so gcc
+ * fuses, where FMA exists, exactly these expressions of theirs:
+ *   r*r + im*im                          -> fmaf(r, r, im*im)
+ *   mean_old + dslope*(float)ii          -> fmaf(dslope, (float)ii, mean_old)
+ *   y*(1.5f - 0.5f*x*y*y)                -> y*fmaf(-(0.5f*x*y), y, 1.5f)
+ *   h1r + a - f, a = h2r*wr, f = h2i*wi  -> fma(-h2i, wi, fma(h2r, wr, h1r)), and the
+ *                                           other three unpack outputs alike
+ *   tmp1*wpr - wi*wpi + wr               -> fma(tmp1, wpr, -(wi*wpi)) + wr
+ *   wi*wpr + tmp1*wpi + wi               -> fma(wi, wpr, tmp1*wpi) + wi
+ *   (f -/+ 0.5*w)*T*scale                -> fma(-/+0.5, w, f)*T*scale
+ * This restatement writes the fused forms out and is compiled without
+ * contraction like the rest.  With ORC_SOURCE_FORM defined it keeps the
+ * reference's unfused source expressions instead, for a build with
+ * -ffp-contract=fast whose outputs must equal the fused forms bit for bit. */
+#ifdef ORC_SOURCE_FORM
+#define ORC_POW(r, i) ((r) * (r) + (i) * (i))
+#define ORC_LEVEL(m, d, ii) ((m) + (d) * (float)(ii))
+#define ORC_NEWTON(x, y) ((y) * (1.5f - 0.5f * (x) * (y) * (y)))
+#define ORC_PQM(p, a, b, c, d) ((p) + (a) * (b) - (c) * (d))
+#define ORC_MQP(p, a, b, c, d) ((p) - (a) * (b) + (c) * (d))
+#define ORC_PPP(p, a, b, c, d) ((p) + (a) * (b) + (c) * (d))
+#define ORC_WR(t, wpr, wi, wpi, wr) ((t) * (wpr) - (wi) * (wpi) + (wr))
+#define ORC_WI(wi, wpr, t, wpi) ((wi) * (wpr) + (t) * (wpi) + (wi))
+#define ORC_EDGE(f, h, w) (((f) + (h) * (w)))
+#else
+#define ORC_POW(r, i) fmaf((r), (r), (i) * (i))
+#define ORC_LEVEL(m, d, ii) fmaf((d), (float)(ii), (m))
+#define ORC_NEWTON(x, y) ((y) * fmaf(-(0.5f * (x) * (y)), (y), 1.5f))
+#define ORC_PQM(p, a, b, c, d) fma(-(c), (d), fma((a), (b), (p)))
+#define ORC_MQP(p, a, b, c, d) fma((c), (d), fma(-(a), (b), (p)))
+#define ORC_PPP(p, a, b, c, d) fma((c), (d), fma((a), (b), (p)))
+#define ORC_WR(t, wpr, wi, wpi, wr) (fma((t), (wpr), -((wi) * (wpi))) + (wr))
+#define ORC_WI(wi, wpr, t, wpi) (fma((wi), (wpr), (t) * (wpi)) + (wi))
+#define ORC_EDGE(f, h, w) fma((h), (w), (f))
+#endif
+
 int orc_realfft(const float *samples, long long N, float *out)
 {
     if (N <= 0 || (N & 1)) return -1;
@@ -2034,7 +2071,15 @@ int orc_realfft(const float *samples, long long N, float *out)
     float *d = out;
     for (long i = 0; i < n2; i++) { d[2 * i] = (float)re[i]; d[2 * i + 1] = (float)im[i]; }
     free(re); free(im);
-    /* Numerical Recipes unpack, double trig (datpipe.c:101-138) */
+    return orc_nr_unpack(d, N);
+}
+
+/* Numerical Recipes unpack of the N/2-point complex FFT in d (N floats, in
+ * place), double trig and recurrence (datpipe.c:101-138) */
+int orc_nr_unpack(float *d, long long N)
+{
+    if (N <= 0 || (N & 1)) return -1;
+    const long n2 = (long)(N / 2);
     const double cc = -0.5, theta = -2.0 * M_PI / (double)N;
     double wr = cos(theta), wi = sin(theta), tmp1 = sin(0.5 * theta);
     const double wpr = -2.0 * tmp1 * tmp1, wpi = wi;
@@ -2044,14 +2089,13 @@ int orc_realfft(const float *samples, long long N, float *out)
         double h1i = 0.5 * (d[2 * il + 1] - d[2 * ih + 1]);
         double h2r = -cc * (d[2 * il + 1] + d[2 * ih + 1]);
         double h2i = cc * (d[2 * il] - d[2 * ih]);
-        double a = h2r * wr, b = h2r * wi, e = h2i * wr, f = h2i * wi;
-        d[2 * il] = (float)(h1r + a - f);
-        d[2 * il + 1] = (float)(h1i + e + b);
-        d[2 * ih] = (float)(h1r - a + f);
-        d[2 * ih + 1] = (float)(-h1i + e + b);
+        d[2 * il] = (float)ORC_PQM(h1r, h2r, wr, h2i, wi);
+        d[2 * il + 1] = (float)ORC_PPP(h1i, h2i, wr, h2r, wi);
+        d[2 * ih] = (float)ORC_MQP(h1r, h2r, wr, h2i, wi);
+        d[2 * ih + 1] = (float)ORC_PPP(-h1i, h2i, wr, h2r, wi);
         tmp1 = wr;
-        wr = tmp1 * wpr - wi * wpi + wr;
-        wi = wi * wpr + tmp1 * wpi + wi;
+        wr = ORC_WR(tmp1, wpr, wi, wpi, wr);
+        wi = ORC_WI(wi, wpr, tmp1, wpi);
     }
     tmp1 = d[0];
     d[0] = (float)(tmp1 + d[1]);
@@ -2059,13 +2103,14 @@ int orc_realfft(const float *samples, long long N, float *out)
     return 0;
 }
 
+
 static float presto_invsqrtf(float x)
 {
     union { float f; int32_t i; } t;
     t.f = x;
     t.i = 0x5f3759df - (t.i >> 1);
     float y = t.f;
-    return y * (1.5f - 0.5f * x * y * y);
+    return ORC_NEWTON(x, y);
 }
 
 #define ESWAP(a, b) do { float t_ = (a); (a) = (b); (b) = t_; } while (0)
@@ -2097,6 +2142,16 @@ static float presto_median(float *arr, int n)
     }
 }
 
+/* the two rednoise helpers on their own, for the pins */
+float orc_presto_invsqrtf(float x) { return presto_invsqrtf(x); }
+float orc_presto_median(const float *x, int n)
+{
+    float buf[1024];
+    if (n < 1 || n > 1024) return NAN;
+    memcpy(buf, x, sizeof(float) * (size_t)n);
+    return presto_median(buf, n);
+}
+
 /* rednoise.c restatement (datpipe.c:143-253), startwidth 6, endwidth 100,
  * endfreq 6 Hz.  in/out: ncomplex interleaved complex. */
 static int rednoise_o(const float *in, float *out, long long nc, long long N, double dt)
@@ -2118,7 +2173,7 @@ static int rednoise_o(const float *in, float *out, long long nc, long long N, do
     mid_old = nold / 2;
     for (ii = 0; ii < nold; ii++) {
         float r = oldb[2 * ii], im = oldb[2 * ii + 1];
-        powbuf[ii] = r * r + im * im;
+        powbuf[ii] = ORC_POW(r, im);
     }
     mean_old = (float)(presto_median(powbuf, nold) / log(2.0));
     norm = presto_invsqrtf(mean_old);
@@ -2139,18 +2194,18 @@ static int rednoise_o(const float *in, float *out, long long nc, long long N, do
         mid_new = nnew / 2;
         for (ii = 0; ii < nnew; ii++) {
             float r = newb[2 * ii], im = newb[2 * ii + 1];
-            powbuf[ii] = r * r + im * im;
+            powbuf[ii] = ORC_POW(r, im);
         }
         mean_new = (float)(presto_median(powbuf, nnew) / log(2.0));
         dslope = (float)((mean_new - mean_old) / (0.5 * (nold + nnew)));
         ii = 0;
         for (ind = mid_old; ind < nold; ii++, ind++) {
-            norm = presto_invsqrtf(mean_old + dslope * (float)ii);
+            norm = presto_invsqrtf(ORC_LEVEL(mean_old, dslope, ii));
             outb[2 * ii] = oldb[2 * ind] * norm;
             outb[2 * ii + 1] = oldb[2 * ind + 1] * norm;
         }
         for (ind = 0; ind < mid_new; ii++, ind++) {
-            norm = presto_invsqrtf(mean_old + dslope * (float)ii);
+            norm = presto_invsqrtf(ORC_LEVEL(mean_old, dslope, ii));
             outb[2 * ii] = newb[2 * ind] * norm;
             outb[2 * ii + 1] = newb[2 * ind + 1] * norm;
         }
@@ -2167,7 +2222,7 @@ static int rednoise_o(const float *in, float *out, long long nc, long long N, do
     }
     ii = 0;
     for (ind = mid_old; ind < nold; ii++, ind++) {
-        norm = presto_invsqrtf(mean_old + dslope * (float)ii);
+        norm = presto_invsqrtf(ORC_LEVEL(mean_old, dslope, ii));
         outb[2 * ii] = oldb[2 * ind] * norm;
         outb[2 * ii + 1] = oldb[2 * ind + 1] * norm;
     }
@@ -2219,7 +2274,7 @@ static float orc_zap_median(const float *amps, long long namps, long long start,
     float buf[200];
     for (int i = 0; i < count; i++) {
         const float re = amps[2 * (start + i)], im = amps[2 * (start + i) + 1];
-        buf[i] = re * re + im * im;
+        buf[i] = ORC_POW(re, im);
     }
     qsort(buf, (size_t)count, sizeof(float), orc_float_cmp);
     return buf[(count - 1) / 2];
@@ -2232,8 +2287,8 @@ int orc_zap(float *amps, long long namps, long long N, double T, const double *f
     OrcZapBin *b = malloc(sizeof(OrcZapBin) * (size_t)nbirds);
     for (int i = 0; i < nbirds; i++) {
         const double scale = bary[i] ? 1.0 : (1.0 + baryv);
-        double lo = (freq_hz[i] - 0.5 * width_hz[i]) * T * scale;
-        double hi = (freq_hz[i] + 0.5 * width_hz[i]) * T * scale;
+        double lo = ORC_EDGE(freq_hz[i], -0.5, width_hz[i]) * T * scale;
+        double hi = ORC_EDGE(freq_hz[i], 0.5, width_hz[i]) * T * scale;
         if (hi - lo < 1.0) {
             const double mid = 0.5 * (hi + lo);
             lo = mid - 0.5;

@@ -168,6 +168,12 @@ int orc_realfft_rednoise(const float *samples, long long N, double dt,
                          float *out);
 /* Forward realfft only (PRESTO packing: out[0]=DC, out[1]=Nyquist). */
 int orc_realfft(const float *samples, long long N, float *out);
+/* the Numerical Recipes unpack alone, in place on the N/2-point FFT (N floats) */
+int orc_nr_unpack(float *d, long long N);
+/* PRESTO's bit-trick inverse square root with one Newton step, and its
+ * quickselect median (element (n-1)/2 ascending, n <= 1024), as rednoise uses them */
+float orc_presto_invsqrtf(float x);
+float orc_presto_median(const float *x, int n);
 /* rednoise alone (datpipe.c:143-253) of a packed realfft output in[N]. */
 int orc_rednoise(const float *in, long long N, double dt, float *out);
 

@@ -106,6 +106,9 @@ def lib():
         L.orc_realfft_rednoise.argtypes = [P, LL, D, P]; L.orc_realfft_rednoise.restype = I
         L.orc_realfft.argtypes = [P, LL, P]; L.orc_realfft.restype = I
         L.orc_rednoise.argtypes = [P, LL, D, P]; L.orc_rednoise.restype = I
+        L.orc_nr_unpack.argtypes = [P, LL]; L.orc_nr_unpack.restype = I
+        L.orc_presto_invsqrtf.argtypes = [C.c_float]; L.orc_presto_invsqrtf.restype = C.c_float
+        L.orc_presto_median.argtypes = [P, I]; L.orc_presto_median.restype = C.c_float
         L.orc_zap.argtypes = [P, LL, LL, D, P, P, P, I, D]; L.orc_zap.restype = I
         L.orc_delay_from_dm.argtypes = [D, D]; L.orc_delay_from_dm.restype = D
         L.orc_choose_good_N.argtypes = [LL]; L.orc_choose_good_N.restype = LL
@@ -661,6 +664,24 @@ def clip_all(cfg, raw):
     if n < 0:
         raise ValueError("not whole blocks")
     return x, lv, n
+
+
+def nr_unpack(fft_out):
+    """the Numerical Recipes unpack alone of an N/2-point complex FFT (N floats)"""
+    d = np.array(fft_out, dtype=np.float32, copy=True)
+    if lib().orc_nr_unpack(ptr(d), d.size) != 0:
+        raise ValueError("unpack failed")
+    return d
+
+
+def presto_invsqrtf(x):
+    f = lib().orc_presto_invsqrtf
+    return np.array([f(float(v)) for v in np.asarray(x, np.float32).ravel()], dtype=np.float32)
+
+
+def presto_median(x):
+    x = np.ascontiguousarray(x, np.float32)
+    return lib().orc_presto_median(ptr(x), x.size)
 
 
 def avg_var(x):
