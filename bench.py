@@ -467,6 +467,9 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="engine worker streams")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch shapes)")
+    ap.add_argument("--wide-only", action="store_true",
+                    help="only time the nharms 120 / maxdecim 16 and nharms 128 / maxdecim 6 "
+                         "searches of the 2^24-sample spectrum (prints one JSON line)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -507,6 +510,10 @@ def main():
     spec = cs.redfft(x, dt)
     del x
     log("staging (synth + realfft + rednoise): %.2f s" % (time.perf_counter() - t))
+    if args.wide_only:
+        print(json.dumps({"wide_harmonics": wide_harmonics(spec, steps=max(1, args.steps))}),
+              flush=True)
+        return
     pkw = dict(nharms=60, m=16, maxdecim=6)
     params = cs.make_params(**pkw)
     tune = []

@@ -8,7 +8,7 @@
  *                                         A.im (WS = m + 4, 16-byte rows)
  *   hinfo   int2[nh]                      (rfloor0, first residue with carry)
  *   tw      float2[TW_SIZE]               twiddles, N(N-1)/2 + j for N <= TW_NMAX
- *   chunks  ChunkDesc[nchunks]            i0, n, filled mask (128 bits), per-slot nvalid,
+ *   chunks  ChunkDesc[nchunks]            i0, n, filled mask (512 bits), per-slot nvalid,
  *                                         per-slot per-width 1/(sigma*sd_w)
  *   tiles   int4[ntiles]                  (chunk, first trial, count, 0)
  *   recs    CandRec[cap] + counter        above-threshold records
@@ -20,7 +20,7 @@
 
 #define CS_TILE 256            /* trials per workgroup (one per lane) */
 #define CS_MAXSLOT 17          /* base + up to 16 decimations */
-#define CS_NSEQ 12             /* width candidates 1,2,3,4,6,9,13,19,28,42,63,94 */
+#define CS_NSEQ 16             /* width candidates 1,2,3,4,6,9,13,19,28,42,63,94,141,211,316,474 */
 #define CS_TW_NMAX 257
 #define CS_SIGCAP 8192         /* measured sigma: samples per (chunk, decimation slot) */
 #define CS_TW_SIZE (CS_TW_NMAX * (CS_TW_NMAX - 1) / 2)
@@ -36,7 +36,7 @@ typedef struct {
     long long i0;
     int n;
     int pad;
-    unsigned long long filled[2];   /* bit (h-1) & 63 of word (h-1) >> 6: harmonic h fits */
+    unsigned long long filled[8];   /* bit (h-1) & 63 of word (h-1) >> 6: harmonic h fits */
     int nvalid[CS_MAXSLOT];         /* trials below the decimated Nyquist */
     int valid[CS_MAXSLOT];          /* 1/sigma > 0 */
     float invsw[CS_MAXSLOT][CS_NSEQ];     /* f32 path: (float)(invsigma*shape_w) */

@@ -42,6 +42,7 @@
 
 #include "cands.h"
 #include "core.h"
+#include "cs_device.h"
 #include "cs_gpu.h"
 #include "cs_stage.h"
 
@@ -127,8 +128,9 @@ long long footprint_of(const CsSearchRequest *r, const SearchParams *p)
     const double trials = std::max(0.0, (r->hifreq * T - r_lo) / lodr + 1.0);
     const int nprof = r->blocksize > 0 ? r->blocksize : CS_GPU_DEFAULT_BLOCKSIZE;
     const double nchunks = trials / nprof + 2.0 * p->nharms + 2.0;
-    b += (long long)(trials / 256.0 * 16.0 + nchunks * (16.0 + 1024.0));
+    b += (long long)(trials / 256.0 * 16.0 + nchunks * (16.0 + (double)sizeof(ChunkDesc)));
     b += (1 << 16) * 16 + (1 << 20);
+    b += cs_session_general_bytes(p, lodr);
     return b;
 }
 

@@ -25,6 +25,7 @@
 #include "cands.h"
 #include "core.h"
 #include "cs_device.h"
+#include "cs_general.h"
 #include "cs_gpu.h"
 #include "cs_stage.h"
 #include "fast_kernel.hpp"
@@ -156,6 +157,9 @@ __global__ __launch_bounds__(256) void sigma_kernel(const T *__restrict__ samp,
     }
 }
 
+/* HBM the general kernel's work buffer may take per session */
+#define CS_GEN_WORK_BUDGET (1536ll << 20)
+
 struct GpuSession {
     const SearchParams *p;
     int device;
@@ -206,6 +210,14 @@ struct GpuSession {
     std::vector<int2> h_hinfo;
     void *d_mout;                 /* every trial's metric per slot (metric statistics) */
     bool mout_ready;
+    /* the general kernel (configurations no register kernel holds) */
+    bool gen;
+    CsGenPlan plan;
+    void *d_plan;                 /* ops, pool, twiddles */
+    void *d_work;                 /* per-lane value slots and stacks */
+    size_t work_bytes;
+    std::vector<char> h_planbuf;  /* host copy kept alive for the asynchronous upload */
+    GenArgs gargs;
 };
 
 /* cos and sin as two separate libm calls, through pointers the compiler cannot
@@ -290,6 +302,8 @@ static void session_release(GpuSession *S)
     dfree(S, S->d_block);
     dfree(S, S->d_stack);
     dfree(S, S->d_mout);
+    dfree(S, S->d_plan);
+    dfree(S, S->d_work);
     if (S->gexec) hipGraphExecDestroy(S->gexec);
     if (S->graph) hipGraphDestroy(S->graph);
     if (S->ev0) hipEventDestroy(S->ev0);
@@ -436,6 +450,123 @@ __attribute__((visibility("hidden"))) void cs_session_set_launch(GpuSession *S, 
     S->graph = NULL;
 }
 
+/* the persistent grid's workgroups for a plan of nvslots value slots */
+static int gen_grid(int nvslots, int nh, bool f64, int ntiles)
+{
+    const double lane_bytes = (double)cs_gen_work_bytes(nvslots, nh, 1, f64);
+    long long g = (long long)((double)CS_GEN_WORK_BUDGET / (lane_bytes * CS_TILE));
+    g = std::min<long long>(g, CS_GEN_GRID);
+    g = std::min<long long>(g, std::max(ntiles, 1));
+    return (int)std::max<long long>(g, 1);
+}
+
+/* The general kernel's plan, tables and work buffer, and its fixed
+ * arguments (the per-launch ones are set in enqueue_search_into). */
+static int gen_setup(GpuSession *S)
+{
+    int Hk[CS_MAXSLOT], ks[CS_MAXSLOT], nextra = 0;
+    GenArgs &g = S->gargs;
+    memset((void *)&g, 0, sizeof(g));
+    for (int s = 0; s < S->nslot; s++) {
+        const int k = S->slot_k[s];
+        g.slot_k[s] = k;
+        g.slot_Hk[s] = S->slot_Hk[s];
+        g.wmask[s] = S->args.wmask[s];
+        if (s == 0) {
+            g.slot_c2r[s] = 0;
+        } else if (S->nh % k == 0) {
+            g.slot_c2r[s] = -1;                 /* a fold of the base profile */
+        } else {
+            Hk[nextra] = S->slot_Hk[s];
+            ks[nextra] = k;
+            g.slot_c2r[s] = ++nextra;
+        }
+        if (g.wmask[s] >> 14) {
+            fprintf(stderr, "Error: boxcar widths above 211 are not supported by the GPU "
+                            "search\n");
+            return -1;
+        }
+    }
+    if (cs_gen_build_plan(S->nh, nextra, Hk, ks, &S->plan) != 0) {
+        fprintf(stderr, "Error: c2r plan for %d harmonics failed\n", S->nh);
+        return -1;
+    }
+    const CsGenPlan &P = S->plan;
+    const size_t tsz = S->f64 ? 16 : 8;
+    size_t off = 0;
+    const size_t o_ops = cs_part(&off, P.ops.size() * sizeof(CsGenOp));
+    const size_t o_pool = cs_part(&off, P.pool.size() * sizeof(int));
+    const size_t o_tw = cs_part(&off, std::max<size_t>(P.twc.size(), 1) * tsz);
+    const size_t bytes = off;
+    S->h_planbuf.assign(bytes, 0);
+    char *h = S->h_planbuf.data();
+    memcpy(h + o_ops, P.ops.data(), P.ops.size() * sizeof(CsGenOp));
+    memcpy(h + o_pool, P.pool.data(), P.pool.size() * sizeof(int));
+    for (size_t i = 0; i < P.twc.size(); i++) {
+        if (S->f64) {
+            const double v[2] = {P.twc[i], P.tws[i]};
+            memcpy(h + o_tw + i * tsz, v, tsz);
+        } else {
+            const float v[2] = {(float)P.twc[i], (float)P.tws[i]};
+            memcpy(h + o_tw + i * tsz, v, tsz);
+        }
+    }
+    if (dalloc(S, &S->d_plan, bytes) != hipSuccess ||
+        hipMemcpyAsync(S->d_plan, h, bytes, hipMemcpyHostToDevice, S->st) != hipSuccess) {
+        fprintf(stderr, "Error: c2r plan upload failed\n");
+        return -1;
+    }
+    const int nv = P.nslots + P.nscratch;
+    const int grid = gen_grid(nv, S->nh, S->f64, S->args.ntiles);
+    g.nlanes = (long long)grid * CS_TILE;
+    const size_t wbytes = cs_gen_work_bytes(nv, S->nh, g.nlanes, S->f64);
+    if (dalloc(S, &S->d_work, wbytes) != hipSuccess) {
+        fprintf(stderr, "Error: general-kernel work buffer of %zu bytes failed\n", wbytes);
+        return -1;
+    }
+    S->work_bytes = wbytes;
+    g.amps = S->d_amps;
+    g.namps = S->namps;
+    g.wtab = S->d_wtab;
+    g.hinfo = S->d_hinfo;
+    g.tiles = S->d_tiles;
+    g.ntiles = S->args.ntiles;
+    g.q = S->q;
+    g.pnum = S->pnum;
+    g.ws = S->args.ws;
+    g.m = S->m;
+    g.mp = (S->m + 3) / 4 * 4;
+    g.nh = S->nh;
+    g.chunks = S->d_chunks;
+    g.nslot = S->nslot;
+    g.ops = (const CsGenOp *)((char *)S->d_plan + o_ops);
+    g.pool = (const int *)((char *)S->d_plan + o_pool);
+    g.tw = (const void *)((char *)S->d_plan + o_tw);
+    g.nc2r = (int)P.c2r.size();
+    for (int c = 0; c < g.nc2r; c++) g.c2r[c] = P.c2r[(size_t)c];
+    g.nslots = P.nslots;
+    g.nvslots = nv;
+    g.work = S->d_work;
+    return 0;
+}
+
+/* Device bytes beyond the tables that a session of p at trial step lodr
+ * takes for the general kernel's work buffer (at most), 0 when a register
+ * kernel serves it -- the engine's admission estimate. */
+__attribute__((visibility("hidden"))) long long cs_session_general_bytes(const SearchParams *p,
+                                                                        double lodr)
+{
+    const int nh = p->nharms, m = p->m, mp = m <= 16 ? 16 : (m <= 32 ? 32 : 0);
+    const bool f64 = p->precision == CS_PREC_F64;
+    for (const FastVariant &v : kFast) {
+        if (v.nh != nh || v.m != mp || (f64 && !v.fn64)) continue;
+        double need = 0.0;
+        for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + mp + 2;
+        if (need <= (double)CS_WIN_CAP(nh, mp)) return 0;
+    }
+    return CS_GEN_WORK_BUDGET + (16ll << 20);
+}
+
 /* ext: run on the caller's stream with stream-ordered allocations (the
  * engine's concurrent sessions); null: own stream, plain allocations. */
 __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFTFile *ft, const SearchParams *p,
@@ -478,6 +609,11 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->d_samp = NULL;
     S->last_ms = 0.0;
     S->fast = NULL;
+    S->gen = false;
+    S->d_plan = NULL;
+    S->d_work = NULL;
+    S->work_bytes = 0;
+    memset((void *)&S->gargs, 0, sizeof(S->gargs));
 
     const int nh = p->nharms, m = p->m;
     std::vector<float> wtab;
@@ -512,7 +648,9 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         S->slot_Hk[S->nslot] = nh / k;
         S->nslot++;
     }
-    /* the kernel of the smallest padded tap count M >= m */
+    /* the register kernel of the smallest padded tap count M >= m, when one
+     * holds this configuration and the tile's windows fit its LDS buffer;
+     * otherwise the general kernel */
     mp = m <= 16 ? 16 : (m <= 32 ? 32 : 0);
     for (const FastVariant &v : kFast) {
         if (v.nh != nh || v.m != mp || (S->f64 && !v.fn64)) continue;
@@ -520,12 +658,25 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         for (int s = 1; s < S->nslot; s++) ok = ok && S->slot_k[s] <= v.kmax;
         if (ok) { S->fast = &v; break; }
     }
+    if (S->fast) {
+        double need = 0.0;
+        for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + mp + 2;
+        if (need > (double)CS_WIN_CAP(nh, mp)) S->fast = NULL;
+    }
     if (!S->fast) {
-        fprintf(stderr, "Error: no GPU kernel for nharms=%d m=%d %s with these decimations "
-                        "(kernels: nharms 1-64 with m <= 32 in f32 or f64, nharms 65-128 with "
-                        "m <= 16 in f32, decimations up to %d)\n",
-                nh, m, S->f64 ? "f64" : "f32", CS_KMAX);
-        goto fail;
+        if (nh > CS_GEN_MAXH || m < 2 || m > 64 || (m & 1)) {
+            fprintf(stderr, "Error: no GPU kernel for nharms=%d m=%d (the GPU search holds up to "
+                            "%d harmonics and an even m up to 64)\n", nh, m, CS_GEN_MAXH);
+            goto fail;
+        }
+        for (int s = 1; s < S->nslot; s++)
+            if (S->slot_k[s] > CS_KMAX) {
+                fprintf(stderr, "Error: decimation %d is not supported by the GPU search\n",
+                        S->slot_k[s]);
+                goto fail;
+            }
+        S->gen = true;
+        mp = (m + 3) / 4 * 4;
     }
     S->q = dplans->plans[0].q;
     S->pnum = dplans->plans[0].pnum;
@@ -540,15 +691,6 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         (double)nh * ((double)total * S->pnum / S->q + 2.0) >= 2147483000.0) {
         fprintf(stderr, "Error: trial grid too large for the GPU residue walk\n");
         goto fail;
-    }
-    /* all harmonic windows of a tile must fit the kernel's LDS buffer */
-    {
-        double need = 0.0;
-        for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + mp + 2;
-        if (need > (double)CS_WIN_CAP(nh, mp)) {
-            fprintf(stderr, "Error: --hidr too large for the GPU tile window\n");
-            goto fail;
-        }
     }
 
     /* plan arena: row (h, res) = M - m zeros, W[0..m-1], A.re, A.im, zero
@@ -686,7 +828,9 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->args.counter = S->d_counter;
     S->args.cap = S->cap;
     S->args.xcd_stripe = 0;
-    {
+    if (S->gen) {
+        if (gen_setup(S) != 0) goto fail;
+    } else {
         /* a tuned launch for this kernel class, when there is one */
         CsTuneResult tr;
         if (cs_tune_lookup(device, nh, mp, S->f64, S->measured, &tr))
@@ -735,7 +879,22 @@ static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_ou
     const int ntiles = S->args.ntiles;
     const int unit = 8 * std::max(S->args.xcd_stripe, 1);
     const int nblk = ((ntiles + unit - 1) / unit) * unit;
-    fast_launch_fn fn = S->f64 ? S->fast->fn64 : S->fast->fn;
+    auto fn = [S](const csk::FastArgs *a, int nb, hipStream_t st) -> hipError_t {
+        if (!S->gen) return (S->f64 ? S->fast->fn64 : S->fast->fn)(a, nb, st);
+        GenArgs &g = S->gargs;
+        g.recs = a->recs;
+        g.counter = a->counter;
+        g.cap = a->cap;
+        g.threshold = a->threshold;
+        g.samp = a->samp;
+        g.mout = a->mout;
+        g.mstride = a->mstride;
+        g.stack_out = a->stack_out;
+        g.dump_t0 = a->dump_t0;
+        g.dump_n = a->dump_n;
+        g.grp = a->grp;
+        return cs_gen_launch(&g, S->f64, st);
+    };
     hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, counter);
     if (hipGetLastError() != hipSuccess) return -1;
     if (S->measured && ntiles > 0) {

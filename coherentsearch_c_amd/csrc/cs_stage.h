@@ -84,6 +84,11 @@ __attribute__((visibility("hidden"))) int cs_session_stage_graph(GpuSession *s, 
 __attribute__((visibility("hidden"))) int cs_session_stage_loaded(GpuSession *s, CsStage *g,
                                                                   float *amps_host);
 
+/* device bytes a session of p at trial step lodr adds for the general
+ * kernel's work buffer (an upper bound; 0 when a register kernel serves it) */
+__attribute__((visibility("hidden"))) long long cs_session_general_bytes(const SearchParams *p,
+                                                                        double lodr);
+
 /* launch shape of the search kernel (no effect on results) */
 __attribute__((visibility("hidden"))) void cs_session_set_launch(GpuSession *s, int lane_order,
                                                                  int xcd_stripe);

@@ -40,7 +40,11 @@ CS_API int cs_gpu_available(int device, char *errbuf, size_t errlen);
  * device-resident plan arena, chunk and tile tables for one observation
  * geometry.  dplans must be rational (every P > 0); p is borrowed for the
  * session's lifetime and compared by pointer in _matches, like the
- * reference.  Returns NULL on failure. */
+ * reference.  Any nharms 1-512, even m <= 64, f32 or f64, decimations up to
+ * 16 and the fsp 1.5 width bank up to width 211: register kernels serve
+ * nharms <= 64 (m <= 32) and 65-128 (m <= 16, f32) when the tile's windows
+ * fit their LDS buffer, the general kernel everything else.  Returns NULL on
+ * failure or outside that range. */
 CS_API GpuSession *cs_gpu_session_create(const FFTFile *ft, const SearchParams *p,
                                          const DirectPlanSet *dplans,
                                          const long long *cstarts, long long total,

@@ -248,9 +248,15 @@ def test_session_reuse_and_mismatch(oracle):
 
 
 def test_unsupported_configuration_fails_loudly():
+    """Outside the GPU search's range cs_gpu_session_create returns NULL with
+    a message: more than 512 harmonics, m above 64, decimations above 16;
+    inside it the general kernel takes what the register kernels cannot
+    hold."""
     amps = synth.spectrum_with_pulsar(1 << 14, 300.1, 4, 0.05, 5.0, 1)
     spec = cs.Spectrum(amps, 1 << 15, synth.DT)
-    for kw in (dict(nharms=16, m=34), dict(nharms=129), dict(nharms=72, m=32),
-               dict(nharms=72, precision="f64"), dict(nharms=16, hidr=0.9)):
+    for kw in (dict(nharms=513), dict(nharms=16, m=66), dict(nharms=60, maxdecim=20)):
         with pytest.raises(RuntimeError):
             cs.Session(spec, cs.make_params(**kw), blocksize=2048)
+    for kw in (dict(nharms=16, m=34), dict(nharms=129), dict(nharms=72, m=32),
+               dict(nharms=72, precision="f64"), dict(nharms=16, hidr=0.9)):
+        cs.Session(spec, cs.make_params(**kw), blocksize=2048).close()
