@@ -117,7 +117,7 @@ namespace {
 long long footprint_of(const CsSearchRequest *r, const SearchParams *p)
 {
     const long long N = r->N;
-    long long b = 4 * N + 16 * N + 4 * N + 4 * N + N / 2;
+    long long b = 4 * N + cs_stage_fft_bytes(N) + 4 * N + 4 * N + N / 2;
     int pnum = 0, q = 0;
     if (!csx_trial_grid_rational(p->hidr / (double)p->nharms, &pnum, &q)) q = 1;
     b += (long long)p->nharms * q * 4 * (p->m + 12);

@@ -53,6 +53,10 @@ extern "C" __attribute__((visibility("hidden"))) void cs_stage_bind_stream(CsSta
 
 extern "C" __attribute__((visibility("hidden"))) long long cs_stage_N(const CsStage *g);
 
+/* device bytes of the transform buffers of a stage plan for N samples
+ * (Bluestein's three convolution buffers when N/2 has a prime factor above 7) */
+extern "C" __attribute__((visibility("hidden"))) long long cs_stage_fft_bytes(long long N);
+
 /* a stage plan on the caller's stream with stream-ordered allocations */
 extern "C" __attribute__((visibility("hidden"))) CsStage *
 cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double endfreq,

@@ -8,7 +8,13 @@
  *      four-step transform, each level a batch of <= 512-point sub-FFTs done
  *      in LDS (Stockham stages of radix 8/7/5/4/3/2), the inter-level
  *      twiddles exp(-2 pi i (i*k mod LS)/LS) from sincospi of an exact integer
- *      phase, the last level writing in transposed (natural) order;
+ *      phase, the last level writing in transposed (natural) order.  When
+ *      N/2 has a prime factor above 7 the transform is Bluestein's chirp-z:
+ *      x_j w_j (w_j = exp(-i pi j^2/n), j^2 mod 2n exact) zero-padded to a
+ *      7-smooth M >= 2n - 1, convolved with the conjugate chirp by two
+ *      forward M-point transforms of the same machinery (the chirp's own
+ *      transform computed once per plan; the inverse as a forward transform
+ *      of the conjugate), then scaled by w_k / M;
  *   3. Numerical-Recipes unpack (datpipe.c:104-137): the FFT rounded to float,
  *      the half-sum/half-difference in float, the twiddle products in double,
  *      stored as float; DC and Nyquist packed into bin 0.  The twiddle of bin
@@ -512,6 +518,69 @@ __global__ __launch_bounds__(128) void rednoise_kernel(const float2 *__restrict_
     }
 }
 
+/* the chirp exp(-i pi j^2 / n) from the exact phase j^2 mod 2n */
+__device__ __forceinline__ double2 chirp(long long j, long long n)
+{
+    long long m = (long long)(((unsigned long long)j * (unsigned long long)j) %
+                              (unsigned long long)(2 * n));
+    if (m > n) m -= 2 * n;
+    double s, c;
+    sincospi(-(double)m / (double)n, &s, &c);
+    return make_double2(c, s);
+}
+
+/* Bluestein input: the float series as n complex points times the chirp,
+ * zero-padded to M */
+__global__ void chirp_in_kernel(const float2 *__restrict__ x, double2 *__restrict__ z,
+                                long long n, long long M)
+{
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    double2 v = make_double2(0.0, 0.0);
+    if (j < n) {
+        const float2 a = x[j];
+        v = cmul(make_double2((double)a.x, (double)a.y), chirp(j, n));
+    }
+    z[j] = v;
+}
+
+/* the convolution kernel b: conj(chirp) at j and M - j (|j| < n), zero between */
+__global__ void chirp_b_kernel(double2 *__restrict__ b, long long n, long long M)
+{
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
+    double2 v = make_double2(0.0, 0.0);
+    long long a = -1;
+    if (j < n) a = j;
+    else if (j > M - n) a = M - j;
+    if (a >= 0) {
+        const double2 w = chirp(a, n);
+        v = make_double2(w.x, -w.y);
+    }
+    b[j] = v;
+}
+
+/* z <- conj(z * bf): the product of the transforms, conjugated so that the
+ * next forward transform computes the inverse (conjugated) */
+__global__ void chirp_mul_kernel(double2 *__restrict__ z, const double2 *__restrict__ bf,
+                                 long long M)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= M) return;
+    const double2 p = cmul(z[k], bf[k]);
+    z[k] = make_double2(p.x, -p.y);
+}
+
+/* X_k = conj(y_k) / M * chirp(k), k < n */
+__global__ void chirp_out_kernel(const double2 *__restrict__ y, double2 *__restrict__ X,
+                                 long long n, double invM)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double2 v = y[k];
+    X[k] = cmul(make_double2(v.x * invM, -v.y * invM), chirp(k, n));
+}
+
 bool factor7(long long n, std::vector<int> &pf)
 {
     for (int p : {2, 3, 5, 7})
@@ -548,6 +617,26 @@ bool plan_levels(long long n, std::vector<int> &Ls)
     return false;
 }
 
+/* the smallest 2^a 3^b 5^c 7^d >= x that plan_levels accepts (Bluestein's
+ * convolution length), 0 if none fits the levels */
+long long smooth7_at_least(long long x)
+{
+    long long best = 0;
+    const long long lim = (long long)1 << 40;
+    for (long long a = 1; a < lim; a *= 2)
+        for (long long b = a; b < lim; b *= 3)
+            for (long long c = b; c < lim; c *= 5)
+                for (long long d = c; d < lim; d *= 7) {
+                    if (d < x) continue;
+                    if (!best || d < best) {
+                        std::vector<int> Ls;
+                        if (plan_levels(d, Ls)) best = d;
+                    }
+                    break;
+                }
+    return best;
+}
+
 void radices(int L, std::vector<int> &r)
 {
     while (L % 8 == 0) { r.push_back(8); L /= 8; }
@@ -561,6 +650,9 @@ void radices(int L, std::vector<int> &r)
 
 struct CsStage {
     long long N, n2;
+    long long fl;                 /* complex FFT length: n2, or Bluestein's M */
+    bool blue;                    /* N/2 has a prime factor above 7: chirp-z */
+    double2 *d_bf;                /* Bluestein: transform of the conjugate chirp, fl points */
     double dt;
     int device;
     int nlev;
@@ -694,6 +786,18 @@ extern "C" CS_API void cs_gpu_stage_free(CsStage *g)
         }                                                                          \
     } while (0)
 
+static double2 *run_fft(CsStage *g, const void *in, bool floatin, double2 *a_buf,
+                        double2 *b_buf, hipStream_t st);
+
+/* device bytes of a stage plan's transform buffers for N samples */
+extern "C" __attribute__((visibility("hidden"))) long long cs_stage_fft_bytes(long long N)
+{
+    std::vector<int> Ls;
+    const long long n2 = N / 2;
+    if (plan_levels(n2, Ls)) return 32 * n2;
+    return 48 * smooth7_at_least(2 * n2 - 1);
+}
+
 extern "C" __attribute__((visibility("hidden"))) CsStage *
 cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double endfreq,
                    int device, hipStream_t ext)
@@ -712,10 +816,17 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
     }
     std::vector<int> Ls;
     const long long n2 = N / 2;
+    long long fl = n2;
+    bool blue = false;
     if (!plan_levels(n2, Ls)) {
-        fprintf(stderr, "Error: device realfft needs N/2 with prime factors <= 7 and at most "
-                        "%d^%d points (N = %lld)\n", kLmax, kMaxLevels, N);
-        return nullptr;
+        blue = true;
+        fl = smooth7_at_least(2 * n2 - 1);
+        Ls.clear();
+        if (fl <= 0 || !plan_levels(fl, Ls)) {
+            fprintf(stderr, "Error: device realfft of N = %lld exceeds the transform's %d^%d "
+                            "points\n", N, kLmax, kMaxLevels);
+            return nullptr;
+        }
     }
     /* host tables, packed into the pinned staging buffer at the end */
     std::vector<double2> h_tw;
@@ -724,6 +835,9 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
     CsStage *g = new CsStage();
     g->N = N;
     g->n2 = n2;
+    g->fl = fl;
+    g->blue = blue;
+    g->d_bf = nullptr;
     g->dt = dt;
     g->device = device;
     g->nlev = (int)Ls.size();
@@ -839,8 +953,9 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
         /* one device block: work buffers, then the tables in upload order */
         const size_t nbk = (size_t)g->nblocks, nramp = (size_t)g->nramp;
         size_t off = 0;
-        const size_t o_z0 = cs_part(&off, (size_t)n2 * sizeof(double2));
-        const size_t o_z1 = cs_part(&off, (size_t)n2 * sizeof(double2));
+        const size_t o_z0 = cs_part(&off, (size_t)fl * sizeof(double2));
+        const size_t o_z1 = cs_part(&off, (size_t)fl * sizeof(double2));
+        const size_t o_bf = cs_part(&off, blue ? (size_t)fl * sizeof(double2) : 0);
         const size_t o_x = cs_part(&off, (size_t)N * sizeof(float));
         const size_t o_packed = cs_part(&off, (size_t)n2 * sizeof(float2));
         const size_t o_mean = cs_part(&off, nbk * sizeof(float));
@@ -853,6 +968,7 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
         char *b = (char *)g->d_block;
         g->d_z0 = (double2 *)(b + o_z0);
         g->d_z1 = (double2 *)(b + o_z1);
+        g->d_bf = blue ? (double2 *)(b + o_bf) : nullptr;
         g->d_x = (float *)(b + o_x);
         g->d_packed = (float2 *)(b + o_packed);
         g->d_mean = (float *)(b + o_mean);
@@ -870,6 +986,16 @@ cs_stage_create_on(long long N, double dt, int startwidth, int endwidth, double 
         memcpy(h + t_blen, h_blen.data(), nramp * sizeof(int));
         SCHK(cs_upload(tb, toff, g->st));
     }
+    if (blue) {
+        /* the conjugate chirp's transform, once per plan */
+        const unsigned nb = (unsigned)((fl + 255) / 256);
+        chirp_b_kernel<<<nb, 256, 0, g->st>>>(g->d_z0, n2, fl);
+        SCHK(hipGetLastError());
+        double2 *R = run_fft(g, g->d_z0, false, g->d_z1, g->d_z0, g->st);
+        if (!R) goto fail;
+        SCHK(hipMemcpyAsync(g->d_bf, R, (size_t)fl * sizeof(double2), hipMemcpyDeviceToDevice,
+                            g->st));
+    }
     return g;
 fail:
     cs_gpu_stage_free(g);
@@ -882,32 +1008,60 @@ extern "C" CS_API CsStage *cs_gpu_stage_create(long long N, double dt, int start
     return cs_stage_create_on(N, dt, startwidth, endwidth, endfreq, device, nullptr);
 }
 
-/* Enqueue staging of N host samples into d_out (float2[N/2]) on stream st
- * (the stage's own stream when st is null).  0 on success. */
+/* The complex FFT levels of length fl: level 0 reads `in` (the float series
+ * when floatin), level l writes a then b then a ...; returns the buffer
+ * holding the transform, NULL on a launch error. */
+static double2 *run_fft(CsStage *g, const void *in, bool floatin, double2 *a_buf,
+                        double2 *b_buf, hipStream_t st)
+{
+    const void *src = in;
+    double2 *dst = a_buf;
+    for (int l = 0; l < g->nlev; l++) {
+        const LevelArgs &a = g->lev[l];
+        const bool pow2 = (a.L & (a.L - 1)) == 0, first = l == 0 && floatin,
+                   last = l == g->nlev - 1;
+        const size_t lds = (size_t)a.L * a.ldr * sizeof(double2);
+        const dim3 grid((unsigned)g->grid[l]);
+#define CS_LEVEL(P, F, Z) \
+    fft_level_kernel<P, F, Z><<<grid, kThreads, lds, st>>>(src, dst, g->d_tw, a)
+        if (pow2) {
+            if (first) { if (last) CS_LEVEL(true, true, true); else CS_LEVEL(true, true, false); }
+            else { if (last) CS_LEVEL(true, false, true); else CS_LEVEL(true, false, false); }
+        } else {
+            if (first) { if (last) CS_LEVEL(false, true, true); else CS_LEVEL(false, true, false); }
+            else { if (last) CS_LEVEL(false, false, true); else CS_LEVEL(false, false, false); }
+        }
+#undef CS_LEVEL
+        if (hipGetLastError() != hipSuccess) return nullptr;
+        src = dst;
+        dst = dst == a_buf ? b_buf : a_buf;
+    }
+    return (double2 *)src;
+}
+
 /* the staging kernels (samples already in d_x) */
 static int stage_kernels(CsStage *g, float2 *d_out, hipStream_t st)
 {
     {
-        double2 *src = nullptr, *dst = g->d_z0;
-        for (int l = 0; l < g->nlev; l++) {
-            const LevelArgs &a = g->lev[l];
-            const bool pow2 = (a.L & (a.L - 1)) == 0, first = l == 0, last = l == g->nlev - 1;
-            const void *in = first ? (const void *)g->d_x : (const void *)src;
-            const size_t lds = (size_t)a.L * a.ldr * sizeof(double2);
-            const dim3 grid((unsigned)g->grid[l]);
-#define CS_LEVEL(P, F, Z) \
-    fft_level_kernel<P, F, Z><<<grid, kThreads, lds, st>>>(in, dst, g->d_tw, a)
-            if (pow2) {
-                if (first) { if (last) CS_LEVEL(true, true, true); else CS_LEVEL(true, true, false); }
-                else { if (last) CS_LEVEL(true, false, true); else CS_LEVEL(true, false, false); }
-            } else {
-                if (first) { if (last) CS_LEVEL(false, true, true); else CS_LEVEL(false, true, false); }
-                else { if (last) CS_LEVEL(false, false, true); else CS_LEVEL(false, false, false); }
-            }
-#undef CS_LEVEL
+        double2 *src = nullptr;
+        if (!g->blue) {
+            src = run_fft(g, g->d_x, true, g->d_z0, g->d_z1, st);
+            if (!src) goto fail;
+        } else {
+            const unsigned nb = (unsigned)((g->fl + 255) / 256);
+            chirp_in_kernel<<<nb, 256, 0, st>>>((const float2 *)g->d_x, g->d_z0, g->n2, g->fl);
             SCHK(hipGetLastError());
-            src = dst;
-            dst = dst == g->d_z0 ? g->d_z1 : g->d_z0;
+            double2 *R1 = run_fft(g, g->d_z0, false, g->d_z1, g->d_z0, st);
+            if (!R1) goto fail;
+            chirp_mul_kernel<<<nb, 256, 0, st>>>(R1, g->d_bf, g->fl);
+            SCHK(hipGetLastError());
+            double2 *o1 = R1 == g->d_z0 ? g->d_z1 : g->d_z0;
+            double2 *R2 = run_fft(g, R1, false, o1, R1, st);
+            if (!R2) goto fail;
+            src = R2 == g->d_z0 ? g->d_z1 : g->d_z0;
+            chirp_out_kernel<<<(unsigned)((g->n2 + 255) / 256), 256, 0, st>>>(
+                R2, src, g->n2, 1.0 / (double)g->fl);
+            SCHK(hipGetLastError());
         }
         const long long nq = g->N >> 2;
         const long long nthr = nq > 0 ? nq : 1;

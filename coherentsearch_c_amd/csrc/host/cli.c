@@ -60,7 +60,7 @@ static void usage(FILE *f, int list)
 "      --metricstats   per-block and per-(k, frequency) metric statistics report\n"
 "      --normalize     metric in noise units per (k, frequency window), two passes\n"
 "      --host-staging  realfft + rednoise of .dat inputs on the host (default: on\n"
-"                      the GPU for lengths whose half is 7-smooth)\n"
+"                      the GPU)\n"
 "  -v, --verbose       --threads N  --nowisdom  --wisdomfile F  --progressbar\n"
 "      --noprogress    --noplot     --help\n");
 }

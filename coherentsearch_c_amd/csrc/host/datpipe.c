@@ -7,8 +7,7 @@
  * fast inverse square root and the lower-median quickselect).  The complex
  * transform is this library's double-precision FFT rounded to float where the
  * reference uses FFTW in single precision.  With datpipe_set_device the same
- * staging runs on the GPU (cs_gpu_stage_*), for lengths whose half has no
- * prime factor above 7; other lengths stay on the host.
+ * staging runs on the GPU (cs_gpu_stage_*) for any even length.
  *
  * The reference compiles datpipe.c with -ffp-contract=fast
  * (src/Makefile:129-133); the products gcc fuses there are fused here with
@@ -298,21 +297,11 @@ static int host_buffers(DatPipe *p)
     return p->work && p->scratch && p->packed ? 0 : -1;
 }
 
-/* 1 when every prime factor of n is at most 7 (the device transform's plans) */
-static int smooth7(long long n)
-{
-    if (n < 1) return 0;
-    for (int f = 2; f <= 7; f++)
-        while (n % f == 0) n /= f;
-    return n == 1;
-}
-
 /* realfft + rednoise on the GPU into p->red (cs_gpu_stage_*, the stage plan
- * kept while N and dt repeat): 0 done, 1 when the device transform has no plan
- * for this length (the host transform stages it), -1 on a device error */
+ * kept while N and dt repeat; any even N, chirp-z on the device when N/2 has
+ * a prime factor above 7): 0 done, -1 on a device error */
 static int stage_device(DatPipe *p, const float *samples, long long N, double dt)
 {
-    if (!smooth7(N / 2)) return 1;
     if (!p->gst || p->gN != N || p->gdt != dt) {
         if (p->gst) cs_gpu_stage_free(p->gst);
         p->gst = cs_gpu_stage_create(N, dt, p->startwidth, p->endwidth, p->endfreq, p->device);

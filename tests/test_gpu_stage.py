@@ -28,9 +28,13 @@ def say(*a):
     sys.stdout.flush()
 
 
-# even N with N/2 7-smooth: powers of two, mixed radices, 1-, 2- and 3-level plans
+# even N with N/2 7-smooth: powers of two, mixed radices, 1-, 2- and 3-level plans;
+# then N/2 with a factor 11 or 13 (a choose_good_N length among them), a prime
+# above 1000 and a prime N/2: Bluestein's chirp-z on the device
 SIZES = [1 << 12, 2 * 3 * 5 * 7 * 64, 1 << 16, 2 * 3**5 * 2**8, 2 * 7**3 * 2**9, 1 << 20,
-         2 * 3 * 5 * 7 * 2**13]
+         2 * 3 * 5 * 7 * 2**13,
+         2 * 11 * 2**10, 352000, 2 * 13 * 3 * 2**9, 2 * 1009 * 64, 2 * 100003,
+         2 * 11 * 13 * 17 * 64, 2 * 1048573]
 
 
 @pytest.mark.parametrize("N", SIZES)
@@ -85,3 +89,25 @@ def test_search_of_device_staged_spectrum(oracle):
     sess.close()
     hs.close()
     g.close()
+
+
+def test_engine_request_at_a_good_length_with_factor_11(oracle):
+    """A request whose length is one prepsubband's choose_good_N picks
+    (N/2 = 2^7 5^3 11) stages on the device and returns the oracle's list of
+    the staged spectrum, which the host staging matches within 1e-4."""
+    from coherentsearch_c_amd import abi, engine as eng
+    N = abi.host().cs_choose_good_N(348468)
+    assert N == 352000 and (N // 2) % 11 == 0
+    x = synth.series_with_pulsar(N, 3.1e-3, 0.04, 15.0, 77)
+    p = cs.make_params(nharms=32, m=16, maxdecim=6)
+    with eng.Engine(nslots=1) as E:
+        raw, info, spec = E.wait(E.submit(x, synth.DT, p, threshold=6.0, blocksize=8192))
+    assert info["status"] == 0
+    o = oracle.Oracle(spec.amps, spec.N, spec.T, oracle.config(nharms=32, m=16, blocksize=8192))
+    want = o.search(6.0, 1)
+    o.close()
+    say("good length", N, "candidates", raw.size)
+    assert raw.size == want.size and np.array_equal(raw["metric"], want["metric"])
+    ref = oracle.realfft_rednoise(x, synth.DT)
+    d = np.abs(spec.amps - ref)
+    assert np.max(d) <= 1e-5 * np.max(np.abs(ref))
