@@ -328,11 +328,16 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
      * LDS; a clipped row's channel levels replace its samples where the sums
      * read them */
     __shared__ int rblk[CLIP ? kRows + kMaxClipDelay : 1];
+    int anyclip = 0;
     if constexpr (CLIP) {
+        int mine = 0;
         for (int r = (int)threadIdx.x; r < nr; r += kRows) {
             const long long t = t0 + r;
             rblk[r] = t < rows ? clip_block(cv, cv.row0 + t) : -1;
+            mine |= rblk[r] >= 0;
         }
+        /* the tile's rows all unclipped (the usual case): the plain sums */
+        anyclip = __syncthreads_or(mine);
     }
     {
         /* loads per row nv; the lanes take rpi rows per step, the division by
@@ -375,8 +380,10 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
             const int d = idisp[c0 + ch];
             float x = (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
             if constexpr (CLIP) {
-                const int b = rblk[r + d];
-                if (b >= 0) x = cv.levels[(long long)b * nchan + c0 + ch];
+                if (anyclip) {
+                    const int b = rblk[r + d];
+                    if (b >= 0) x = cv.levels[(long long)b * nchan + c0 + ch];
+                }
             }
             v += x;
         }
@@ -471,6 +478,8 @@ dm_tiled(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, in
  * in row order), then the scan (one workgroup).  The subband kernels then
  * read a clipped row's channel levels in place of its samples. */
 
+constexpr int kClipRcpMax = 4096;     /* block lengths whose reciprocals fit an LDS table */
+
 struct BlockStat {
     double avg, std;       /* of the good rows' zero-DM sums */
     float lo, hi;          /* the good range, median -/+ 3 std of all rows */
@@ -493,11 +502,29 @@ __device__ __forceinline__ float key_float(unsigned k)
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
+/* LDS bytes of clip_zdm_kernel: the row tile, or the keys of the sort */
+template <typename T>
+constexpr int zdm_pitch()
+{
+    return (sizeof(T) == 1 ? 64 : 32) * (int)sizeof(T) + 4;      /* 17 or 33 dwords */
+}
+template <typename T>
+static size_t zdm_smem(int P)
+{
+    int n2 = 1;
+    while (n2 < P) n2 <<= 1;
+    const size_t tile = 256 * (size_t)zdm_pitch<T>(), keys = P <= 8192 ? (size_t)n2 * 4 : 0;
+    return tile > keys ? tile : keys;
+}
+
 /* zero-DM sums (float, channel order) and their element of rank (P-1)/2 by
  * an 8-bit radix select; one raw block per workgroup.  The sums go through
  * LDS: 256 rows x 64 (float: 32) channels at a time are loaded coalesced (rows
  * padded to an odd number of dwords), then lane r adds row r's channels in order,
- * its sum carried from one channel chunk to the next. */
+ * its sum carried from one channel chunk to the next.  Where every row loads
+ * as 16-byte vectors, the next tile's loads are in flight (in registers)
+ * while the lanes add the current one.  LDS is sized to the block (dynamic),
+ * so blocks of 2048 rows take 17 KB and eight share a CU. */
 template <typename T>
 __global__ void __launch_bounds__(256)
 clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int vec16,
@@ -506,52 +533,80 @@ clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int
     __shared__ unsigned hist[256];
     __shared__ unsigned s_prefix, s_k;
     constexpr int CW = sizeof(T) == 1 ? 64 : 32;             /* channels per chunk */
-    constexpr int PITCH = CW * (int)sizeof(T) + 4;           /* bytes: 17 or 33 dwords */
+    constexpr int PITCH = zdm_pitch<T>();
     constexpr int NSORT = 8192;                              /* blocks sorted in LDS */
-    constexpr int SMEM = 256 * PITCH > NSORT * 4 ? 256 * PITCH : NSORT * 4;
     /* the row tile of the sums, then the keys of the sort */
-    __shared__ __align__(16) unsigned char smem[SMEM];
+    extern __shared__ __align__(16) unsigned char smem[];
     unsigned char *rows_ = smem;
     const int b = (int)blockIdx.x;
     const T *blk = chunk + (long long)b * P * nch;
     float *z = zdm + (blk0 + b) * P;
-    for (int i0 = 0; i0 < P; i0 += 256) {
-        const int nrow = min(256, P - i0);
+    if (vec16 && nch % CW == 0) {
+        /* 16 bytes a lane: the address arithmetic of one load serves 16
+         * (float: 4) samples; four dword stores, as the padded rows are
+         * 4-byte aligned.  Tiles in order (row chunk, then channel chunk). */
+        constexpr int NV = CW * (int)sizeof(T) / 16, RPI = 256 / NV, NL = 256 / RPI;
+        const int lr = (int)threadIdx.x / NV, lv = (int)threadIdx.x % NV;
+        const int nc0 = nch / CW, nt = (P + 255) / 256 * nc0;
+        uint4 w[NL];
+        auto load = [&](int t) {
+            const int i0 = t / nc0 * 256, c0 = t % nc0 * CW, nrow = min(256, P - i0);
+            /* rows past the block's end load its last row (unconditional
+             * loads, which the waits can count) and are not stored */
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                const int r = min(lr + k * RPI, nrow - 1);
+                w[k] = *(const uint4 *)((const unsigned char *)(blk + (long long)(i0 + r) * nch +
+                                                                c0) + lv * 16);
+            }
+        };
+        load(0);
         float sum = 0.0f;
-        for (int c0 = 0; c0 < nch; c0 += CW) {
-            const int cw = min(CW, nch - c0);
+        for (int t = 0; t < nt; t++) {
+            const int i0 = t / nc0 * 256, nrow = min(256, P - i0);
             __syncthreads();
-            if (vec16 && cw == CW) {
-                /* 16 bytes a lane: the address arithmetic of one load serves
-                 * 16 (float: 4) samples; four dword stores, as the padded
-                 * rows are 4-byte aligned */
-                constexpr int NV = CW * (int)sizeof(T) / 16, RPI = 256 / NV;
-                const int lr = (int)threadIdx.x / NV, lv = (int)threadIdx.x % NV;
-                for (int r = lr; r < nrow; r += RPI) {
-                    const uint4 w = *(const uint4 *)((const unsigned char *)(
-                                                         blk + (long long)(i0 + r) * nch + c0) +
-                                                     lv * 16);
+#pragma unroll
+            for (int k = 0; k < NL; k++) {
+                const int r = lr + k * RPI;
+                if (r < nrow) {
                     uint32_t *dst = (uint32_t *)(rows_ + r * PITCH + lv * 16);
-                    dst[0] = w.x;
-                    dst[1] = w.y;
-                    dst[2] = w.z;
-                    dst[3] = w.w;
+                    dst[0] = w[k].x;
+                    dst[1] = w[k].y;
+                    dst[2] = w[k].z;
+                    dst[3] = w[k].w;
                 }
-            } else {
+            }
+            __syncthreads();
+            if (t + 1 < nt) load(t + 1);
+            if (t % nc0 == 0) sum = 0.0f;
+            if ((int)threadIdx.x < nrow) {
+                const unsigned char *row = rows_ + threadIdx.x * PITCH;
+#pragma unroll 16
+                for (int e = 0; e < CW; e++) sum += (float)*(const T *)(row + e * (int)sizeof(T));
+                if (t % nc0 == nc0 - 1) z[i0 + threadIdx.x] = sum;
+            }
+        }
+    } else {
+        for (int i0 = 0; i0 < P; i0 += 256) {
+            const int nrow = min(256, P - i0);
+            float sum = 0.0f;
+            for (int c0 = 0; c0 < nch; c0 += CW) {
+                const int cw = min(CW, nch - c0);
+                __syncthreads();
                 for (int idx = (int)threadIdx.x; idx < nrow * CW; idx += 256) {
                     const int r = idx / CW, e = idx % CW;
                     if (e < cw)
                         *(T *)(rows_ + r * PITCH + e * (int)sizeof(T)) =
                             blk[(long long)(i0 + r) * nch + c0 + e];
                 }
+                __syncthreads();
+                if ((int)threadIdx.x < nrow) {
+                    const unsigned char *row = rows_ + threadIdx.x * PITCH;
+                    for (int e = 0; e < cw; e++) sum += (float)*(const T *)(row + e * (int)sizeof(T));
+                }
             }
-            __syncthreads();
-            if ((int)threadIdx.x < nrow) {
-                const unsigned char *row = rows_ + threadIdx.x * PITCH;
-                for (int e = 0; e < cw; e++) sum += (float)*(const T *)(row + e * (int)sizeof(T));
-            }
+            if ((int)threadIdx.x < nrow) z[i0 + threadIdx.x] = sum;
         }
-        if ((int)threadIdx.x < nrow) z[i0 + threadIdx.x] = sum;
     }
     /* the median: blocks of up to 8192 spectra by a bitonic sort of their
      * order-preserving keys in LDS (sums of nearby values would pile the
@@ -625,10 +680,119 @@ __device__ __forceinline__ void as52_step(double x, int &n, double &m, double &v
     n++;
 }
 
+/* d / a correctly rounded, given y = RN(1/a): q = RN(d y) is within an ulp
+ * of d / a, the residual d - q a is exact, and RN(q + r y) is the correctly
+ * rounded quotient (Markstein, IBM J. Res. Dev. 34 (1990) 111-119; the
+ * theorem needs y within half an ulp of 1/a and no underflow, which the
+ * zero-DM sums and divisors up to a block length give).  Three dependent
+ * operations where a division takes a dozen. */
+__device__ __forceinline__ double div_rcp(double d, double a, double y)
+{
+    const double q = d * y;
+    const double r = __builtin_fma(-q, a, d);
+    return __builtin_fma(r, y, q);
+}
+
+/* as52_step after the first sample (n >= 1); y = RN(1 / (n + 1)) */
+__device__ __forceinline__ void as52_next(double x, int &n, double &m, double &v, double &an1,
+                                          double y)
+{
+    const double an = (double)(n + 1);
+    an1 = (double)n;
+    const double dx = div_rcp(x - m, an, y);
+    v += an * an1 * dx * dx;
+    m += dx;
+    n++;
+}
+
+/* as52_step of x where take holds, else nothing; by selects, so a run of
+ * steps has no branch (the arithmetic of a taken step is as52_step's);
+ * y = RN(1 / (n + 1)) */
+__device__ __forceinline__ void as52_if(bool take, double x, int &n, double &m, double &v,
+                                        double &an1, double y)
+{
+    const double an = (double)(n + 1), a1 = (double)n;
+    const double dx = div_rcp(x - m, an, y);
+    const double vn = v + an * a1 * dx * dx, mn = m + dx;
+    const bool upd = take && n > 0;
+    m = take ? (n > 0 ? mn : x) : m;
+    v = upd ? vn : v;
+    an1 = upd ? a1 : an1;
+    n += take ? 1 : 0;
+}
+
 /* one lane per block: the statistics of all rows, the good range, and the
- * statistics of the good rows */
-__global__ void clip_seq_kernel(const float *__restrict__ zdm, const float *__restrict__ med,
-                                long long blk0, int nblk, int P, BlockStat *__restrict__ bs)
+ * statistics of the good rows, in sample order.  Every division of AS 52
+ * is by a step count, so the workgroup first tabulates RN(1/k) for k up to
+ * the block length in LDS; the first pass reads its reciprocal as one
+ * broadcast, the second (whose count advances on good rows only) counts its
+ * good rows 16 at a time and gathers their reciprocals ahead of the
+ * recurrence.  The sums are loaded 16 at a time ahead of their steps, and
+ * the first step of the first pass (the mean := the first sum) is peeled
+ * off. */
+__global__ void __launch_bounds__(64) clip_seq_kernel(const float *__restrict__ zdm,
+                                                      const float *__restrict__ med,
+                                                      long long blk0, int nblk, int P,
+                                                      BlockStat *__restrict__ bs)
+{
+    __shared__ double rcp[kClipRcpMax + 1];
+    for (int k = (int)threadIdx.x + 1; k <= P; k += 64) rcp[k] = 1.0 / (double)k;
+    __syncthreads();
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= nblk) return;
+    const float *z = zdm + (blk0 + b) * P;
+    constexpr int G = 16;
+    int n = 1;
+    double m = (double)z[0], v = 0.0, an1 = 0.0;
+    int i = 1;
+    for (; i + G <= P; i += G) {
+        float xs[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) xs[q] = z[i + q];
+#pragma unroll
+        for (int q = 0; q < G; q++) as52_next((double)xs[q], n, m, v, an1, rcp[i + q + 1]);
+    }
+    for (; i < P; i++) as52_next((double)z[i], n, m, v, an1, rcp[i + 1]);
+    if (n > 1) v /= an1;
+    const double sd = sqrt(v);
+    const float md = med[blk0 + b];
+    const float lo = md - 3.0 * sd, hi = md + 3.0 * sd;
+    int g = 0;
+    double gm = 0.0, gv = 0.0, gan1 = 0.0;
+    for (i = 0; i + G <= P; i += G) {
+        float xs[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) xs[q] = z[i + q];
+        bool tk[G];
+        double ys[G];
+        int gg = g;
+#pragma unroll
+        for (int q = 0; q < G; q++) {
+            tk[q] = xs[q] > lo && xs[q] < hi;
+            ys[q] = rcp[gg + 1];
+            gg += tk[q] ? 1 : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < G; q++) as52_if(tk[q], (double)xs[q], g, gm, gv, gan1, ys[q]);
+    }
+    for (; i < P; i++) {
+        const float x = z[i];
+        as52_if(x > lo && x < hi, (double)x, g, gm, gv, gan1, rcp[g + 1]);
+    }
+    if (g > 1) gv /= gan1;
+    BlockStat r;
+    r.avg = gm;
+    r.std = sqrt(gv);
+    r.lo = lo;
+    r.hi = hi;
+    r.ngood = g;
+    r.pad_ = 0;
+    bs[blk0 + b] = r;
+}
+
+/* one lane per block, dividing as written, for blocks longer than the table */
+__global__ void clip_seq_lane_kernel(const float *__restrict__ zdm, const float *__restrict__ med,
+                                     long long blk0, int nblk, int P, BlockStat *__restrict__ bs)
 {
     const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (b >= nblk) return;
@@ -695,6 +859,79 @@ clip_chavg_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch,
     if (c < nch) chavg[(blk0 + b) * nch + c] = st.ngood > 0 ? acc / st.ngood : 0.0;
 }
 
+/* clip_chavg_kernel with four channels a lane: one 4-byte (float: 16-byte)
+ * load per row, masked to zero on bad rows (the same +0.0 addends), each
+ * channel's mean still summed in double in row order.  The rows' masks go
+ * through LDS 2048 at a time. */
+template <typename T>
+__global__ void __launch_bounds__(256)
+clip_chavg4_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch,
+                   const float *__restrict__ zdm, const BlockStat *__restrict__ bs,
+                   double *__restrict__ chavg)
+{
+    using V = typename std::conditional<sizeof(T) == 1, uint32_t, uint4>::type;
+    __shared__ uint32_t gmask[2048];
+    const int c = 4 * (int)(blockIdx.x * blockDim.x + threadIdx.x), b = (int)blockIdx.y;
+    const BlockStat st = bs[blk0 + b];
+    const T *blk = chunk + (long long)b * P * nch;
+    const float *z = zdm + (blk0 + b) * P;
+    const int cc = min(c, nch - 4);             /* lanes past the channels load the last four */
+    const V *col = (const V *)(blk + cc);
+    const long long vstride = nch / 4;          /* V per row */
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i0 = 0; i0 < P; i0 += 2048) {
+        const int n = min(2048, P - i0);
+        __syncthreads();
+        for (int i = (int)threadIdx.x; i < n; i += 256) {
+            const float x = z[i0 + i];
+            gmask[i] = (x > st.lo && x < st.hi) ? 0xFFFFFFFFu : 0u;
+        }
+        __syncthreads();
+        const V *row = (const V *)((const T *)col + (long long)i0 * nch);
+#pragma unroll 8
+        for (int i = 0; i < n; i++) {
+            const uint32_t g = gmask[i];
+            if constexpr (sizeof(T) == 1) {
+                const uint32_t w = row[(long long)i * vstride] & g;
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[e] += (double)((w >> (8 * e)) & 0xFFu);
+            } else {
+                const uint4 w = row[(long long)i * vstride];
+                acc[0] += (double)__uint_as_float(w.x & g);
+                acc[1] += (double)__uint_as_float(w.y & g);
+                acc[2] += (double)__uint_as_float(w.z & g);
+                acc[3] += (double)__uint_as_float(w.w & g);
+            }
+        }
+    }
+    if (c < nch) {
+        double *o = chavg + (blk0 + b) * nch + c;
+#pragma unroll
+        for (int e = 0; e < 4; e++) o[e] = st.ngood > 0 ? acc[e] / st.ngood : 0.0;
+    }
+}
+
+/* one block of clip_times' running state (clipping.c): the zero-DM mean
+ * and std, and one channel's level, each 0.9 old + 0.1 new from the second
+ * block on; a block without good rows carries the old values */
+__device__ __forceinline__ void scan_step(const BlockStat &s, double chmean, int &nread,
+                                          float &ra, float &rs, float &cr)
+{
+    const bool none = s.ngood < 1;
+    const double ca = none ? (double)ra : s.avg, cs = none ? (double)rs : s.std;
+    const double cv = none ? (double)cr : chmean;
+    if (nread) {
+        ra = 0.9 * ra + 0.1 * ca;
+        rs = 0.9 * rs + 0.1 * cs;
+        cr = (float)(0.9 * cr + 0.1 * cv);
+    } else {
+        ra = ca;
+        rs = cs;
+        cr = (float)cv;
+    }
+    nread++;
+}
+
 /* the running state over the blocks [blk0, blk0 + nblk), continuing from
  * `in` into `out` (two buffers: every workgroup reads the state the previous
  * launch left).  One lane per channel walks the blocks: the channel levels'
@@ -719,29 +956,61 @@ clip_scan_kernel(long long blk0, int nblk, int nch, float clip_sigma,
         __syncthreads();
         for (int i = (int)threadIdx.x; i < n; i += 256) sb[i] = bs[blk0 + b0 + i];
         __syncthreads();
-#pragma unroll 8
-        for (int i = 0; i < n; i++) {
-            const long long B = blk0 + b0 + i;
-            const BlockStat s = sb[i];
-            const bool none = s.ngood < 1;
-            const double ca = none ? (double)ra : s.avg, cs = none ? (double)rs : s.std;
-            if (nread) {
-                ra = 0.9 * ra + 0.1 * ca;
-                rs = 0.9 * rs + 0.1 * cs;
-            } else {
-                ra = ca;
-                rs = cs;
+        /* 16 blocks a group: their channel means are loaded a group ahead,
+         * and the group's levels and running values are stored after its
+         * steps.  A wave's loads and stores complete in order, so a store
+         * between a load and its use would make the use wait for the store;
+         * here the wait for a group's means passes over the previous group's
+         * stores.  The loads are unconditional (block and channel indices
+         * clamped into the arrays), so the waits count them exactly; blocks
+         * past the last full group take the plain loop. */
+        constexpr int G = 16;
+        const int nfull = n / G * G;
+        const int cc = min(c, nch - 1);
+        const double *cva = chavg + (blk0 + b0) * nch + cc;
+        /* one group's steps on the means in cur, loading the next group's
+         * into nxt; two arrays in turn, so no copy waits for the loads */
+        auto group = [&](const double (&cur)[G], double (&nxt)[G], int i0) {
+#pragma unroll
+            for (int q = 0; q < G; q++) nxt[q] = cva[(long long)min(i0 + G + q, n - 1) * nch];
+            float lv[G], av[G], tv[G];
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                scan_step(sb[i0 + q], cur[q], nread, ra, rs, cr);
+                lv[q] = cr;
+                av[q] = ra;
+                tv[q] = clip_sigma * rs;
             }
+            const long long B0 = blk0 + b0 + i0;
             if (c < nch) {
-                const double cv = none ? (double)cr : chavg[B * nch + c];
-                cr = nread ? (float)(0.9 * cr + 0.1 * cv) : (float)cv;
-                levels[B * nch + c] = cr;
+#pragma unroll
+                for (int q = 0; q < G; q++) levels[(B0 + q) * nch + c] = lv[q];
             }
+            if (lead) {
+#pragma unroll
+                for (int q = 0; q < G; q++) {
+                    ravg[B0 + q] = av[q];
+                    trig[B0 + q] = tv[q];
+                }
+            }
+        };
+        double ca_[G], cb_[G];
+#pragma unroll
+        for (int q = 0; q < G; q++) ca_[q] = cva[(long long)min(q, n - 1) * nch];
+        int i0 = 0;
+        for (; i0 + 2 * G <= nfull; i0 += 2 * G) {
+            group(ca_, cb_, i0);
+            group(cb_, ca_, i0 + G);
+        }
+        if (i0 < nfull) group(ca_, cb_, i0);
+        for (int i = nfull; i < n; i++) {
+            const long long B = blk0 + b0 + i;
+            scan_step(sb[i], cva[(long long)i * nch], nread, ra, rs, cr);
+            if (c < nch) levels[B * nch + c] = cr;
             if (lead) {
                 ravg[B] = ra;
                 trig[B] = clip_sigma * rs;
             }
-            nread++;
         }
     }
     if (c < nch) crun[c] = cr;
@@ -1097,13 +1366,27 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         const int nb = (int)((r1 - r0) / P);
         const T *src = j < 0 ? spectra : chunk_ptr(j);
         const int vec16 = (nch * (int)sizeof(T)) % 16 == 0 && (uintptr_t)src % 16 == 0;
-        clip_zdm_kernel<T><<<nb, 256, 0, d->st>>>(src, blk0, P, nch, vec16, d->d_zdm, d->d_med);
-        clip_seq_kernel<<<(nb + 63) / 64, 64, 0, d->st>>>(d->d_zdm, d->d_med, blk0, nb, P, d->d_bs);
+        clip_zdm_kernel<T><<<nb, 256, zdm_smem<T>(P), d->st>>>(src, blk0, P, nch, vec16,
+                                                               d->d_zdm, d->d_med);
+        if (P <= kClipRcpMax)
+            clip_seq_kernel<<<(nb + 63) / 64, 64, 0, d->st>>>(d->d_zdm, d->d_med, blk0, nb, P,
+                                                              d->d_bs);
+        else
+            clip_seq_lane_kernel<<<(nb + 63) / 64, 64, 0, d->st>>>(d->d_zdm, d->d_med, blk0, nb, P,
+                                                                   d->d_bs);
+        /* four channels a lane where rows are whole vectors */
+        const bool four = nch % 4 == 0 && (uintptr_t)src % (sizeof(T) == 1 ? 4 : 16) == 0;
         for (int b0 = 0; b0 < nb; b0 += 65535) {           /* grid y: at most 65535 */
             const int n = std::min(65535, nb - b0);
-            clip_chavg_kernel<T><<<dim3((unsigned)((nch + 255) / 256), (unsigned)n), 256, 0,
-                                   d->st>>>(src + (long long)b0 * P * nch, blk0 + b0, P, nch,
-                                            d->d_zdm, d->d_bs, d->d_chavg);
+            const T *s0 = src + (long long)b0 * P * nch;
+            if (four)
+                clip_chavg4_kernel<T><<<dim3((unsigned)((nch / 4 + 255) / 256), (unsigned)n), 256,
+                                        0, d->st>>>(s0, blk0 + b0, P, nch, d->d_zdm, d->d_bs,
+                                                    d->d_chavg);
+            else
+                clip_chavg_kernel<T><<<dim3((unsigned)((nch + 255) / 256), (unsigned)n), 256, 0,
+                                       d->st>>>(s0, blk0 + b0, P, nch, d->d_zdm, d->d_bs,
+                                                d->d_chavg);
         }
         const ClipState *sin = d->d_cst + (nscan & 1);
         ClipState *sout = d->d_cst + ((nscan + 1) & 1);

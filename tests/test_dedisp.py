@@ -211,3 +211,97 @@ def test_clipped_blocked_loop_equals_closed_form(oracle, cfg):
     x, _, _ = oracle.clip_all(c, dirty)
     direct = oracle.dedisp_direct(c, x, n)
     assert np.array_equal(out[:, :n].view(np.uint32), direct.view(np.uint32))
+
+
+MARKSTEIN_C = r"""
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <stdint.h>
+/* the clipping kernels' quotient: q = RN(d y), r = d - q a (fused), RN(q + r y) */
+static double div_rcp(double d, double a, double y)
+{
+    const double q = d * y;
+    const double r = fma(-q, a, d);
+    return fma(r, y, q);
+}
+static uint64_t s = 0x9e3779b97f4a7c15ull;
+static uint64_t next(void) { s ^= s << 13; s ^= s >> 7; s ^= s << 17; return s; }
+static double unit(void) { return (double)(next() >> 11) / 9007199254740992.0; }
+typedef unsigned __int128 u128;
+/* a dividend whose quotient by a lies 1/(2 a') of an ulp from a rounding
+ * midpoint (a' the odd part of a), the hardest case of division by a: with
+ * quotients K in [2^52, 2^53) and d = a' (2K + 1) / 2 -+ 1/2 a multiple of
+ * d's ulp u, which a' (2K + 1) == +-1 (mod 2u) gives.  Its correctly
+ * rounded quotient goes to *want; 0 if the construction left the binade. */
+static int hard_case(int a, double *d, double *want)
+{
+    int sh = 0, ao = a;
+    while (!(ao & 1)) { ao >>= 1; sh++; }
+    const uint64_t K0 = (1ull << 52) + (next() >> 12);
+    int t = 0;
+    while (((u128)ao * K0) >> (53 + t)) t++;          /* d's ulp 2^t */
+    const uint64_t m2u = 2ull << t;                    /* 2u */
+    uint64_t inv = 1;                                  /* ao^-1 mod 2^64 */
+    for (int i = 0; i < 6; i++) inv *= 2 - (uint64_t)ao * inv;
+    const int up = (int)(next() & 1);
+    const uint64_t r = (up ? inv : (0 - inv)) & (m2u - 1);   /* odd residue of 2K + 1 */
+    const uint64_t u = m2u / 2;
+    uint64_t K = (K0 / u) * u + (r - 1) / 2;
+    if (K < (1ull << 52) || K >= (1ull << 53)) return 0;
+    const u128 T = (u128)ao * (2 * (u128)K + 1);          /* == +-1 mod 2u */
+    const u128 D = up ? (T - 1) / 2 : (T + 1) / 2;      /* quotient K + 1/2 -+ 1/(2 ao) */
+    if ((D >> (53 + t)) || !(D >> (52 + t)) || (D & (u - 1))) return 0;
+    const int e = (int)(next() % 120) - 60;
+    *d = ldexp((double)(uint64_t)(D >> t), t + e);
+    *want = ldexp((double)(up ? K : K + 1), e - sh);
+    return 1;
+}
+int main(int argc, char **argv)
+{
+    const int amax = atoi(argv[1]), per = atoi(argv[2]);
+    long long bad = 0, n = 0, gen_bad = 0;
+    for (int a = 1; a <= amax; a++) {
+        const double A = (double)a, y = 1.0 / A;
+        for (int j = 0; j < per; j++) {
+            double d, want;
+            if (j & 1) {
+                if (!hard_case(a, &d, &want)) continue;
+                if (a > 1 && (a & (a - 1)) && d / A != want) gen_bad++;
+            } else {
+                d = ldexp(1.0 + unit(), (int)(next() % 200) - 100);
+            }
+            if (next() & 1) d = -d;
+            n++;
+            if (div_rcp(d, A, y) != d / A) bad++;
+        }
+    }
+    printf("%lld %lld %lld\n", n, bad, gen_bad);
+    return 0;
+}
+"""
+
+
+def test_clip_reciprocal_division_is_correctly_rounded(tmp_path):
+    """The clipping kernels divide AS 52's differences by the step count with
+    a tabulated reciprocal and one fused correction (Markstein).  For every
+    divisor of the table (1 .. 4096) that equals IEEE division, on random
+    dividends over 60 decades and on the hardest dividends for each divisor
+    (quotients 1/(2a) of an ulp from a rounding midpoint, whose rounding the
+    construction fixes independently); the uncorrected product d * RN(1/a)
+    fails on them."""
+    import subprocess
+
+    def run(source, per):
+        src = tmp_path / "markstein.c"
+        src.write_text(source)
+        exe = str(tmp_path / "markstein")
+        subprocess.run(["gcc", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-o", exe,
+                        str(src), "-lm"], check=True)
+        out = subprocess.run([exe, "4096", str(per)], check=True, capture_output=True, text=True)
+        return map(int, out.stdout.split())
+
+    n, bad, gen_bad = run(MARKSTEIN_C, 2000)
+    assert n > 4096 * 1900 and bad == 0 and gen_bad == 0
+    n, bad, _ = run(MARKSTEIN_C.replace("return fma(r, y, q);", "(void)r; return q;"), 200)
+    assert bad > 10000

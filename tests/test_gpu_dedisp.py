@@ -33,6 +33,8 @@ CASES = [
     (32, 900.0, 1.0, 64e-6, 192, 8, 0.0, 2.0, 9, 3, 48, False),
     # raw blocks of 16384 spectra (clipping: the radix-select median)
     (16, 1400.0, 8.0, 1e-4, 16384, 4, 0.0, 10.0, 5, 1, 8, True),
+    # raw blocks of 200 spectra: clipping's group loops end in a partial group
+    (24, 1250.0, 2.0, 64e-6, 200, 8, 10.0, 2.0, 11, 2, 60, True),
 ]
 
 
@@ -90,7 +92,8 @@ def test_series_match_oracle(oracle, c, kernels, chunk, monkeypatch):
 
 
 CLIPPED = [  # (case, passes of 2^16 spectra or one pass)
-    (CASES[0], 0), (CASES[1], 0), (CASES[3], 1 << 16), (CASES[5], 0), (CASES[6], 0)]
+    (CASES[0], 0), (CASES[1], 0), (CASES[3], 1 << 16), (CASES[5], 0), (CASES[6], 0),
+    (CASES[7], 0)]
 
 
 @pytest.mark.parametrize("kernels", ["tiled", "untiled"])
