@@ -467,6 +467,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="engine worker streams")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner (default launch shapes)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only run the 64-file DM sweep legs (prints one JSON line)")
     ap.add_argument("--wide-only", action="store_true",
                     help="only time the nharms 120 / maxdecim 16 and nharms 128 / maxdecim 6 "
                          "searches of the 2^24-sample spectrum (prints one JSON line)")
@@ -490,6 +492,9 @@ def main():
     if args.dedisp_only:
         print(json.dumps({"dedisp": dedisp_timing(steps=max(1, args.steps),
                                                   ref_flow=False)}), flush=True)
+        return
+    if args.sweep_only:
+        print(json.dumps({"dm_sweep": dm_sweep(args.sweep_files, args.slots)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards by request or by "
