@@ -196,21 +196,25 @@ __device__ __forceinline__ float sample(const T *p)
 
 /* Time-domain clipping as the subband kernels see it (clip_times replaces a
  * raw row whose zero-DM sum is more than trig from the running mean by the
- * block's channel levels): zdm == nullptr means no clipping. */
+ * block's channel levels): flag == nullptr means no clipping. */
 struct ClipView {
-    const float *zdm;      /* zero-DM sum of every spectrum (absolute rows) */
-    const float *ravg;     /* per raw block: running mean of the zero-DM sums */
-    const float *trig;     /* per raw block: clip_sigma x running std */
+    const unsigned char *flag; /* 1 for a clipped spectrum (absolute rows) */
     const float *levels;   /* [block][numchan]: the channel levels */
     long long row0;        /* absolute row of the pass's chunk start */
     int P;                 /* raw block length */
 };
 
-/* the block of a clipped row, or -1 */
-__device__ __forceinline__ int clip_block(const ClipView &cv, long long row)
+/* the clip test of every spectrum of raw blocks [row0 / P, (row0 + n) / P),
+ * once: |zero-DM sum - running mean| > clip_sigma x running std */
+__global__ void __launch_bounds__(256)
+clip_flag_kernel(const float *__restrict__ zdm, const float *__restrict__ ravg,
+                 const float *__restrict__ trig, long long row0, long long n, int P,
+                 unsigned char *__restrict__ flag)
 {
-    const int b = (int)(row / cv.P);
-    return fabsf(cv.zdm[row] - cv.ravg[b]) > cv.trig[b] ? b : -1;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long long row = row0 + i, b = row / P;
+    flag[row] = fabsf(zdm[row] - ravg[b]) > trig[b] ? 1 : 0;
 }
 
 /* One downsampled subband sample per lane: wave w of the workgroup takes
@@ -234,8 +238,8 @@ subband_kernel(const T *__restrict__ ra, const T *__restrict__ rb, long long chu
             const int ch = c0 + j;
             const long long t = t0 + idisp[ch];
             const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
-            const int b = cv.zdm ? clip_block(cv, cv.row0 + t) : -1;
-            v += b >= 0 ? cv.levels[(long long)b * nchan + ch] : sample(row + ch);
+            const long long at = cv.row0 + t;
+            v += cv.flag && cv.flag[at] ? cv.levels[at / cv.P * nchan + ch] : sample(row + ch);
         }
         acc = ds > 1 ? acc + v : v;
     }
@@ -270,7 +274,6 @@ constexpr int kDmThreads = 256;
 constexpr int kGMax = 32;             /* DMs per DM tile (register accumulators) */
 constexpr int kSpanFloats = 8192;     /* LDS floats of one subband batch of a DM tile */
 constexpr int kDmWorkgroups = 2048;   /* DM tiles per pass worth keeping the GPU busy with */
-constexpr int kMaxClipDelay = 2048;   /* deepest subband delay the clipped tiles take */
 
 template <int VEC>
 struct Vec;
@@ -324,17 +327,17 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
     const int c0 = s0 * cps;
     const long long t0 = (long long)blockIdx.y * kRows;
     const int nr = kRows + maxd;
-    /* with clipping: each row's clip test once (its block, or -1), kept in
-     * LDS; a clipped row's channel levels replace its samples where the sums
-     * read them */
-    __shared__ int rblk[CLIP ? kRows + kMaxClipDelay : 1];
+    /* with clipping: the rows' clip flags in LDS after the tile; a clipped
+     * row's channel levels replace its samples where the sums read them */
+    unsigned char *rflag = tile + (size_t)nr * pitch;
     int anyclip = 0;
     if constexpr (CLIP) {
         int mine = 0;
         for (int r = (int)threadIdx.x; r < nr; r += kRows) {
             const long long t = t0 + r;
-            rblk[r] = t < rows ? clip_block(cv, cv.row0 + t) : -1;
-            mine |= rblk[r] >= 0;
+            const unsigned char f = t < rows ? cv.flag[cv.row0 + t] : 0;
+            rflag[r] = f;
+            mine |= f;
         }
         /* the tile's rows all unclipped (the usual case): the plain sums */
         anyclip = __syncthreads_or(mine);
@@ -373,28 +376,37 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
     const int r = (int)threadIdx.x;
     const int k = r % ds, base = (int)(threadIdx.x & 63) - k;
     const long long u = (t0 + r) / ds;
-    for (int s = 0; s < ns; s++) {
-        float v = 0.0f;
-        for (int j = 0; j < cps; j++) {
-            const int ch = s * cps + j;
-            const int d = idisp[c0 + ch];
-            float x = (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
-            if constexpr (CLIP) {
-                if (anyclip) {
-                    const int b = rblk[r + d];
-                    if (b >= 0) x = cv.levels[(long long)b * nchan + c0 + ch];
+    /* the sums, with or without the clip test (the workgroup takes one
+     * version: its loop stays as tight as the unclipped kernel's) */
+    auto sums = [&](auto withclip) {
+        for (int s = 0; s < ns; s++) {
+            float v = 0.0f;
+            for (int j = 0; j < cps; j++) {
+                const int ch = s * cps + j;
+                const int d = idisp[c0 + ch];
+                float x =
+                    (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
+                if constexpr (decltype(withclip)::value) {
+                    if (rflag[r + d]) {
+                        const long long b = (cv.row0 + t0 + r + d) / cv.P;
+                        x = cv.levels[b * nchan + c0 + ch];
+                    }
                 }
+                v += x;
             }
-            v += x;
+            float acc = v;
+            if (ds > 1) {
+                acc = 0.0f;
+                for (int q = 0; q < ds; q++) acc += __shfl(v, base + q, 64);
+                acc /= (float)ds;
+            }
+            if (k == 0 && u < nu) out[(long long)(s0 + s) * dsc + u] = acc;
         }
-        float acc = v;
-        if (ds > 1) {
-            acc = 0.0f;
-            for (int q = 0; q < ds; q++) acc += __shfl(v, base + q, 64);
-            acc /= (float)ds;
-        }
-        if (k == 0 && u < nu) out[(long long)(s0 + s) * dsc + u] = acc;
-    }
+    };
+    if (CLIP && __builtin_amdgcn_readfirstlane(anyclip))
+        sums(std::true_type{});
+    else
+        sums(std::false_type{});
 }
 
 /* DM samples from LDS spans: the workgroup takes 512 consecutive samples of
@@ -1056,6 +1068,7 @@ struct CsDedisp {
     /* clipping (clip_sigma > 0): per spectrum, per raw block, state */
     float *d_zdm = nullptr, *d_med = nullptr, *d_ravg = nullptr, *d_trig = nullptr;
     float *d_levels = nullptr, *d_crun = nullptr;
+    unsigned char *d_cflag = nullptr;
     double *d_chavg = nullptr;
     BlockStat *d_bs = nullptr;
     ClipState *d_cst = nullptr;
@@ -1110,7 +1123,7 @@ extern "C" CS_API void cs_gpu_dedisp_free(CsDedisp *d)
     hipFree(d->d_gtab);
     for (void *q : {(void *)d->d_zdm, (void *)d->d_med, (void *)d->d_ravg, (void *)d->d_trig,
                     (void *)d->d_levels, (void *)d->d_crun, (void *)d->d_chavg, (void *)d->d_bs,
-                    (void *)d->d_cst})
+                    (void *)d->d_cst, (void *)d->d_cflag})
         hipFree(q);
     if (d->h_dm0) hipHostFree(d->h_dm0);
     if (d->st) hipStreamDestroy(d->st);
@@ -1166,6 +1179,7 @@ extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long l
         if (c->clip_sigma > 0.0f) {
             const size_t nb = (size_t)(nspec / P), nc = (size_t)c->numchan;
             DCHK(hipMalloc(&d->d_zdm, sizeof(float) * (size_t)nspec));
+            DCHK(hipMalloc(&d->d_cflag, (size_t)nspec));
             DCHK(hipMalloc(&d->d_med, sizeof(float) * nb));
             DCHK(hipMalloc(&d->d_ravg, sizeof(float) * nb));
             DCHK(hipMalloc(&d->d_trig, sizeof(float) * nb));
@@ -1347,9 +1361,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
     auto clip_view = [&](long long j) {
         ClipView v{};
         if (clip) {
-            v.zdm = d->d_zdm;
-            v.ravg = d->d_ravg;
-            v.trig = d->d_trig;
+            v.flag = d->d_cflag;
             v.levels = d->d_levels;
             v.row0 = j * C;
             v.P = P;
@@ -1394,6 +1406,9 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         clip_scan_kernel<<<(nch + 255) / 256, 256, 0, d->st>>>(
             blk0, nb, nch, c.clip_sigma, d->d_bs, d->d_chavg, sin, sout, d->d_crun, d->d_ravg,
             d->d_trig, d->d_levels);
+        const long long nrow = (long long)nb * P;
+        clip_flag_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, d->st>>>(
+            d->d_zdm, d->d_ravg, d->d_trig, blk0 * P, nrow, P, d->d_cflag);
         return hipGetLastError();
     };
     /* the subband pass of chunk j: nu downsampled samples into `sub` */
@@ -1406,7 +1421,8 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             const long long nrow = nu * ds, here = std::min(rows - j * C, nrow + t.maxd);
             const dim3 grid((unsigned)((nsub + sb_gs - 1) / sb_gs),
                             (unsigned)((nrow + kRows - 1) / kRows));
-            const size_t lds = (size_t)(kRows + t.maxd) * sb_pitch;
+            const size_t lds = (size_t)(kRows + t.maxd) * sb_pitch +
+                               (clip ? (size_t)(kRows + t.maxd + 3) / 4 * 4 : 0);
 #define CS_SUB(V, K)                                                                       \
     subband_tiled<T, V, K><<<grid, kRows, lds, d->st>>>(ra, rb, C, nch, cps, sb_gs, nsub, ds, \
                                                         d->d_idisp, t.maxd, sb_pitch, here, \
@@ -1503,7 +1519,7 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
                            (!on_device || (uintptr_t)spectra % 4 == 0)
                      ? 4
                      : 1;
-        for (int gs = nsub; gs >= 1 && !(clip && t.maxd > kMaxClipDelay); gs--) {
+        for (int gs = nsub; gs >= 1; gs--) {
             const int pitch = (gs * cps * es + 15) / 16 * 16 + 4;    /* odd dwords */
             if ((long long)(kRows + t.maxd) * pitch <= kTileBytes) {
                 sb_gs = gs;
