@@ -514,6 +514,26 @@ __device__ __forceinline__ float key_float(unsigned k)
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
+/* ascending bitonic sort of n2 (a power of two) keys in LDS by a workgroup
+ * of 256 lanes; the keys are in place when every lane returns */
+__device__ __forceinline__ void bitonic_sort_256(unsigned *keys, int n2)
+{
+    for (int kk = 2; kk <= n2; kk <<= 1)
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            for (int i = (int)threadIdx.x; i < n2; i += 256) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned a = keys[i], c = keys[ixj];
+                    if ((a > c) == ((i & kk) == 0)) {
+                        keys[i] = c;
+                        keys[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
 /* LDS bytes of clip_zdm_kernel: the row tile, or the keys of the sort */
 template <typename T>
 constexpr int zdm_pitch()
@@ -631,20 +651,7 @@ clip_zdm_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch, int
         for (int i = (int)threadIdx.x; i < n2; i += 256)
             keys[i] = i < P ? order_key(z[i]) : 0xFFFFFFFFu;
         __syncthreads();
-        for (int kk = 2; kk <= n2; kk <<= 1)
-            for (int j = kk >> 1; j > 0; j >>= 1) {
-                for (int i = (int)threadIdx.x; i < n2; i += 256) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const unsigned a = keys[i], c = keys[ixj];
-                        if ((a > c) == ((i & kk) == 0)) {
-                            keys[i] = c;
-                            keys[ixj] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
+        bitonic_sort_256(keys, n2);
         if (threadIdx.x == 0) med[blk0 + b] = key_float(keys[(P - 1) / 2]);
         return;
     }
@@ -923,6 +930,143 @@ clip_chavg4_kernel(const T *__restrict__ chunk, long long blk0, int P, int nch,
     }
 }
 
+/* uint8 spectra: every sum clip_times takes of samples is an integer -- a
+ * spectrum's zero-DM sum below 2^24 (numchan <= 4096), a channel's block sum
+ * below 2^32 -- so its float and double additions are exact in any order.
+ * One read of a block then gives its zero-DM sums, their lower median (an
+ * integer sort in LDS: the order of non-negative floats), and every
+ * channel's sum over all the block's rows; clip_badrows_u8_kernel takes the
+ * bad rows' samples off for the means of the good rows.  NV = numchan / 16
+ * lanes per spectrum (16 bytes each), a power of two up to 256; a row's
+ * lanes add their byte sums by shuffles (and, past one wave, LDS atomics). */
+template <int NV>
+__global__ void __launch_bounds__(256)
+clip_zdm_u8_kernel(const uint8_t *__restrict__ chunk, long long blk0, int P, int nch,
+                   float *__restrict__ zdm, float *__restrict__ med, uint32_t *__restrict__ chsum)
+{
+    extern __shared__ uint32_t zsm[];    /* n2 row sums (the sort's keys), then nch channel sums */
+    constexpr int RPI = NV >= 256 ? 1 : 256 / NV, U = 4, GL = NV < 64 ? NV : 64;
+    const int b = (int)blockIdx.x, tid = (int)threadIdx.x;
+    const uint8_t *blk = chunk + (long long)b * P * nch;
+    int n2 = 1;
+    while (n2 < P) n2 <<= 1;
+    uint32_t *cs = zsm + n2;
+    for (int i = tid; i < nch; i += 256) cs[i] = 0;
+    if constexpr (NV > 64)
+        for (int i = tid; i < P; i += 256) zsm[i] = 0;
+    __syncthreads();
+    const int lr = tid / NV, lv = tid % NV;
+    uint32_t acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] = 0;
+    for (int r0 = 0; r0 < P; r0 += RPI * U) {
+        /* U rows a lane in flight; rows past the block load its last row,
+         * which is not counted */
+        uint4 w[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int r = min(r0 + q * RPI + lr, P - 1);
+            w[q] = *(const uint4 *)(blk + (long long)r * nch + lv * 16);
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const int r = r0 + q * RPI + lr;
+            const bool ok = r < P;
+            const uint32_t keep = ok ? 0xFFFFFFFFu : 0u;
+            const uint32_t wv[4] = {w[q].x, w[q].y, w[q].z, w[q].w};
+            uint32_t sum = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                sum = __builtin_amdgcn_sad_u8(wv[k], 0u, sum);
+                const uint32_t x = wv[k] & keep;
+#pragma unroll
+                for (int e = 0; e < 4; e++) acc[4 * k + e] += (x >> (8 * e)) & 0xFFu;
+            }
+#pragma unroll
+            for (int o = 1; o < GL; o <<= 1) sum += __shfl_xor(sum, o, 64);
+            if constexpr (NV <= 64) {
+                if (lv == 0 && ok) zsm[r] = sum;
+            } else {
+                if ((lv & 63) == 0 && ok) atomicAdd(&zsm[r], sum);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++) atomicAdd(&cs[lv * 16 + j], acc[j]);
+    __syncthreads();
+    for (int i = tid; i < nch; i += 256) chsum[(blk0 + b) * nch + i] = cs[i];
+    float *z = zdm + (blk0 + b) * P;
+    for (int i = tid; i < n2; i += 256) {
+        if (i < P)
+            z[i] = (float)zsm[i];
+        else
+            zsm[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    bitonic_sort_256(zsm, n2);
+    if (tid == 0) med[blk0 + b] = (float)zsm[(P - 1) / 2];
+}
+
+constexpr int kBadWords = 8192 / 64;    /* bad-row masks of a block of up to 8192 spectra */
+
+/* the channel means over each block's good rows from the all-row sums: the
+ * bad rows (zero-DM sum outside (lo, hi)) are marked 64 at a time by ballot,
+ * then every lane walks the marks (the same for all lanes) and takes each bad
+ * row's samples off its four channels, four rows' loads at a time */
+__global__ void __launch_bounds__(256)
+clip_badrows_u8_kernel(const uint8_t *__restrict__ chunk, long long blk0, int P, int nch,
+                       const float *__restrict__ zdm, const BlockStat *__restrict__ bs,
+                       const uint32_t *__restrict__ chsum, double *__restrict__ chavg)
+{
+    __shared__ unsigned long long bm[kBadWords];
+    const int b = (int)blockIdx.x, tid = (int)threadIdx.x;
+    const BlockStat st = bs[blk0 + b];
+    const float *z = zdm + (blk0 + b) * P;
+    const int nw = (P + 63) / 64;
+    for (int w0 = 0; w0 < nw; w0 += 4) {
+        const int wi = w0 + tid / 64, i = wi * 64 + (tid & 63);
+        bool bad = false;
+        if (i < P) {
+            const float x = z[i];
+            bad = !(x > st.lo && x < st.hi);
+        }
+        const unsigned long long m = __ballot(bad);
+        if (wi < nw && (tid & 63) == 0) bm[wi] = m;
+    }
+    __syncthreads();
+    const uint8_t *blk = chunk + (long long)b * P * nch;
+    for (int c = 4 * tid; c < nch; c += 1024) {
+        uint32_t sb[4] = {0, 0, 0, 0};
+        if (st.ngood > 0) {
+            int wi = 0;
+            unsigned long long m = bm[0];
+            for (;;) {
+                int idx[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    while (!m && wi + 1 < nw) m = bm[++wi];
+                    idx[q] = m ? wi * 64 + __builtin_ctzll(m) : -1;
+                    m &= m - 1;
+                }
+                if (idx[0] < 0) break;
+                uint32_t w[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    w[q] = idx[q] >= 0 ? *(const uint32_t *)(blk + (long long)idx[q] * nch + c) : 0u;
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) sb[e] += (w[q] >> (8 * e)) & 0xFFu;
+            }
+        }
+        const uint32_t *csum = chsum + (blk0 + b) * nch + c;
+        double *o = chavg + (blk0 + b) * nch + c;
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            o[e] = st.ngood > 0 ? (double)(csum[e] - sb[e]) / st.ngood : 0.0;
+    }
+}
+
 /* one block of clip_times' running state (clipping.c): the zero-DM mean
  * and std, and one channel's level, each 0.9 old + 0.1 new from the second
  * block on; a block without good rows carries the old values */
@@ -1069,6 +1213,7 @@ struct CsDedisp {
     float *d_zdm = nullptr, *d_med = nullptr, *d_ravg = nullptr, *d_trig = nullptr;
     float *d_levels = nullptr, *d_crun = nullptr;
     unsigned char *d_cflag = nullptr;
+    uint32_t *d_chsum = nullptr;
     double *d_chavg = nullptr;
     BlockStat *d_bs = nullptr;
     ClipState *d_cst = nullptr;
@@ -1123,7 +1268,7 @@ extern "C" CS_API void cs_gpu_dedisp_free(CsDedisp *d)
     hipFree(d->d_gtab);
     for (void *q : {(void *)d->d_zdm, (void *)d->d_med, (void *)d->d_ravg, (void *)d->d_trig,
                     (void *)d->d_levels, (void *)d->d_crun, (void *)d->d_chavg, (void *)d->d_bs,
-                    (void *)d->d_cst, (void *)d->d_cflag})
+                    (void *)d->d_cst, (void *)d->d_cflag, (void *)d->d_chsum})
         hipFree(q);
     if (d->h_dm0) hipHostFree(d->h_dm0);
     if (d->st) hipStreamDestroy(d->st);
@@ -1186,6 +1331,7 @@ extern "C" CS_API CsDedisp *cs_gpu_dedisp_create(const CsDedispConfig *c, long l
             DCHK(hipMalloc(&d->d_levels, sizeof(float) * nb * nc));
             DCHK(hipMalloc(&d->d_crun, sizeof(float) * nc));
             DCHK(hipMalloc(&d->d_chavg, sizeof(double) * nb * nc));
+            DCHK(hipMalloc(&d->d_chsum, sizeof(uint32_t) * nb * nc));
             DCHK(hipMalloc(&d->d_bs, sizeof(BlockStat) * nb));
             DCHK(hipMalloc(&d->d_cst, 2 * sizeof(ClipState)));
         }
@@ -1378,8 +1524,37 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
         const int nb = (int)((r1 - r0) / P);
         const T *src = j < 0 ? spectra : chunk_ptr(j);
         const int vec16 = (nch * (int)sizeof(T)) % 16 == 0 && (uintptr_t)src % 16 == 0;
-        clip_zdm_kernel<T><<<nb, 256, zdm_smem<T>(P), d->st>>>(src, blk0, P, nch, vec16,
-                                                               d->d_zdm, d->d_med);
+        /* uint8 rows of 16 x 2^k channels (up to 4096) in blocks of up to
+         * 8192: one read for the zero-DM sums and the channel sums */
+        const int nv = nch / 16;
+        const bool fused = sizeof(T) == 1 && nch % 16 == 0 && nv <= 256 && !(nv & (nv - 1)) &&
+                           P <= 8192 && (uintptr_t)src % 16 == 0;
+        if (fused) {
+            if constexpr (sizeof(T) == 1) {
+                int n2 = 1;
+                while (n2 < P) n2 <<= 1;
+                const size_t lds = 4 * ((size_t)n2 + (size_t)nch);
+                const uint8_t *s8 = (const uint8_t *)src;
+#define CS_ZU8(V)                                                                          \
+    clip_zdm_u8_kernel<V><<<nb, 256, lds, d->st>>>(s8, blk0, P, nch, d->d_zdm, d->d_med, \
+                                                   d->d_chsum)
+                switch (nv) {
+                case 1: CS_ZU8(1); break;
+                case 2: CS_ZU8(2); break;
+                case 4: CS_ZU8(4); break;
+                case 8: CS_ZU8(8); break;
+                case 16: CS_ZU8(16); break;
+                case 32: CS_ZU8(32); break;
+                case 64: CS_ZU8(64); break;
+                case 128: CS_ZU8(128); break;
+                default: CS_ZU8(256); break;
+                }
+#undef CS_ZU8
+            }
+        } else {
+            clip_zdm_kernel<T><<<nb, 256, zdm_smem<T>(P), d->st>>>(src, blk0, P, nch, vec16,
+                                                                   d->d_zdm, d->d_med);
+        }
         if (P <= kClipRcpMax)
             clip_seq_kernel<<<(nb + 63) / 64, 64, 0, d->st>>>(d->d_zdm, d->d_med, blk0, nb, P,
                                                               d->d_bs);
@@ -1388,7 +1563,12 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
                                                                    d->d_bs);
         /* four channels a lane where rows are whole vectors */
         const bool four = nch % 4 == 0 && (uintptr_t)src % (sizeof(T) == 1 ? 4 : 16) == 0;
-        for (int b0 = 0; b0 < nb; b0 += 65535) {           /* grid y: at most 65535 */
+        if (fused) {
+            clip_badrows_u8_kernel<<<nb, 256, 0, d->st>>>((const uint8_t *)src, blk0, P, nch,
+                                                          d->d_zdm, d->d_bs, d->d_chsum,
+                                                          d->d_chavg);
+        }
+        for (int b0 = 0; !fused && b0 < nb; b0 += 65535) {  /* grid y: at most 65535 */
             const int n = std::min(65535, nb - b0);
             const T *s0 = src + (long long)b0 * P * nch;
             if (four)

@@ -35,6 +35,10 @@ CASES = [
     (16, 1400.0, 8.0, 1e-4, 16384, 4, 0.0, 10.0, 5, 1, 8, True),
     # raw blocks of 200 spectra: clipping's group loops end in a partial group
     (24, 1250.0, 2.0, 64e-6, 200, 8, 10.0, 2.0, 11, 2, 60, True),
+    # 1024 and 2048 uint8 channels (clipping's one-read statistics, a spectrum
+    # over one wave and over two)
+    (1024, 1200.0, 0.29296875, 64e-6, 2048, 64, 0.0, 2.0, 8, 4, 8, True),
+    (2048, 1200.0, 0.146484375, 64e-6, 512, 32, 0.0, 2.0, 6, 2, 8, True),
 ]
 
 
@@ -93,7 +97,7 @@ def test_series_match_oracle(oracle, c, kernels, chunk, monkeypatch):
 
 CLIPPED = [  # (case, passes of 2^16 spectra or one pass)
     (CASES[0], 0), (CASES[1], 0), (CASES[3], 1 << 16), (CASES[5], 0), (CASES[6], 0),
-    (CASES[7], 0)]
+    (CASES[7], 0), (CASES[8], 0), (CASES[9], 0)]
 
 
 @pytest.mark.parametrize("kernels", ["tiled", "untiled"])
