@@ -250,30 +250,35 @@ def metric_stats_timing(spec, params, reps=3):
     return out
 
 
-def wide_harmonics(spec, steps=5):
+def wide_harmonics(spec, steps=5, general=True):
     """The reference README's wider configurations on the same resident
-    spectrum: nharms 120 with maxdecim 16 (two 64-bit filled words, one wave
-    per SIMD), timed like the headline step."""
+    spectrum, timed like the headline step: nharms 120 with maxdecim 16 and
+    nharms 128 (the register kernels' two-word filled mask, one wave per
+    SIMD); with general=True also -n 240 --maxdecim 16 and --m 64 at
+    nharms 60 (the general kernel, README.md:448, 688-695)."""
     import coherentsearch_c_amd as cs
     out = {}
-    for nh, md in ((120, 16), (128, 6)):
-        p = cs.make_params(nharms=nh, m=16, maxdecim=md)
+    cases = [(120, 16, 16, steps), (128, 6, 16, steps)]
+    if general:
+        cases += [(240, 16, 16, 2), (60, 6, 64, 2)]
+    for nh, md, m, nst in cases:
+        p = cs.make_params(nharms=nh, m=m, maxdecim=md)
         sess = cs.Session(spec, p, blocksize=65536)
         ht, trials, _ = sess.work()
         sess.upload()
         sess.search_resident(8.0)
         kms = []
         t = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(nst):
             sess.search_resident(8.0)
             kms.append(sess.kernel_ms())
-        wall = (time.perf_counter() - t) / steps
+        wall = (time.perf_counter() - t) / nst
         sess.close()
-        out["nharms%d_maxdecim%d" % (nh, md)] = {
-            "trials": trials, "harmonic_trials": ht, "ms_per_search": 1e3 * wall,
-            "kernel_ms": float(np.mean(kms)), "ht_s": ht / wall}
-        log("nharms %d maxdecim %d: %.2f ms per search, %.4g ht/s" % (nh, md, 1e3 * wall,
-                                                                     ht / wall))
+        key = "nharms%d_maxdecim%d" % (nh, md) + ("" if m == 16 else "_m%d" % m)
+        out[key] = {"trials": trials, "harmonic_trials": ht, "ms_per_search": 1e3 * wall,
+                    "kernel_ms": float(np.mean(kms)), "ht_s": ht / wall}
+        log("nharms %d maxdecim %d m %d: %.2f ms per search, %.4g ht/s"
+            % (nh, md, m, 1e3 * wall, ht / wall))
     return out
 
 
@@ -472,6 +477,11 @@ def main():
     ap.add_argument("--wide-only", action="store_true",
                     help="only time the nharms 120 / maxdecim 16 and nharms 128 / maxdecim 6 "
                          "searches of the 2^24-sample spectrum (prints one JSON line)")
+    ap.add_argument("--no-general", action="store_true",
+                    help="with --wide-only: only the register-kernel cases (nharms 120/128)")
+    ap.add_argument("--c2-only", action="store_true",
+                    help="only the headline 2^24-sample search steps and the search_file "
+                         "pass (no autotune, other legs or CPU baseline; for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -516,12 +526,16 @@ def main():
     del x
     log("staging (synth + realfft + rednoise): %.2f s" % (time.perf_counter() - t))
     if args.wide_only:
-        print(json.dumps({"wide_harmonics": wide_harmonics(spec, steps=max(1, args.steps))}),
+        print(json.dumps({"wide_harmonics": wide_harmonics(spec, steps=max(1, args.steps),
+                                                           general=not args.no_general)}),
               flush=True)
         return
     pkw = dict(nharms=60, m=16, maxdecim=6)
     params = cs.make_params(**pkw)
     tune = []
+    if args.c2_only:
+        args.no_tune = args.no_dedisp = args.no_sweep = args.no_serve = True
+        args.no_cpu_baseline = True
     if not args.no_tune:
         # launch shapes of every kernel class the bench runs (the sweep and the
         # served stream mix nharms 16/32/60); later sessions use them
@@ -593,8 +607,9 @@ def main():
                                        "xcd_stripe 0 contiguous / S tiles; blocksize is "
                                        "advice (requests keep 65536)",
                            "classes": tune}
-    out["metric_stats"] = metric_stats_timing(spec, params)
-    out["wide_harmonics"] = wide_harmonics(spec)
+    if not args.c2_only:
+        out["metric_stats"] = metric_stats_timing(spec, params)
+        out["wide_harmonics"] = wide_harmonics(spec)
     if not args.no_dedisp:
         out["dedisp"] = dedisp_timing()
     if not args.no_sweep:

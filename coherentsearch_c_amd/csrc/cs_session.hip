@@ -839,6 +839,8 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     return S;
 fail:
     session_release(S);
+    /* the failed call's error must not be reported by a later launch check */
+    (void)hipGetLastError();
     return NULL;
 }
 

@@ -55,12 +55,16 @@ def _plan(c, nspec, numout=0):
 CHUNKED = [(CASES[3], "tiled"), (CASES[3], "untiled"), (CASES[4], "tiled")]
 
 
-@pytest.mark.parametrize("chunk", [0, 1 << 16])
-@pytest.mark.parametrize("kernels", ["tiled", "untiled"])
-@pytest.mark.parametrize("c", CASES)
+# every case in one device pass with both kernel sets; the CHUNKED ones also
+# in passes of 2^16 spectra
+SERIES = [pytest.param(c, k, 0, id="c%d-%s-0" % (i, k))
+          for i, c in enumerate(CASES) for k in ("tiled", "untiled")] + \
+         [pytest.param(c, k, 1 << 16, id="c%d-%s-65536" % (CASES.index(c), k))
+          for c, k in CHUNKED]
+
+
+@pytest.mark.parametrize("c,kernels,chunk", SERIES)
 def test_series_match_oracle(oracle, c, kernels, chunk, monkeypatch):
-    if chunk and (c, kernels) not in CHUNKED:
-        pytest.skip("one pass at this size")
     # the untiled kernels serve shapes the LDS tiles do not fit
     monkeypatch.setenv("CS_DEDISP_SIMPLE", "1" if kernels == "untiled" else "0")
     monkeypatch.setenv("CS_DEDISP_CHUNK", str(chunk))

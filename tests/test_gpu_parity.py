@@ -247,6 +247,37 @@ def test_session_reuse_and_mismatch(oracle):
     sess.close()
 
 
+@pytest.mark.parametrize("nh,m,precision", [(16, 16, "f32"), (60, 32, "f32"), (32, 16, "f64"),
+                                             (72, 32, "f32")])
+def test_search_region_one_shot(oracle, nh, m, precision):
+    """cs_gpu_search_region (src/cuda.cu:1238-1255): create, run and free in
+    one call; r_hi is ignored.  Its list is the oracle's, trial for trial, and
+    the resident session's."""
+    import ctypes as C
+    spec, p, o, sess = _case(oracle, nh, m, nbins=1 << 14, precision=precision)
+    say("search_region nh", nh, "m", m, precision, "trials", o.total)
+    L = sess.L
+    cl = L.cs_gpu_search_region(C.byref(spec.ft), C.byref(p), C.byref(sess.plans),
+                                sess._cs.ctypes.data_as(C.POINTER(C.c_longlong)), sess.total,
+                                sess.Nprof, sess.nchunks, sess.r_lo, -12345.0, sess.lodr,
+                                -np.inf, 0)
+    got = cs.take_list(cl)
+    want = o.search(-np.inf, 1)
+    assert got.size == want.size
+    for f in ("r", "freq", "nharm", "metric"):
+        assert np.array_equal(got[f], want[f]), f
+    # a device that does not exist: an empty list, as a failed create, and
+    # the library's later searches are unaffected
+    bad = cs.take_list(L.cs_gpu_search_region(
+        C.byref(spec.ft), C.byref(p), C.byref(sess.plans),
+        sess._cs.ctypes.data_as(C.POINTER(C.c_longlong)), sess.total, sess.Nprof,
+        sess.nchunks, sess.r_lo, 0.0, sess.lodr, 8.0, 4096))
+    assert bad.size == 0
+    again = sess.run(-np.inf)
+    assert np.array_equal(again["metric"], got["metric"])
+    sess.close()
+
+
 def test_unsupported_configuration_fails_loudly():
     """Outside the GPU search's range cs_gpu_session_create returns NULL with
     a message: more than 512 harmonics, m above 64, decimations above 16;
