@@ -481,7 +481,8 @@ def main():
                     help="with --wide-only: only the register-kernel cases (nharms 120/128)")
     ap.add_argument("--c2-only", action="store_true",
                     help="only the headline 2^24-sample search steps and the search_file "
-                         "pass (no autotune, other legs or CPU baseline; for profiling)")
+                         "pass, after tuning the nharms 60 class (no other legs or CPU "
+                         "baseline; for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 256 MiB device-to-device copies (a plain copy "
                          "kernel of known size, to scale FETCH_SIZE/WRITE_SIZE counters)")
@@ -533,14 +534,17 @@ def main():
     pkw = dict(nharms=60, m=16, maxdecim=6)
     params = cs.make_params(**pkw)
     tune = []
+    tune_classes = (60, 32, 16)
     if args.c2_only:
-        args.no_tune = args.no_dedisp = args.no_sweep = args.no_serve = True
+        # the headline class's tuned launch shape only
+        tune_classes = (60,)
+        args.no_dedisp = args.no_sweep = args.no_serve = True
         args.no_cpu_baseline = True
     if not args.no_tune:
         # launch shapes of every kernel class the bench runs (the sweep and the
         # served stream mix nharms 16/32/60); later sessions use them
         t = time.perf_counter()
-        for nh in (60, 32, 16):
+        for nh in tune_classes:
             r = cs.autotune(cs.make_params(nharms=nh, m=16, maxdecim=6))
             tune.append({k: r[k] for k in ("nharms", "lane_order", "xcd_stripe", "blocksize",
                                            "ntried", "default_ms", "best_ms")})

@@ -56,8 +56,9 @@ typedef struct {
     X(29) X(30) X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41)     \
     X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54)     \
     X(55) X(56) X(57) X(58) X(59) X(60) X(61) X(62) X(63) X(64)
-/* harmonic counts above 64 (up to 128, where the c2r twiddle table ends),
- * f32 profiles and M = 16 padded taps only: the stack and profiles take
+/* harmonic counts above 64 (up to 128, where the c2r twiddle table ends)
+ * with a register profile pass of the two-pass search (f32 profiles; the
+ * stack pass takes M = 16, 32 or 64 padded taps): the stack and profiles take
  * about 440 registers, one wave per SIMD */
 #define CS_NH_BIG_LIST(X)                                                              \
     X(65) X(66) X(67) X(68) X(69) X(70) X(71) X(72) X(73) X(74) X(75) X(76) X(77)     \

@@ -336,6 +336,9 @@ void worker(CsEngine *E)
     hipSetDevice(E->device);
     hipStream_t st = NULL;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) st = NULL;
+    /* this thread's pinned table-upload buffer at its working size up front:
+     * growing it later (hipHostFree + hipHostMalloc) stalls every stream */
+    (void)cs_upload_buffer((size_t)32 << 20);
     for (;;) {
         EngReq *r = NULL;
         {

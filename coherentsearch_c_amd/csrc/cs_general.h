@@ -89,6 +89,10 @@ struct GenArgs {
     int nvslots;                /* value slots per lane, scratch included */
     void *work;                 /* nvslots values, then nh + 1 stack entries, per lane */
     long long nlanes;           /* lanes of the persistent grid */
+    int given;                  /* 1: the stacks of tiles tile0 .. tile0 + nslice - 1 are in
+                                   the work buffer already (the stack pass of
+                                   split_kernel.hpp), workgroup u takes tile tile0 + u */
+    int tile0, nslice;
 };
 
 /* work buffer bytes for nvslots value slots and nh harmonics over nlanes lanes */

@@ -607,7 +607,9 @@ __global__ __launch_bounds__(CS_TILE) void general_search_kernel(GenArgs a)
     L.V = (cx<T> *)a.work + lane_g;
     L.X = (float2 *)((char *)a.work + (size_t)a.nvslots * a.nlanes * sizeof(cx<T>)) + lane_g;
     const int q = a.q;
-    for (int tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int tbeg = a.given ? a.tile0 + (int)blockIdx.x : (int)blockIdx.x;
+    const int tend = a.given ? (blockIdx.x < (unsigned)a.nslice ? tbeg + 1 : tbeg) : a.ntiles;
+    for (int tile = tbeg; tile < tend; tile += gridDim.x) {
         const int4 Tl = a.tiles[tile];
         const ChunkDesc *C = a.chunks + Tl.x;
         const long long i0 = C->i0;
@@ -621,9 +623,9 @@ __global__ __launch_bounds__(CS_TILE) void general_search_kernel(GenArgs a)
         const int rho = (int)(tp % q);
         const int qb = (int)(tp / q);
 
-        /* 1. the harmonic stack */
+        /* 1. the harmonic stack (unless the stack pass wrote it) */
         L.x(0) = make_float2(0.0f, 0.0f);
-        {
+        if (!a.given) {
             int res = 0, qi = 0;
             for (int h = 1; h <= a.nh; h++) {
                 res += rho;

@@ -29,6 +29,7 @@
 #include "cs_gpu.h"
 #include "cs_stage.h"
 #include "fast_kernel.hpp"
+#include "split_kernel.hpp"
 
 #define HIDDEN __attribute__((visibility("hidden")))
 
@@ -42,7 +43,12 @@ typedef hipError_t (*fast_launch_fn)(const csk::FastArgs *, int, hipStream_t);
                                                                   int, hipStream_t);
 #define CS_DECL_NH(NH) CS_DECL_FAST(NH, 16, 16) CS_DECL_FAST(NH, 32, 16) \
     CS_DECL_FAST64(NH, 16, 16) CS_DECL_FAST64(NH, 32, 16)
-#define CS_DECL_NH_BIG(NH) CS_DECL_FAST(NH, 16, 16)
+typedef hipError_t (*profile_launch_fn)(const csk::FastArgs *, const csk::SplitArgs *, int,
+                                        hipStream_t);
+#define CS_DECL_NH_BIG(NH)                                                        \
+    extern "C" HIDDEN hipError_t cs_launch_profile_##NH(const csk::FastArgs *,    \
+                                                        const csk::SplitArgs *, int, \
+                                                        hipStream_t);
 CS_NH_LIST(CS_DECL_NH)
 CS_NH_BIG_LIST(CS_DECL_NH_BIG)
 
@@ -57,8 +63,15 @@ struct FastVariant {
 #define CS_ENTRY_FAST(NH, M, K)                                                    \
     {NH, M, K, cs_launch_fast_##NH##_##M##_##K, cs_launch_fast64_##NH##_##M##_##K},
 #define CS_ENTRY_NH(NH) CS_ENTRY_FAST(NH, 16, 16) CS_ENTRY_FAST(NH, 32, 16)
-#define CS_ENTRY_NH_BIG(NH) {NH, 16, 16, cs_launch_fast_##NH##_16_16, nullptr},
-const FastVariant kFast[] = {CS_NH_LIST(CS_ENTRY_NH) CS_NH_BIG_LIST(CS_ENTRY_NH_BIG)};
+const FastVariant kFast[] = {CS_NH_LIST(CS_ENTRY_NH)};
+
+/* profile passes of the two-pass search, f32, decimations up to 16 */
+struct ProfileVariant {
+    int nh;
+    profile_launch_fn fn;
+};
+#define CS_ENTRY_NH_BIG(NH) {NH, cs_launch_profile_##NH},
+const ProfileVariant kProfile[] = {CS_NH_BIG_LIST(CS_ENTRY_NH_BIG)};
 
 #define HIPCHK(x)                                                                  \
     do {                                                                           \
@@ -159,6 +172,12 @@ __global__ __launch_bounds__(256) void sigma_kernel(const T *__restrict__ samp,
 
 /* HBM the general kernel's work buffer may take per session */
 #define CS_GEN_WORK_BUDGET (1536ll << 20)
+/* HBM the two-pass search's stack buffer may take per session (one slice) */
+#define CS_SPLIT_BUDGET (1024ll << 20)
+/* dynamic LDS of the stack pass: a group's windows up to the first bound
+ * when a group of several harmonics fits it, else up to the second */
+#define CS_SPLIT_LDS_SOFT (24 << 10)
+#define CS_SPLIT_LDS_MAX (48 << 10)
 
 struct GpuSession {
     const SearchParams *p;
@@ -218,7 +237,56 @@ struct GpuSession {
     size_t work_bytes;
     std::vector<char> h_planbuf;  /* host copy kept alive for the asynchronous upload */
     GenArgs gargs;
+    /* the two-pass search (split_kernel.hpp): 0 off; 1 stack pass + register
+     * profile pass; 2 stack pass + the general kernel's c2r and boxcar */
+    int split;
+    int split_m;                  /* padded taps of the stack pass */
+    size_t split_lds;             /* its dynamic LDS bytes */
+    profile_launch_fn prof;
+    csk::SplitArgs sargs;         /* per launch: tile0, nslice */
+    int slice;                    /* tiles per slice */
+    float2 *d_stk;                /* stack of one slice (split 1) */
 };
+
+/* Harmonics per window group of the stack pass for nh harmonics, trial step
+ * lodr and M padded taps (0 when not even one harmonic's window fits), and
+ * the LDS bytes of the largest group.  A group's windows take at most
+ * sum_h (ceil(255*h*lodr) + M + 2) bins. */
+static int split_group(int nh, double lodr, int M, size_t *lds)
+{
+    for (int pass = 0; pass < 2; pass++) {
+        const double cap = (pass == 0 ? CS_SPLIT_LDS_SOFT : CS_SPLIT_LDS_MAX) / 8.0;
+        for (int hg = 64; hg >= (pass == 0 ? 2 : 1); hg >>= 1) {
+            double worst = 0.0;
+            for (int g0 = 0; g0 < nh; g0 += hg) {
+                double need = 0.0;
+                for (int h = g0 + 1; h <= std::min(nh, g0 + hg); h++)
+                    need += ceil((CS_TILE - 1) * h * lodr) + M + 2;
+                worst = std::max(worst, need);
+            }
+            if (worst <= cap) {
+                *lds = ((size_t)worst * 8 + 255) / 256 * 256;
+                return hg;
+            }
+        }
+    }
+    return 0;
+}
+
+/* stack pass launch for the session's padded tap count */
+static hipError_t launch_stack_pass(const csk::FastArgs *a, const csk::SplitArgs *sa, int M,
+                                    int nb, size_t lds, hipStream_t st)
+{
+    if (M == 16)
+        hipLaunchKernelGGL(csk::stack_pass_kernel<16>, dim3(nb), dim3(CS_TILE), lds, st, *a, *sa);
+    else if (M == 32)
+        hipLaunchKernelGGL(csk::stack_pass_kernel<32>, dim3(nb), dim3(CS_TILE), lds, st, *a, *sa);
+    else if (M == 64)
+        hipLaunchKernelGGL(csk::stack_pass_kernel<64>, dim3(nb), dim3(CS_TILE), lds, st, *a, *sa);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
 
 /* cos and sin as two separate libm calls, through pointers the compiler cannot
  * merge into one sincos call (which differs from them in the last bit for some
@@ -243,6 +311,13 @@ static float2 *shared_twiddles(int device)
                 tw[(size_t)cs_tw_off(Nn) + j] =
                     make_float2((float)cs_cos_fn(ang), (float)cs_sin_fn(ang));
             }
+        /* the kernels' compile-time twiddles must be this table */
+        static_assert(sizeof(kTwF) == sizeof(float) * 2 * CS_TW_SIZE, "twiddle table size");
+        if (memcmp(tw.data(), kTwF, sizeof(kTwF)) != 0) {
+            fprintf(stderr, "Error: this host's libm cos/sin differ from the twiddles the "
+                            "kernels were built with; rebuild the library on this host\n");
+            return NULL;
+        }
         float2 *d = NULL;
         if (hipMalloc(&d, tw.size() * sizeof(float2)) != hipSuccess) return NULL;
         if (hipMemcpy(d, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice) !=
@@ -269,6 +344,12 @@ static double2 *shared_twiddles64(int device)
                 const double ang = 2.0 * M_PI * (double)j / (double)Nn;
                 tw[(size_t)cs_tw_off(Nn) + j] = make_double2(cs_cos_fn(ang), cs_sin_fn(ang));
             }
+        static_assert(sizeof(kTwD) == sizeof(double) * 2 * CS_TW_SIZE, "twiddle table size");
+        if (memcmp(tw.data(), kTwD, sizeof(kTwD)) != 0) {
+            fprintf(stderr, "Error: this host's libm cos/sin differ from the twiddles the "
+                            "kernels were built with; rebuild the library on this host\n");
+            return NULL;
+        }
         double2 *d = NULL;
         if (hipMalloc(&d, tw.size() * sizeof(double2)) != hipSuccess) return NULL;
         if (hipMemcpy(d, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice) !=
@@ -304,6 +385,7 @@ static void session_release(GpuSession *S)
     dfree(S, S->d_mout);
     dfree(S, S->d_plan);
     dfree(S, S->d_work);
+    dfree(S, S->d_stk);
     if (S->gexec) hipGraphExecDestroy(S->gexec);
     if (S->graph) hipGraphDestroy(S->graph);
     if (S->ev0) hipEventDestroy(S->ev0);
@@ -535,7 +617,7 @@ static int gen_setup(GpuSession *S)
     g.pnum = S->pnum;
     g.ws = S->args.ws;
     g.m = S->m;
-    g.mp = (S->m + 3) / 4 * 4;
+    g.mp = S->args.m2e + S->m / 2;      /* the table's padded tap count */
     g.nh = S->nh;
     g.chunks = S->d_chunks;
     g.nslot = S->nslot;
@@ -551,8 +633,9 @@ static int gen_setup(GpuSession *S)
 }
 
 /* Device bytes beyond the tables that a session of p at trial step lodr
- * takes for the general kernel's work buffer (at most), 0 when a register
- * kernel serves it -- the engine's admission estimate. */
+ * takes for the two-pass search's stack buffer or the general kernel's work
+ * buffer (at most), 0 when a one-pass register kernel serves it -- the
+ * engine's admission estimate. */
 __attribute__((visibility("hidden"))) long long cs_session_general_bytes(const SearchParams *p,
                                                                         double lodr)
 {
@@ -564,6 +647,10 @@ __attribute__((visibility("hidden"))) long long cs_session_general_bytes(const S
         for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + mp + 2;
         if (need <= (double)CS_WIN_CAP(nh, mp)) return 0;
     }
+    size_t lds = 0;
+    const int ms = m <= 16 ? 16 : (m <= 32 ? 32 : 64);
+    if (!f64 && nh >= 65 && nh <= 128 && split_group(nh, lodr, ms, &lds) > 0)
+        return CS_SPLIT_BUDGET + (16ll << 20);
     return CS_GEN_WORK_BUDGET + (16ll << 20);
 }
 
@@ -614,6 +701,13 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->d_work = NULL;
     S->work_bytes = 0;
     memset((void *)&S->gargs, 0, sizeof(S->gargs));
+    S->split = 0;
+    S->split_m = 0;
+    S->split_lds = 0;
+    S->prof = NULL;
+    memset((void *)&S->sargs, 0, sizeof(S->sargs));
+    S->slice = 0;
+    S->d_stk = NULL;
 
     const int nh = p->nharms, m = p->m;
     std::vector<float> wtab;
@@ -675,8 +769,24 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
                         S->slot_k[s]);
                 goto fail;
             }
-        S->gen = true;
-        mp = (m + 3) / 4 * 4;
+        /* the two-pass search whenever a window group fits the stack pass's
+         * LDS: its register profile pass for 65-128 harmonics in f32, the
+         * general kernel's c2r and boxcar otherwise */
+        const int ms = m <= 16 ? 16 : (m <= 32 ? 32 : 64);
+        const int hg = split_group(nh, lodr, ms, &S->split_lds);
+        if (hg > 0) {
+            S->split_m = ms;
+            S->sargs.hg = hg;
+            S->sargs.nh = nh;
+            if (!S->f64 && nh >= 65 && nh <= 128) {
+                S->split = 1;
+                S->prof = kProfile[nh - 65].fn;
+            } else {
+                S->split = 2;
+            }
+        }
+        S->gen = S->split != 1;
+        mp = S->split ? ms : (m + 3) / 4 * 4;
     }
     S->q = dplans->plans[0].q;
     S->pnum = dplans->plans[0].pnum;
@@ -830,7 +940,37 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     S->args.xcd_stripe = 0;
     if (S->gen) {
         if (gen_setup(S) != 0) goto fail;
-    } else {
+    }
+    if (S->split == 1) {
+        /* the stack of a slice of tiles, up to the budget */
+        const long long per = (long long)nh * CS_TILE * (long long)sizeof(float2);
+        S->slice = (int)std::max<long long>(
+            1, std::min<long long>(std::max(S->args.ntiles, 1), CS_SPLIT_BUDGET / per));
+        HIPCHK(dalloc(S, (void **)&S->d_stk, (size_t)S->slice * (size_t)per));
+        S->sargs.stk = S->d_stk;
+        S->sargs.tstride = (long long)nh * CS_TILE;
+        S->sargs.hstride = CS_TILE;
+        S->sargs.hbase = 0;
+    } else if (S->split == 2) {
+        /* straight into the general kernel's stack entries: slice tile u is
+         * its workgroup u, harmonic h at X + h*nlanes + u*256 + lane */
+        S->slice = (int)(S->gargs.nlanes / CS_TILE);
+        S->sargs.stk = (float2 *)((char *)S->d_work + (size_t)S->gargs.nvslots *
+                                                          (size_t)S->gargs.nlanes *
+                                                          (S->f64 ? 16 : 8));
+        S->sargs.tstride = CS_TILE;
+        S->sargs.hstride = S->gargs.nlanes;
+        S->sargs.hbase = 1;
+    }
+    if (S->split) {
+        /* CS_SPLIT_SLICE=n: at most n tiles per slice (tests of the slice loop) */
+        if (const char *e = getenv("CS_SPLIT_SLICE")) {
+            const int n = atoi(e);
+            if (n > 0 && n < S->slice) S->slice = n;
+        }
+        /* stripes of 16 tiles per XCD: neighbouring tiles' windows share L2 */
+        S->args.xcd_stripe = 16;
+    } else if (!S->gen) {
         /* a tuned launch for this kernel class, when there is one */
         CsTuneResult tr;
         if (cs_tune_lookup(device, nh, mp, S->f64, S->measured, &tr))
@@ -881,7 +1021,21 @@ static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_ou
     const int ntiles = S->args.ntiles;
     const int unit = 8 * std::max(S->args.xcd_stripe, 1);
     const int nblk = ((ntiles + unit - 1) / unit) * unit;
-    auto fn = [S](const csk::FastArgs *a, int nb, hipStream_t st) -> hipError_t {
+    auto fn = [S, unit](const csk::FastArgs *a, int nb, hipStream_t st) -> hipError_t {
+        if (S->split == 1) {
+            /* per slice: the stack pass, then the profile pass */
+            for (int t0 = 0; t0 < a->ntiles; t0 += S->slice) {
+                csk::SplitArgs sa = S->sargs;
+                sa.tile0 = t0;
+                sa.nslice = std::min(S->slice, a->ntiles - t0);
+                const int sb = (sa.nslice + unit - 1) / unit * unit;
+                hipError_t e = launch_stack_pass(a, &sa, S->split_m, sb, S->split_lds, st);
+                if (e != hipSuccess) return e;
+                e = S->prof(a, &sa, sb, st);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
         if (!S->gen) return (S->f64 ? S->fast->fn64 : S->fast->fn)(a, nb, st);
         GenArgs &g = S->gargs;
         g.recs = a->recs;
@@ -895,6 +1049,25 @@ static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_ou
         g.dump_t0 = a->dump_t0;
         g.dump_n = a->dump_n;
         g.grp = a->grp;
+        if (S->split == 2) {
+            /* per slice: the stack pass into the work buffer, then the
+             * general kernel's c2r and boxcar, workgroup u on slice tile u */
+            for (int t0 = 0; t0 < a->ntiles; t0 += S->slice) {
+                csk::SplitArgs sa = S->sargs;
+                sa.tile0 = t0;
+                sa.nslice = std::min(S->slice, a->ntiles - t0);
+                const int sb = (sa.nslice + unit - 1) / unit * unit;
+                hipError_t e = launch_stack_pass(a, &sa, S->split_m, sb, S->split_lds, st);
+                if (e != hipSuccess) return e;
+                g.tile0 = t0;
+                g.nslice = sa.nslice;
+                g.given = 1;
+                e = cs_gen_launch(&g, S->f64, st);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
+        g.given = 0;
         return cs_gen_launch(&g, S->f64, st);
     };
     hipLaunchKernelGGL(cs_zero_u32, dim3(1), dim3(1), 0, S->st, counter);

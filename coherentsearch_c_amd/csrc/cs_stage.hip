@@ -713,7 +713,8 @@ __attribute__((visibility("hidden"))) void *cs_upload_buffer(size_t bytes)
         if (u.host) hipHostFree(u.host);
         u.host = nullptr;
         u.cap = 0;
-        const size_t c = std::max<size_t>(bytes + bytes / 2, (size_t)1 << 20);
+        const size_t c = std::max(std::max<size_t>(bytes + bytes / 2, 2 * u.cap),
+                                  (size_t)1 << 20);
         if (hipHostMalloc(&u.host, c, hipHostMallocDefault) != hipSuccess) {
             u.host = nullptr;
             return nullptr;

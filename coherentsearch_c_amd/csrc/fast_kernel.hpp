@@ -24,9 +24,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <utility>
 
 #include "cs_device.h"
+#include "cs_twiddles.h"
 
 namespace csk {
 
@@ -96,20 +98,17 @@ constexpr int ncand(int nb)
 }
 constexpr int wcmax(int nb) { return ncand(nb) > 0 ? wseq(ncand(nb) - 1) : 0; }
 
-/* Twiddles are read through the constant address space: every index is a
- * compile-time constant, so they become scalar loads and SGPR operands.  The
- * f32 path uses (float)cos/sin, the f64 path the double values. */
-typedef const __attribute__((address_space(4))) float twc_t;
-typedef const __attribute__((address_space(4))) double twd_t;
+/* Twiddles are compile-time constants (cs_twiddles.h, written at build time
+ * from libm cos and sin exactly as the session code tabulates them, which
+ * checks the two agree): every index below is a constant after unrolling, so
+ * each twiddle becomes a scalar move or a literal operand -- no loads and no
+ * waits on them.  The f32 path uses (float)cos/sin, the f64 path the double
+ * values. */
+struct TwF {};
+struct TwD {};
 
-__device__ __forceinline__ cx<float> ldtw(twc_t *__restrict__ tw, int i)
-{
-    return {tw[2 * i], tw[2 * i + 1]};
-}
-__device__ __forceinline__ cx<double> ldtw(twd_t *__restrict__ tw, int i)
-{
-    return {tw[2 * i], tw[2 * i + 1]};
-}
+__device__ __forceinline__ cx<float> ldtw(TwF, int i) { return {kTwF[2 * i], kTwF[2 * i + 1]}; }
+__device__ __forceinline__ cx<double> ldtw(TwD, int i) { return {kTwD[2 * i], kTwD[2 * i + 1]}; }
 
 __device__ __forceinline__ float fm(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fm(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -422,6 +421,103 @@ constexpr int waves_for(int nh)
     return sizeof(T) == 4 ? (nh > 64 ? 1 : (nh > 32 ? 2 : 3)) : (nh > 16 ? 1 : 2);
 }
 
+/* Everything after the interpolated stack, for trial t (row lj of the chunk
+ * at i0, the lane active when the trial is in its tile): the optional stack
+ * dump, the c2r of every non-divisor decimation, the base profile and its
+ * folds, then per decimation slot the boxcar metric with its record, metric
+ * or measured-sigma sample output. */
+template <int NH, int KMAX, typename T>
+__device__ __forceinline__ void search_tail(const cf (&X)[NH + 1], const FastArgs &a,
+                                            const ChunkDesc *__restrict__ C, int chunk,
+                                            long long i0,
+                                            int lj, long long t, bool active)
+{
+    if (a.stack_out != nullptr) {
+        const long long d = t - a.dump_t0;
+        if (active && d >= 0 && d < a.dump_n) {
+            sfor<NH + 1>([&](auto I) {
+                constexpr int h = decltype(I)::value;
+                a.stack_out[d * (NH + 1) + h] = make_float2(X[h].x, X[h].y);
+            });
+        }
+    }
+
+    const double thr = a.threshold;
+    /* the c2r twiddles of this precision */
+    using TwP = std::conditional_t<sizeof(T) == 4, TwF, TwD>;
+    const TwP twp{};
+
+    /* decimations k not dividing nharms: strided c2r of the stack */
+    sfor<KMAX - 1>([&](auto I) {
+        constexpr int k = decltype(I)::value + 2;
+        if constexpr (NH / k >= 1 && NH % k != 0) {
+            const int slot = a.kslot[k];
+            if (slot >= 0) {
+                constexpr int Hk = NH / k;
+                T xk[2 * Hk];
+                c2r<Hk, k, NH + 1>(X, xk, twp);
+                if (a.samp) {
+                    sample_out<2 * Hk>(xk, active, lj, C->nvalid[slot], chunk,
+                                       slot, a);
+                } else {
+                    const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
+                                                C->valid[slot]);
+                    if (a.mout)
+                        metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
+                    else
+                        emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot,
+                             lj, mv, a);
+                }
+            }
+        }
+    });
+
+    /* base profile */
+    T x[2 * NH];
+    c2r<NH, 1, NH + 1>(X, x, twp);
+
+    /* divisor decimations fold the base profile (src/search.c:298-331) */
+    sfor<KMAX - 1>([&](auto I) {
+        constexpr int k = decltype(I)::value + 2;
+        if constexpr (NH / k >= 1 && NH % k == 0) {
+            const int slot = a.kslot[k];
+            if (slot >= 0) {
+                constexpr int nk = 2 * (NH / k);
+                constexpr T invk = T(1) / (T)k;
+                T xk[nk];
+                sfor<nk>([&](auto J) {
+                    constexpr int j = decltype(J)::value;
+                    T s = x[j];
+                    sfor<k - 1>([&](auto Q) {
+                        constexpr int qq = decltype(Q)::value + 1;
+                        s = s + x[j + qq * nk];
+                    });
+                    xk[j] = s * invk;
+                });
+                if (a.samp) {
+                    sample_out<nk>(xk, active, lj, C->nvalid[slot], chunk, slot, a);
+                } else {
+                    const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
+                                            C->valid[slot]);
+                    if (a.mout)
+                        metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
+                    else
+                        emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot,
+                             lj, mv, a);
+                }
+            }
+        }
+    });
+
+    if (a.samp) {
+        sample_out<2 * NH>(x, active, lj, C->nvalid[0], chunk, 0, a);
+        return;
+    }
+    const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0]);
+    if (a.mout) metric_out<T>(active, 0, i0, lj, mb, a);
+    else emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
+}
+
 template <int NH, int M, int KMAX, typename T>
 __global__ __launch_bounds__(CS_TILE) __attribute__((amdgpu_waves_per_eu(waves_for<T>(NH))))
 void fast_search_kernel(FastArgs a)
@@ -575,93 +671,7 @@ void fast_search_kernel(FastArgs a)
         });
     }
 
-    if (a.stack_out != nullptr) {
-        const long long d = t - a.dump_t0;
-        if (active && d >= 0 && d < a.dump_n) {
-            sfor<NH + 1>([&](auto I) {
-                constexpr int h = decltype(I)::value;
-                a.stack_out[d * (NH + 1) + h] = make_float2(X[h].x, X[h].y);
-            });
-        }
-    }
-
-    const double thr = a.threshold;
-    const int chunk = Tl.x;
-    /* the c2r twiddles of this precision */
-    auto twp = [&]() {
-        if constexpr (sizeof(T) == 4) return (twc_t *)a.tw;
-        else return (twd_t *)a.twd;
-    }();
-
-    /* decimations k not dividing nharms: strided c2r of the stack */
-    sfor<KMAX - 1>([&](auto I) {
-        constexpr int k = decltype(I)::value + 2;
-        if constexpr (NH / k >= 1 && NH % k != 0) {
-            const int slot = a.kslot[k];
-            if (slot >= 0) {
-                constexpr int Hk = NH / k;
-                T xk[2 * Hk];
-                c2r<Hk, k, NH + 1>(X, xk, twp);
-                if (a.samp) {
-                    sample_out<2 * Hk>(xk, active, lj, C->nvalid[slot], chunk,
-                                       slot, a);
-                } else {
-                    const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
-                                                C->valid[slot]);
-                    if (a.mout)
-                        metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
-                    else
-                        emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot,
-                             lj, mv, a);
-                }
-            }
-        }
-    });
-
-    /* base profile */
-    T x[2 * NH];
-    c2r<NH, 1, NH + 1>(X, x, twp);
-
-    /* divisor decimations fold the base profile (src/search.c:298-331) */
-    sfor<KMAX - 1>([&](auto I) {
-        constexpr int k = decltype(I)::value + 2;
-        if constexpr (NH / k >= 1 && NH % k == 0) {
-            const int slot = a.kslot[k];
-            if (slot >= 0) {
-                constexpr int nk = 2 * (NH / k);
-                constexpr T invk = T(1) / (T)k;
-                T xk[nk];
-                sfor<nk>([&](auto J) {
-                    constexpr int j = decltype(J)::value;
-                    T s = x[j];
-                    sfor<k - 1>([&](auto Q) {
-                        constexpr int qq = decltype(Q)::value + 1;
-                        s = s + x[j + qq * nk];
-                    });
-                    xk[j] = s * invk;
-                });
-                if (a.samp) {
-                    sample_out<nk>(xk, active, lj, C->nvalid[slot], chunk, slot, a);
-                } else {
-                    const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
-                                            C->valid[slot]);
-                    if (a.mout)
-                        metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
-                    else
-                        emit(active && lj < C->nvalid[slot] && (double)mv > thr, chunk, slot,
-                             lj, mv, a);
-                }
-            }
-        }
-    });
-
-    if (a.samp) {
-        sample_out<2 * NH>(x, active, lj, C->nvalid[0], chunk, 0, a);
-        return;
-    }
-    const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0]);
-    if (a.mout) metric_out<T>(active, 0, i0, lj, mb, a);
-    else emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
+    search_tail<NH, KMAX, T>(X, a, C, Tl.x, i0, lj, t, active);
 }
 
 }  // namespace csk

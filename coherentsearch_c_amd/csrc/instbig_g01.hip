@@ -1,9 +1,9 @@
-/* Fused search kernel instantiations above 64 harmonics for nharms = 66, 82, 98, 114
- * (M = 16 padded taps, f32 profiles; decimations k <= 16).  Harmonic counts are
- * interleaved across the group files so that their compile times balance. */
-#include "fast_kernel.hpp"
+/* Profile passes of the two-pass search above 64 harmonics for nharms = 66, 82, 98, 114
+ * (f32 profiles, decimations k <= 16).  Harmonic counts are interleaved
+ * across the group files so that their compile times balance. */
+#include "split_kernel.hpp"
 
-CS_FAST_INSTANTIATE(66, 16, 16)
-CS_FAST_INSTANTIATE(82, 16, 16)
-CS_FAST_INSTANTIATE(98, 16, 16)
-CS_FAST_INSTANTIATE(114, 16, 16)
+CS_SPLIT_INSTANTIATE(66)
+CS_SPLIT_INSTANTIATE(82)
+CS_SPLIT_INSTANTIATE(98)
+CS_SPLIT_INSTANTIATE(114)

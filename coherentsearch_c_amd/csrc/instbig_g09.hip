@@ -1,9 +1,9 @@
-/* Fused search kernel instantiations above 64 harmonics for nharms = 74, 90, 106, 122
- * (M = 16 padded taps, f32 profiles; decimations k <= 16).  Harmonic counts are
- * interleaved across the group files so that their compile times balance. */
-#include "fast_kernel.hpp"
+/* Profile passes of the two-pass search above 64 harmonics for nharms = 74, 90, 106, 122
+ * (f32 profiles, decimations k <= 16).  Harmonic counts are interleaved
+ * across the group files so that their compile times balance. */
+#include "split_kernel.hpp"
 
-CS_FAST_INSTANTIATE(74, 16, 16)
-CS_FAST_INSTANTIATE(90, 16, 16)
-CS_FAST_INSTANTIATE(106, 16, 16)
-CS_FAST_INSTANTIATE(122, 16, 16)
+CS_SPLIT_INSTANTIATE(74)
+CS_SPLIT_INSTANTIATE(90)
+CS_SPLIT_INSTANTIATE(106)
+CS_SPLIT_INSTANTIATE(122)

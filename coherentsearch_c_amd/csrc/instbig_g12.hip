@@ -1,9 +1,9 @@
-/* Fused search kernel instantiations above 64 harmonics for nharms = 77, 93, 109, 125
- * (M = 16 padded taps, f32 profiles; decimations k <= 16).  Harmonic counts are
- * interleaved across the group files so that their compile times balance. */
-#include "fast_kernel.hpp"
+/* Profile passes of the two-pass search above 64 harmonics for nharms = 77, 93, 109, 125
+ * (f32 profiles, decimations k <= 16).  Harmonic counts are interleaved
+ * across the group files so that their compile times balance. */
+#include "split_kernel.hpp"
 
-CS_FAST_INSTANTIATE(77, 16, 16)
-CS_FAST_INSTANTIATE(93, 16, 16)
-CS_FAST_INSTANTIATE(109, 16, 16)
-CS_FAST_INSTANTIATE(125, 16, 16)
+CS_SPLIT_INSTANTIATE(77)
+CS_SPLIT_INSTANTIATE(93)
+CS_SPLIT_INSTANTIATE(109)
+CS_SPLIT_INSTANTIATE(125)

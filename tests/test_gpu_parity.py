@@ -114,6 +114,34 @@ def test_above_64_harmonics_bit_identical(oracle, nh, maxdecim, sigma, blocksize
     sess.close()
 
 
+@pytest.mark.parametrize("nh,m,maxdecim,precision,slice_", [
+    (120, 16, 16, "f32", 5), (100, 64, 6, "f32", 3), (77, 28, 12, "f32", 0),
+    (60, 64, 6, "f32", 4), (240, 16, 6, "f32", 6), (90, 16, 6, "f64", 7)])
+def test_two_pass_slices_bit_identical(oracle, nh, m, maxdecim, precision, slice_, monkeypatch):
+    """The two-pass search (stack pass, then the register profile pass above
+    64 harmonics in f32, the general kernel's c2r otherwise) over slices of a
+    few tiles, so that every slice boundary is crossed: the stack and every
+    per-trial metric bit-identical to the oracle's."""
+    if slice_:
+        monkeypatch.setenv("CS_SPLIT_SLICE", str(slice_))
+    spec, p, o, sess = _case(oracle, nh, m, nbins=1 << 14, blocksize=2048, maxdecim=maxdecim,
+                             precision=precision, r0=61.7)
+    say("two-pass nh", nh, "m", m, "maxdecim", maxdecim, precision, "slice", slice_,
+        "trials", o.total, "chunks", o.nchunks)
+    for ci in sorted({0, o.nchunks // 2, o.nchunks - 1}):
+        i0, n, _ = o.chunk(ci)
+        want, _ = o.stack(ci)
+        got = sess.dump_stack(i0, n)
+        assert np.array_equal(got.transpose(1, 0, 2).view(np.uint32), want.view(np.uint32)), ci
+    got = sess.run(-np.inf)
+    want = o.search(-np.inf, 1)
+    assert got.size == want.size
+    for f in ("r", "nharm", "metric"):
+        assert np.array_equal(got[f], want[f]), f
+    assert np.array_equal(sess.run(6.0)["metric"], want["metric"][want["metric"] > 6.0])
+    sess.close()
+
+
 @pytest.mark.parametrize("nh,m,maxdecim", [(16, 16, 6), (32, 32, 6), (60, 16, 6), (7, 8, 3),
                                             (45, 16, 12)])
 def test_f64_every_trial_metric_bit_identical(oracle, nh, m, maxdecim):
