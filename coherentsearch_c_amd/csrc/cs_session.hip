@@ -246,6 +246,8 @@ struct GpuSession {
     csk::SplitArgs sargs;         /* per launch: tile0, nslice */
     int slice;                    /* tiles per slice */
     float2 *d_stk;                /* stack of one slice (split 1) */
+    bool slow;                    /* a trial step that is no simple rational (P = 0):
+                                     stack_slow_kernel, hr_lo per harmonic in d_wtab */
 };
 
 /* Harmonics per window group of the stack pass for nh harmonics, trial step
@@ -641,7 +643,10 @@ __attribute__((visibility("hidden"))) long long cs_session_general_bytes(const S
 {
     const int nh = p->nharms, m = p->m, mp = m <= 16 ? 16 : (m <= 32 ? 32 : 0);
     const bool f64 = p->precision == CS_PREC_F64;
+    int pn = 0, qd = 0;
+    const bool rational = csx_trial_grid_rational(lodr, &pn, &qd) != 0;
     for (const FastVariant &v : kFast) {
+        if (!rational) break;
         if (v.nh != nh || v.m != mp || (f64 && !v.fn64)) continue;
         double need = 0.0;
         for (int h = 1; h <= nh; h++) need += ceil((CS_TILE - 1) * h * lodr) + mp + 2;
@@ -649,7 +654,7 @@ __attribute__((visibility("hidden"))) long long cs_session_general_bytes(const S
     }
     size_t lds = 0;
     const int ms = m <= 16 ? 16 : (m <= 32 ? 32 : 64);
-    if (!f64 && nh >= 65 && nh <= 128 && split_group(nh, lodr, ms, &lds) > 0)
+    if (!f64 && nh >= 65 && nh <= 128 && (!rational || split_group(nh, lodr, ms, &lds) > 0))
         return CS_SPLIT_BUDGET + (16ll << 20);
     return CS_GEN_WORK_BUDGET + (16ll << 20);
 }
@@ -708,6 +713,7 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     memset((void *)&S->sargs, 0, sizeof(S->sargs));
     S->slice = 0;
     S->d_stk = NULL;
+    S->slow = false;
 
     const int nh = p->nharms, m = p->m;
     std::vector<float> wtab;
@@ -742,11 +748,21 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         S->slot_Hk[S->nslot] = nh / k;
         S->nslot++;
     }
+    /* a trial step that is no simple rational: per-trial FP64 positions and
+     * coefficients (the stack pass's non-rational form), in f32 only -- the
+     * reference keeps a double stack for it in f64 (src/interp.c:464-467) */
+    S->slow = dplans->plans[0].P == 0;
+    if (S->slow && S->f64) {
+        fprintf(stderr, "Error: --precision f64 needs a simple rational --hidr on the GPU "
+                        "(the non-rational form keeps a double stack)\n");
+        goto fail;
+    }
     /* the register kernel of the smallest padded tap count M >= m, when one
      * holds this configuration and the tile's windows fit its LDS buffer;
      * otherwise the general kernel */
     mp = m <= 16 ? 16 : (m <= 32 ? 32 : 0);
     for (const FastVariant &v : kFast) {
+        if (S->slow) break;
         if (v.nh != nh || v.m != mp || (S->f64 && !v.fn64)) continue;
         bool ok = true;
         for (int s = 1; s < S->nslot; s++) ok = ok && S->slot_k[s] <= v.kmax;
@@ -773,7 +789,8 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
          * LDS: its register profile pass for 65-128 harmonics in f32, the
          * general kernel's c2r and boxcar otherwise */
         const int ms = m <= 16 ? 16 : (m <= 32 ? 32 : 64);
-        const int hg = split_group(nh, lodr, ms, &S->split_lds);
+        const int hg = S->slow ? 64 : split_group(nh, lodr, ms, &S->split_lds);
+        if (S->slow) S->split_lds = 0;
         if (hg > 0) {
             S->split_m = ms;
             S->sargs.hg = hg;
@@ -788,12 +805,13 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         S->gen = S->split != 1;
         mp = S->split ? ms : (m + 3) / 4 * 4;
     }
-    S->q = dplans->plans[0].q;
-    S->pnum = dplans->plans[0].pnum;
+    S->q = S->slow ? 1 : dplans->plans[0].q;
+    S->pnum = S->slow ? 0 : dplans->plans[0].pnum;
     for (int h = 0; h < nh; h++)
-        if (dplans->plans[h].P <= 0 || dplans->plans[h].q != S->q ||
-            dplans->plans[h].pnum != S->pnum || dplans->plans[h].m != m) {
-            fprintf(stderr, "Error: the GPU search needs a rational --hidr (tabulated plans)\n");
+        if ((dplans->plans[h].P == 0) != S->slow || dplans->plans[h].m != m ||
+            (!S->slow && (dplans->plans[h].P < 0 || dplans->plans[h].q != S->q ||
+                          dplans->plans[h].pnum != S->pnum))) {
+            fprintf(stderr, "Error: inconsistent plan set for the GPU search\n");
             goto fail;
         }
     /* 32-bit residue walk bounds: h*rho < 2^31, h*qbase < 2^31 */
@@ -813,7 +831,17 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
     }
     wtab.assign((size_t)nh * S->q * ws, 0.0f);
     hinfo.resize((size_t)nh);
-    for (int h = 1; h <= nh; h++) {
+    if (S->slow) {
+        /* hr_lo = rfloor0 + dr0 per harmonic, as double, in the table's place */
+        wtab.assign((size_t)2 * nh, 0.0f);
+        for (int h = 1; h <= nh; h++) {
+            const DirectPlan *pl = &dplans->plans[h - 1];
+            const double v = (double)(pl->rfloor0 + pl->dr0);
+            memcpy(&wtab[(size_t)2 * (h - 1)], &v, sizeof(double));
+            hinfo[(size_t)h - 1] = make_int2(pl->rfloor0, 1);
+        }
+    }
+    for (int h = 1; h <= nh && !S->slow; h++) {
         const DirectPlan *pl = &dplans->plans[h - 1];
         int cthr = S->q;
         for (int res = 0; res < S->q; res++) {
@@ -963,6 +991,8 @@ __attribute__((visibility("hidden"))) GpuSession *cs_session_create_on(const FFT
         S->sargs.hbase = 1;
     }
     if (S->split) {
+        S->sargs.m = m;
+        S->sargs.lodr = lodr;
         /* CS_SPLIT_SLICE=n: at most n tiles per slice (tests of the slice loop) */
         if (const char *e = getenv("CS_SPLIT_SLICE")) {
             const int n = atoi(e);
@@ -1010,6 +1040,102 @@ __global__ void cs_zero_u32(unsigned int *p) { p[0] = 0u; }
 /* The launch sequence of one search into a record buffer of `cap` records
  * and its counter: the counter cleared by a kernel (also inside a captured
  * graph), the measured-sigma passes, the search kernel. */
+namespace csk {
+
+/* sinpi / cospi of the reference (src/interp.c:17-43): exact at the
+ * multiples of 1/2, sin and cos of pi*r otherwise (r = x mod 2) */
+__device__ __forceinline__ double ref_sinpi(double x)
+{
+    if (!isfinite(x)) return x - x;
+    double r = fmod(x, 2.0);
+    if (r < 0) r += 2.0;
+    if (r == 0.0 || r == 1.0) return 0.0 * r;
+    if (r == 0.5) return 1.0;
+    if (r == 1.5) return -1.0;
+    return sin(M_PI * r);
+}
+__device__ __forceinline__ double ref_cospi(double x)
+{
+    if (!isfinite(x)) return x - x;
+    double r = fmod(x, 2.0);
+    if (r < 0) r += 2.0;
+    if (r == 0.0) return 1.0;
+    if (r == 0.5 || r == 1.5) return 0.0;
+    if (r == 1.0) return -1.0;
+    return cos(M_PI * r);
+}
+
+/* Stack pass for a trial step that is no simple rational (plans with P = 0,
+ * src/interp.c:427-475): per trial and harmonic the FP64 position
+ * r = hr_lo[h] + t*(h*lodr), the taps around floor(r + 1e-15), weights
+ * 1/(dr - (i - m/2)) and the rotation (sin cos, -sin^2)/pi of the fractional
+ * offset, every product rounded and then added in double (no contraction),
+ * rounded to the float stack.  No table: the spectrum is read straight from
+ * HBM, neighbouring lanes' taps sharing cache lines.  hr_lo[h-1] =
+ * rfloor0 + dr0 of harmonic h, as double. */
+__global__ __launch_bounds__(CS_TILE) void stack_slow_kernel(FastArgs a, SplitArgs s,
+                                                             const double *__restrict__ hr_lo)
+{
+    const int u = slice_tile(a);
+    if (u >= s.nslice) return;
+    const int4 Tl = a.tiles[s.tile0 + u];
+    const ChunkDesc *C = a.chunks + Tl.x;
+    const int tid = threadIdx.x;
+    const bool active = tid < Tl.z;
+    const int lj = Tl.y + (active ? tid : Tl.z - 1);
+    const long long t = C->i0 + lj;
+    const int m = s.m, m2 = m / 2;
+    const long long namps = a.namps;
+    float2 *out = s.stk + (long long)u * s.tstride + tid;
+    for (int h = 1; h <= s.nh; h++) {
+        float2 v = make_float2(0.0f, 0.0f);
+        if ((C->filled[(h - 1) >> 6] >> ((h - 1) & 63)) & 1ull) {
+            const double dh = (double)h * s.lodr;
+            const double r = hr_lo[h - 1] + (double)t * dh;
+            const long long rint0 = (long long)floor(r + 1e-15) + 1;
+            double dr = fmod(r, 1.0);
+            if (dr < 0) dr += 1.0;
+            const long long base = rint0 - m2;        /* 1-based */
+            double xr, xi;
+            if (dr == 0.0) {
+                const long long idx = base + m2 - 1;
+                const float2 b = idx >= 0 && idx < namps ? a.amps[idx] : make_float2(0.0f, 0.0f);
+                xr = (double)b.x;
+                xi = (double)b.y;
+            } else {
+                double sre = 0.0, sim = 0.0;
+                for (int i = 1; i <= m; i++) {
+                    const double w = 1.0 / (dr - (double)(i - m2));
+                    const long long idx = base + i - 1;
+                    const float2 b =
+                        idx >= 0 && idx < namps ? a.amps[idx] : make_float2(0.0f, 0.0f);
+                    sre += w * (double)b.x;
+                    sim += w * (double)b.y;
+                }
+                const double sn = ref_sinpi(dr), cs = ref_cospi(dr);
+                const double pr = sn * cs / M_PI;
+                const double pi_ = -sn * sn / M_PI;
+                xr = pr * sre - pi_ * sim;
+                xi = pr * sim + pi_ * sre;
+            }
+            v = make_float2((float)xr, (float)xi);
+        }
+        out[(long long)(h - 1 + s.hbase) * s.hstride] = v;
+    }
+}
+
+}  // namespace csk
+
+/* the session's stack pass over one slice */
+static hipError_t launch_stack(const GpuSession *S, const csk::FastArgs *a,
+                               const csk::SplitArgs *sa, int nb, hipStream_t st)
+{
+    if (!S->slow) return launch_stack_pass(a, sa, S->split_m, nb, S->split_lds, st);
+    hipLaunchKernelGGL(csk::stack_slow_kernel, dim3(nb), dim3(CS_TILE), 0, st, *a, *sa,
+                       (const double *)S->d_wtab);
+    return hipGetLastError();
+}
+
 static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_out,
                                CandRec *recs, unsigned int *counter, unsigned int cap)
 {
@@ -1029,7 +1155,7 @@ static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_ou
                 sa.tile0 = t0;
                 sa.nslice = std::min(S->slice, a->ntiles - t0);
                 const int sb = (sa.nslice + unit - 1) / unit * unit;
-                hipError_t e = launch_stack_pass(a, &sa, S->split_m, sb, S->split_lds, st);
+                hipError_t e = launch_stack(S, a, &sa, sb, st);
                 if (e != hipSuccess) return e;
                 e = S->prof(a, &sa, sb, st);
                 if (e != hipSuccess) return e;
@@ -1057,7 +1183,7 @@ static int enqueue_search_into(GpuSession *S, double threshold, float2 *stack_ou
                 sa.tile0 = t0;
                 sa.nslice = std::min(S->slice, a->ntiles - t0);
                 const int sb = (sa.nslice + unit - 1) / unit * unit;
-                hipError_t e = launch_stack_pass(a, &sa, S->split_m, sb, S->split_lds, st);
+                hipError_t e = launch_stack(S, a, &sa, sb, st);
                 if (e != hipSuccess) return e;
                 g.tile0 = t0;
                 g.nslice = sa.nslice;

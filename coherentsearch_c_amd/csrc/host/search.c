@@ -272,13 +272,6 @@ CandidateList search_file(const FFTFile *ft, const SearchParams *p, double lofre
                 "Search: %lld trials in %lld chunks of %d over r = %.1f ... %.1f bins "
                 "(%.4f ... %.4f Hz), GPU\n",
                 total, nchunks, Nprof, r_lo, r_hi, r_lo / ft->T, r_hi / ft->T);
-    for (int i = 0; i < dplans->n; i++)
-        if (dplans->plans[i].P == 0) {
-            fprintf(stderr, "Error: --hidr is not a simple rational; the GPU search needs "
-                            "the tabulated kernel\n");
-            if (own) { free(cstarts); direct_plans_free(dplans); }
-            return empty;
-        }
 
     CandidateList cands;
     candlist_init(&cands);

@@ -36,7 +36,8 @@ struct SplitArgs {
     int nslice;           /* tiles in the slice */
     int nh;               /* harmonics */
     int hg;               /* harmonics per LDS window group, 1..64 */
-    int pad;
+    int m;                /* taps (the non-rational stack pass) */
+    double lodr;          /* trial step (the non-rational stack pass) */
 };
 
 /* The slice-local tile of this workgroup: the one-pass kernel's XCD-aware

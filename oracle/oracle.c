@@ -128,6 +128,7 @@ int orc_trial_grid_rational(double lodr, int *pout, int *qout)
 typedef struct {
     int h, m, q, s, base_adv, pnum, rfloor0, P;
     double dr0;
+    double lodr;        /* P = 0: the trial step, positions in FP64 per trial */
     int32_t *row;       /* q, 1-based or 0 */
     float *W;           /* P*m */
     float *Are, *Aim;   /* P */
@@ -208,14 +209,17 @@ static int oplan_groups(OPlan *dp)
     return 0;
 }
 
-static int oplan_build(OPlan *dp, int h, int m, double r_lo, int pnum, int q)
+static int oplan_build(OPlan *dp, int h, int m, double r_lo, int pnum, int q, double lodr,
+                       int rational)
 {
     memset(dp, 0, sizeof(*dp));
     const int m2 = m / 2;
     dp->h = h; dp->m = m; dp->pnum = pnum; dp->q = q;
+    dp->lodr = lodr;
     double hr = (double)h * r_lo;
     dp->rfloor0 = (int)floor(hr);
     dp->dr0 = hr - (double)dp->rfloor0;
+    if (!rational) return 0;            /* P = 0: per-trial coefficients (interp.c:155-160) */
     long long step = (long long)h * pnum;
     dp->s = (int)(step % q);
     dp->base_adv = (int)(step / q);
@@ -267,14 +271,22 @@ static inline void ostate(const OPlan *dp, long long t, int *res, long long *qin
     *res = (int)rem;
     *qint = (long long)(prod / q);
 }
+/* P = 0: the trial's FP64 position r = (rfloor0 + dr0) + t * (h * lodr)
+ * (interp.c:359-381) */
+static inline double opos_slow(const OPlan *dp, long long t)
+{
+    return (double)(dp->rfloor0 + dp->dr0) + (double)t * ((double)dp->h * dp->lodr);
+}
 static inline long long obase(const OPlan *dp, long long t)
 {
+    if (dp->P == 0) return (long long)floor(opos_slow(dp, t) + 1e-15);
     int res; long long qint;
     ostate(dp, t, &res, &qint);
     return (long long)dp->rfloor0 + qint + (dp->carry[dp->row[res] - 1] ? 1 : 0);
 }
 static inline long long onyq(const OPlan *dp, long long t)
 {
+    if (dp->P == 0) return (long long)floor(opos_slow(dp, t));
     int res; long long qint;
     ostate(dp, t, &res, &qint);
     return (long long)dp->rfloor0 + qint;
@@ -426,10 +438,14 @@ OrcCtx *orc_ctx_create(const float *amps, long long namps, long long N, double T
     c->Nprof = cfg->blocksize == 0 ? 2048 : cfg->blocksize;
     if (c->Nprof < 1) c->Nprof = 1;
     c->rational = orc_trial_grid_rational(c->lodr, &c->pnum, &c->q);
-    if (!c->rational) { free(c); return NULL; }   /* slow path not restated */
+    /* a non-rational step in double precision keeps a double stack in the
+     * reference (interp.c:464-467); only its f32 form is restated */
+    if (!c->rational && cfg->precision == 1) { free(c); return NULL; }
+    if (!c->rational) { c->pnum = 0; c->q = 1; }
     c->plans = calloc((size_t)nh, sizeof(OPlan));
     for (int h = 1; h <= nh; h++)
-        if (oplan_build(&c->plans[h - 1], h, cfg->m, c->r_lo, c->pnum, c->q) != 0) {
+        if (oplan_build(&c->plans[h - 1], h, cfg->m, c->r_lo, c->pnum, c->q, c->lodr,
+                        c->rational) != 0) {
             orc_ctx_free(c);
             return NULL;
         }
@@ -943,6 +959,49 @@ static void fill_direct(const OrcCtx *c, const OPlan *dp, long long t0, int n,
     }
 }
 
+/* P = 0 (a trial step that is no simple rational): every trial's position
+ * in FP64, the m-tap weights 1/(dr - (i - m/2)) and the rotation from
+ * sinpi/cospi of the fractional offset, summed in double and rounded to the
+ * float stack (interp.c:427-475, fill_row_slow, the f32 workspace) */
+static void fill_slow(const OrcCtx *c, const OPlan *dp, long long t0, int n,
+                      float *rowr, float *rowi)
+{
+    const int m = dp->m, m2 = m / 2;
+    const long long namps = c->namps;
+    const double hr_lo = (double)(dp->rfloor0 + dp->dr0);
+    const double dh = (double)dp->h * dp->lodr;
+    for (int k = 0; k < n; k++) {
+        const double r = hr_lo + (double)(t0 + k) * dh;
+        const long long rint0 = (long long)floor(r + 1e-15) + 1;
+        double dr = fmod(r, 1.0);
+        if (dr < 0) dr += 1.0;
+        const long long base = rint0 - m2;     /* 1-based */
+        double xr, xi;
+        if (dr == 0.0) {
+            const long long idx = base + m2 - 1;
+            xr = (idx >= 0 && idx < namps) ? (double)c->amps[2 * idx] : 0.0;
+            xi = (idx >= 0 && idx < namps) ? (double)c->amps[2 * idx + 1] : 0.0;
+        } else {
+            double sre = 0.0, sim = 0.0;
+            for (int i = 1; i <= m; i++) {
+                const double w = 1.0 / (dr - (double)(i - m2));
+                const long long idx = base + i - 1;
+                const double ar = (idx >= 0 && idx < namps) ? (double)c->amps[2 * idx] : 0.0;
+                const double ai = (idx >= 0 && idx < namps) ? (double)c->amps[2 * idx + 1] : 0.0;
+                sre += w * ar;
+                sim += w * ai;
+            }
+            const double s_ = orc_sinpi(dr), c_ = orc_cospi(dr);
+            const double pr = s_ * c_ / M_PI;
+            const double pi_ = -s_ * s_ / M_PI;
+            xr = pr * sre - pi_ * sim;
+            xi = pr * sim + pi_ * sre;
+        }
+        rowr[k] = (float)xr;
+        rowi[k] = (float)xi;
+    }
+}
+
 static void chunk_filled(const OrcCtx *c, long long t0, int n, uint8_t *filled)
 {
     for (int h = 1; h <= c->cfg.nharms; h++) {
@@ -967,7 +1026,8 @@ static void fill_stack(const OrcCtx *c, long long t0, int n, int form,
             memset(ri, 0, (size_t)n * sizeof(float));
             continue;
         }
-        if (form == 0) fill_group(c, &c->plans[h - 1], t0, n, rr, ri, re, im);
+        if (c->plans[h - 1].P == 0) fill_slow(c, &c->plans[h - 1], t0, n, rr, ri);
+        else if (form == 0) fill_group(c, &c->plans[h - 1], t0, n, rr, ri, re, im);
         else fill_direct(c, &c->plans[h - 1], t0, n, rr, ri);
     }
 }
@@ -1688,10 +1748,15 @@ static int chunk_metrics_fp64_rows(const OrcCtx *c, long long ci, int j0, int nr
                 Xr[h] = Xi[h] = 0.0;
                 if (!f[h - 1]) continue;
                 const OPlan *dp = &c->plans[h - 1];
-                int res; long long qint;
-                ostate(dp, t, &res, &qint);
-                double pos = (double)((long long)dp->rfloor0 + qint) +
-                             (dp->dr0 + (double)res / (double)dp->q);
+                double pos;
+                if (dp->P == 0) {
+                    pos = opos_slow(dp, t);
+                } else {
+                    int res; long long qint;
+                    ostate(dp, t, &res, &qint);
+                    pos = (double)((long long)dp->rfloor0 + qint) +
+                          (dp->dr0 + (double)res / (double)dp->q);
+                }
                 orc_fourier_interp_complex(pos, c->amps, c->namps, c->cfg.m, &Xr[h], &Xi[h]);
             }
             for (int s = 0; s <= c->ndec; s++) {

@@ -161,6 +161,98 @@ def test_interpolation_vs_exact_sinc(oracle):
     assert worst < 2e-5
 
 
+def _nonrational_oracle(oracle, nh=16, m=16, hidr=0.4999, nbins=1 << 14, lobin=100,
+                        blocksize=2048, precision="f32"):
+    amps = synth.spectrum_with_pulsar(nbins, 271.3, nh, 0.05, 12.0, 4)
+    N = 2 * nbins
+    cfg = oracle.config(nharms=nh, m=m, hidr=hidr, lobin=lobin, blocksize=blocksize,
+                        precision=precision)
+    return oracle.Oracle(amps.view(np.float32), N, N * synth.DT, cfg), amps
+
+
+@pytest.mark.parametrize("nh,m,hidr", [(16, 16, 0.4999), (7, 8, 1.0 / math.sqrt(3.0)),
+                                       (32, 32, 0.5 * math.pi / 3.0)])
+def test_nonrational_step_is_the_fp64_sinc_rounded(oracle, nh, m, hidr):
+    """A trial step that is no simple rational (P = 0): every trial's position
+    in FP64 and the m-tap kernel 1/(dr - k) times (sin cos, -sin^2)/pi in
+    double.  That is sinc(off) e^{-i pi off} summed over the same taps, written
+    differently, so the stack equals the FP64 direct-sinc formula (a separate
+    function: sin and cos of pi*off per tap) rounded to float, up to the
+    double rounding of the two forms: within 1 ulp of the float, exact for
+    nearly every value.  A wrong tap offset, sign or position fails by orders
+    of magnitude."""
+    import ctypes as C
+    o, amps = _nonrational_oracle(oracle, nh=nh, m=m, hidr=hidr)
+    assert not o.L.orc_trial_grid_rational(hidr / nh, C.byref(C.c_int()), C.byref(C.c_int()))
+    a = amps.view(np.float32)
+    re, im = C.c_double(), C.c_double()
+    exact = total = 0
+    for ci in sorted({0, o.nchunks // 2, o.nchunks - 1}):
+        st, filled = o.stack(ci)
+        i0, n, _ = o.chunk(ci)
+        assert not np.any(st[0])                      # the DC row
+        for h in range(1, nh + 1):
+            rf, dr0, P, sdum, ba = C.c_int(), C.c_double(), C.c_int(), C.c_int(), C.c_int()
+            o.L.orc_plan_info(o.h, h, C.byref(rf), C.byref(dr0), C.byref(P), C.byref(sdum),
+                              C.byref(ba))
+            assert P.value == 0
+            if not filled[h - 1]:
+                assert not np.any(st[h])
+                continue
+            lodr = hidr / nh
+            for j in range(0, n, max(n // 50, 1)):
+                pos = (rf.value + dr0.value) + (i0 + j) * (h * lodr)
+                o.L.orc_fourier_interp_complex(pos, a.ctypes.data, amps.size, m,
+                                               C.byref(re), C.byref(im))
+                want = np.array([re.value, im.value], dtype=np.float32)
+                got = st[h, j]
+                ulp = np.spacing(np.maximum(np.abs(want), np.float32(1e-30)))
+                assert np.all(np.abs(got - want) <= ulp), (h, j, got, want)
+                exact += int(np.array_equal(got, want))
+                total += 1
+    assert total > 100 and exact >= 0.98 * total
+
+
+def test_nonrational_chunks_and_fits(oracle):
+    """P = 0 fit sets from FP64 positions: constant inside every chunk, chunk
+    starts at the change points and every blocksize, and the same chunk table
+    as the host library's own restatement (core.c) of the same rule."""
+    import ctypes as C
+    from coherentsearch_c_amd import abi
+    o, amps = _nonrational_oracle(oracle, nh=16, lobin=2, nbins=1 << 12, blocksize=512)
+    cs = list(o.chunk_starts()) + [o.total]
+    fits = np.zeros((16, o.total), dtype=bool)
+    for h in range(1, 17):
+        for t in range(o.total):
+            fits[h - 1, t] = o.L.orc_harmonic_fits(o.h, h, t)
+    assert not fits.all() and fits.any()
+    for a_, b_ in zip(cs[:-1], cs[1:]):
+        assert b_ > a_
+        assert np.all(fits[:, a_:b_] == fits[:, a_:a_ + 1])
+    import coherentsearch_c_amd as cs_
+    spec = cs_.Spectrum(amps, 1 << 13, synth.DT)
+    p = cs_.make_params(nharms=16, m=16, hidr=0.4999)
+    L = abi.host()
+    plans = abi.DirectPlanSet()
+    assert L.direct_plans_build(C.byref(plans), C.byref(p), 2.0) == 0
+    assert plans.plans[0].P == 0
+    ptr = C.POINTER(C.c_longlong)()
+    nc = C.c_longlong()
+    assert L.chunk_starts(C.byref(plans), C.byref(spec.ft), C.byref(p), o.total, 512,
+                          C.byref(ptr), C.byref(nc)) == 0
+    host = [ptr[i] for i in range(nc.value)]
+    L.free_(C.cast(ptr, C.c_void_p))
+    L.direct_plans_free(C.byref(plans))
+    assert host == list(o.chunk_starts())
+
+
+def test_nonrational_f64_is_not_restated(oracle):
+    """The reference keeps a double stack for a non-rational step in double
+    precision (src/interp.c:464-467); that form is not restated."""
+    with pytest.raises(ValueError):
+        _nonrational_oracle(oracle, precision="f64")
+
+
 def test_pure_tone_recovered(oracle):
     """A noiseless fractional-bin harmonic train is recovered at its trial with
     the m-tap truncation loss only (about 0.2/m, 1.3% at m=16)."""
