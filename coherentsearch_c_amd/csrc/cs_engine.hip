@@ -152,8 +152,11 @@ bool plan_matches(const Plan *P, long long N, double dt, const SearchParams *p, 
 }
 
 /* under E->mu: drop idle plans, least recently used first, until `need`
- * more bytes fit the budget (need < 0: until at most max_plans remain) */
-void evict_idle(CsEngine *E, long long need)
+ * more bytes fit the budget (need < 0: until at most max_plans remain).  The
+ * dropped plans go to `out`, freed by the caller once it has released
+ * E->mu: freeing device memory, streams' graphs and events can wait on the
+ * device, and under the lock it would hold up every other worker. */
+void evict_idle(CsEngine *E, long long need, std::vector<Plan *> &out)
 {
     for (;;) {
         const bool over = need >= 0 ? E->in_use + need > E->budget
@@ -168,8 +171,16 @@ void evict_idle(CsEngine *E, long long need)
         Plan *P = E->plans[victim];
         E->plans.erase(E->plans.begin() + victim);
         E->in_use -= P->footprint;
-        plan_free(P);
+        out.push_back(P);
     }
+}
+
+void free_plans(CsEngine *E, std::vector<Plan *> &ps)
+{
+    if (ps.empty()) return;
+    hipSetDevice(E->device);
+    for (Plan *P : ps) plan_free(P);
+    ps.clear();
 }
 
 int run_request(CsEngine *E, EngReq *r, hipStream_t st)
@@ -302,6 +313,7 @@ int run_request(CsEngine *E, EngReq *r, hipStream_t st)
     rc = 0;
 done:
     hipStreamSynchronize(st);
+    std::vector<Plan *> dropped;
     {
         std::lock_guard<std::mutex> lk(E->mu);
         if (rc == 0 && E->max_plans > 0) {
@@ -315,7 +327,7 @@ done:
                 r->reserved = 0;
                 E->plans.push_back(P);
             }
-            evict_idle(E, -1);
+            evict_idle(E, -1, dropped);
         } else {
             if (reused) {
                 for (size_t i = 0; i < E->plans.size(); i++)
@@ -325,9 +337,10 @@ done:
                     }
                 E->in_use -= P->footprint;
             }
-            plan_free(P);
+            dropped.push_back(P);
         }
     }
+    free_plans(E, dropped);
     return rc;
 }
 
@@ -341,15 +354,20 @@ void worker(CsEngine *E)
     (void)cs_upload_buffer((size_t)32 << 20);
     for (;;) {
         EngReq *r = NULL;
+        std::vector<Plan *> dropped;
         {
             std::unique_lock<std::mutex> lk(E->mu);
             E->cv_work.wait(lk, [&] {
                 if (E->queue.empty()) return E->stop;
                 const EngReq *h = E->queue.front();
-                if (E->in_use + h->footprint > E->budget) evict_idle(E, h->footprint);
+                if (E->in_use + h->footprint > E->budget) evict_idle(E, h->footprint, dropped);
                 return E->in_use == 0 || E->in_use + h->footprint <= E->budget;
             });
-            if (E->queue.empty()) break;
+            if (E->queue.empty()) {
+                lk.unlock();
+                free_plans(E, dropped);
+                break;
+            }
             r = E->queue.front();
             E->queue.pop_front();
             E->in_use += r->footprint;
@@ -360,6 +378,9 @@ void worker(CsEngine *E)
             r->state = 1;
             r->t_start = clk::now();
         }
+        /* memory of plans evicted for this request: freed now, the lock
+         * released (the budget already counts it as returned) */
+        free_plans(E, dropped);
         const int rc = st ? run_request(E, r, st) : -1;
         {
             std::lock_guard<std::mutex> lk(E->mu);
@@ -537,9 +558,13 @@ extern "C" CS_API void cs_engine_free(CsEngine *E)
 extern "C" CS_API void cs_engine_set_plan_cache(CsEngine *E, int max_plans)
 {
     if (!E) return;
-    std::lock_guard<std::mutex> lk(E->mu);
-    E->max_plans = max_plans > 0 ? max_plans : 0;
-    evict_idle(E, -1);
+    std::vector<Plan *> dropped;
+    {
+        std::lock_guard<std::mutex> lk(E->mu);
+        E->max_plans = max_plans > 0 ? max_plans : 0;
+        evict_idle(E, -1, dropped);
+    }
+    free_plans(E, dropped);
 }
 
 extern "C" CS_API void *cs_host_alloc(long long bytes)

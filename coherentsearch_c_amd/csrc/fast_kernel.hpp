@@ -291,11 +291,26 @@ __device__ __forceinline__ T boxcar(const T (&x)[NB], unsigned wmask,
         constexpr int W = wseq(wi);
         if (wmask & (1u << wi)) {
             T m = P[W] - P[0];
-            sfor<NB - 1>([&](auto J) {
-                constexpr int p = decltype(J)::value + 2;
-                const T d = P[p + W - 1] - P[p - 1];
-                m = tmax(m, d);
-            });
+            if constexpr (sizeof(T) == 4) {
+                /* two interleaved maximum chains: fmaxf ignores NaN and is
+                 * exact, so the maximum of the window sums does not depend
+                 * on their order, and the chains' independence leaves the
+                 * scheduler work to overlap */
+                T m1 = -(T)INFINITY;
+                sfor<NB - 1>([&](auto J) {
+                    constexpr int p = decltype(J)::value + 2;
+                    const T d = P[p + W - 1] - P[p - 1];
+                    if constexpr (p % 2 == 0) m1 = tmax(m1, d);
+                    else m = tmax(m, d);
+                });
+                m = tmax(m, m1);
+            } else {
+                sfor<NB - 1>([&](auto J) {
+                    constexpr int p = decltype(J)::value + 2;
+                    const T d = P[p + W - 1] - P[p - 1];
+                    m = tmax(m, d);
+                });
+            }
             constexpr T duty = (T)W / (T)NB;
             const T dt = duty * stot;
             const T c = (m - dt) * invsw[wi];
@@ -664,7 +679,7 @@ void fast_search_kernel(FastArgs a)
                 ar = __builtin_fmaf(w4.z, a2.x, ar); ai = __builtin_fmaf(w4.z, a2.y, ai);
                 ar = __builtin_fmaf(w4.w, a3.x, ar); ai = __builtin_fmaf(w4.w, a3.y, ai);
             });
-            const float4 A = wr[M / 4];
+            const float2 A = reinterpret_cast<const float2 *>(wr + M / 4)[0];
             const float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
             const bool fit = (((h - 1) < 64 ? filled0 : filled1) >> ((h - 1) & 63)) & 1ull;
             X[h] = {fit ? p1 - p2 : 0.0f, fit ? p3 + p4 : 0.0f};

@@ -188,12 +188,14 @@ __global__ __launch_bounds__(CS_TILE) void stack_pass_kernel(FastArgs a, SplitAr
                 wi[uu] = s_C[ic] + qi + carry;
                 wrow[uu] = (unsigned)((g0 + ic) * q + res) * (unsigned)(16 * WS4);
             });
-            float4 w[2][M / 4 + 1];
+            float4 w[2][M / 4];
+            float2 Aw[2];
             sfor<2>([&](auto U) {
                 constexpr int uu = decltype(U)::value;
                 const float4 *__restrict__ wr =
                     reinterpret_cast<const float4 *>(wbase + wrow[uu]);
-                sfor<M / 4 + 1>([&](auto J) { w[uu][decltype(J)::value] = wr[decltype(J)::value]; });
+                sfor<M / 4>([&](auto J) { w[uu][decltype(J)::value] = wr[decltype(J)::value]; });
+                Aw[uu] = reinterpret_cast<const float2 *>(wr + M / 4)[0];
             });
             float2 o[2];
             sfor<2>([&](auto U) {
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(CS_TILE) void stack_pass_kernel(FastArgs a, SplitAr
                     ar = __builtin_fmaf(w4.z, a2.x, ar); ai = __builtin_fmaf(w4.z, a2.y, ai);
                     ar = __builtin_fmaf(w4.w, a3.x, ar); ai = __builtin_fmaf(w4.w, a3.y, ai);
                 });
-                const float4 A = w[uu][M / 4];
+                const float2 A = Aw[uu];
                 const float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
                 const bool fit = (fitm >> (i + uu)) & 1ull;
                 o[uu] = make_float2(fit ? p1 - p2 : 0.0f, fit ? p3 + p4 : 0.0f);
