@@ -267,17 +267,79 @@ __device__ __forceinline__ void c2r(const cf (&X)[LEN], T (&x)[2 * H], TP tw)
 __device__ __forceinline__ float tmax(float m, float d) { return fmaxf(m, d); }
 __device__ __forceinline__ double tmax(double m, double d) { return d > m ? d : m; }
 
+/* Maximum window sum of width W over the NB wrapped windows of a profile:
+ * max over p of P[p+W-1] - P[p-1], in the prefix order of src/boxcar.c:203-252.
+ * f32: two interleaved maximum chains -- fmaxf ignores NaN and is exact, so
+ * the maximum does not depend on the order, and the chains' independence
+ * leaves the scheduler work to overlap. */
+template <int NB, int W, int NP, typename T>
+__device__ __forceinline__ T winmax(const T (&P)[NP])
+{
+    T m = P[W] - P[0];
+    if constexpr (sizeof(T) == 4) {
+        T m1 = -(T)INFINITY;
+        sfor<NB - 1>([&](auto J) {
+            constexpr int p = decltype(J)::value + 2;
+            const T d = P[p + W - 1] - P[p - 1];
+            if constexpr (p % 2 == 0) m1 = tmax(m1, d);
+            else m = tmax(m, d);
+        });
+        m = tmax(m, m1);
+    } else {
+        sfor<NB - 1>([&](auto J) {
+            constexpr int p = decltype(J)::value + 2;
+            const T d = P[p + W - 1] - P[p - 1];
+            m = tmax(m, d);
+        });
+    }
+    return m;
+}
+
+/* Widths whose maximum window sum the f32 boxcar always computes; every
+ * other width is first bounded by a sum of these (bx_parts). */
+constexpr bool bx_anchor(int w) { return w == 1 || w == 4 || w == 13 || w == 42; }
+/* Greedy split of width w into anchor widths: how many of anchor a. */
+constexpr int bx_count(int w, int a)
+{
+    const int n42 = w / 42, r42 = w % 42;
+    const int n13 = r42 / 13, r13 = r42 % 13;
+    const int n4 = r13 / 4, n1 = r13 % 4;
+    return a == 42 ? n42 : a == 13 ? n13 : a == 4 ? n4 : n1;
+}
+constexpr int bx_parts(int w)
+{
+    return bx_count(w, 42) + bx_count(w, 13) + bx_count(w, 4) + bx_count(w, 1);
+}
+
 /* Boxcar S/N of one profile over the enabled widths (mask over the fsp=1.5
  * candidates), src/boxcar.c:203-252 in precision T: wrapped prefix sums, max
  * window sum per width, (m - duty*total) * invsw_w with invsw_w =
- * invsigma*shape_w rounded to T. */
+ * invsigma*shape_w rounded to T.
+ *
+ * The caller only needs the exact value when it exceeds pthr (the candidate
+ * threshold; -inf when every trial's metric is kept).  In f32 a width w that
+ * is not an anchor (1, 4, 13, 42) is first bounded: a window of w splits into
+ * k windows of anchor widths, so in real arithmetic its sum is at most the
+ * sum of those widths' maxima.  Each computed window differs from the real
+ * sum of its profile values by at most 2u(NB+WC+1)*max|P| (u = 2^-24: the
+ * prefix sums' accumulated rounding plus the subtraction's), which gives
+ *     m_w <= sum of anchor maxima + (k+1) * 2u(NB+WC+1) * max|P|;
+ * the margin used, 4u(k+2)(NB+WC+2k+4) * max|P|, also covers the rounding of
+ * the bound's own sum.  Rounding is monotone and invsw_w > 0, so the bound
+ * put through the same (B - duty*total) * invsw_w is at least the width's
+ * exact value: when it is at most pthr the width cannot raise the result
+ * above pthr and its windows are skipped.  The value returned is then the
+ * exact result whenever that exceeds pthr (and at most pthr otherwise).
+ * NaN and infinite profiles make the bound NaN or +inf and are never
+ * skipped. */
 template <int NB, typename T>
 __device__ __forceinline__ T boxcar(const T (&x)[NB], unsigned wmask,
-                                    const T *__restrict__ invsw, int valid)
+                                    const T *__restrict__ invsw, int valid, double pthr)
 {
     constexpr int NC = ncand(NB);
     constexpr int WC = wcmax(NB);
-    T P[NB + WC + 1];
+    constexpr int NP = NB + WC + 1;
+    T P[NP];
     P[0] = T(0);
     sfor<NB + WC>([&](auto I) {
         constexpr int i = decltype(I)::value + 1;
@@ -286,37 +348,62 @@ __device__ __forceinline__ T boxcar(const T (&x)[NB], unsigned wmask,
     });
     const T stot = P[NB] - P[0];
     T best = -(T)INFINITY;
-    sfor<NC>([&](auto I) {
-        constexpr int wi = decltype(I)::value;
-        constexpr int W = wseq(wi);
-        if (wmask & (1u << wi)) {
-            T m = P[W] - P[0];
-            if constexpr (sizeof(T) == 4) {
-                /* two interleaved maximum chains: fmaxf ignores NaN and is
-                 * exact, so the maximum of the window sums does not depend
-                 * on their order, and the chains' independence leaves the
-                 * scheduler work to overlap */
-                T m1 = -(T)INFINITY;
-                sfor<NB - 1>([&](auto J) {
-                    constexpr int p = decltype(J)::value + 2;
-                    const T d = P[p + W - 1] - P[p - 1];
-                    if constexpr (p % 2 == 0) m1 = tmax(m1, d);
-                    else m = tmax(m, d);
-                });
-                m = tmax(m, m1);
-            } else {
-                sfor<NB - 1>([&](auto J) {
-                    constexpr int p = decltype(J)::value + 2;
-                    const T d = P[p + W - 1] - P[p - 1];
-                    m = tmax(m, d);
-                });
-            }
+    if constexpr (sizeof(T) == 4) {
+        /* the slot's per-width factors, read once through the constant
+         * address space (uniform, so scalar loads issued ahead of use) */
+        using cfloat = const __attribute__((address_space(4))) float;
+        const cfloat *civ = (const cfloat *)invsw;
+        float iv[NC];
+        sfor<NC>([&](auto I) { iv[decltype(I)::value] = civ[decltype(I)::value]; });
+        float pmax = 0.0f;
+        sfor<NP>([&](auto I) { pmax = fmaxf(pmax, fabsf(P[decltype(I)::value])); });
+        float ma1 = -INFINITY, ma4 = -INFINITY, ma13 = -INFINITY, ma42 = -INFINITY;
+        sfor<NC>([&](auto I) {
+            constexpr int wi = decltype(I)::value;
+            constexpr int W = wseq(wi);
             constexpr T duty = (T)W / (T)NB;
             const T dt = duty * stot;
-            const T c = (m - dt) * invsw[wi];
-            best = c > best ? c : best;
-        }
-    });
+            if constexpr (bx_anchor(W)) {
+                const T m = winmax<NB, W>(P);
+                if constexpr (W == 1) ma1 = m;
+                if constexpr (W == 4) ma4 = m;
+                if constexpr (W == 13) ma13 = m;
+                if constexpr (W == 42) ma42 = m;
+                if (wmask & (1u << wi)) {
+                    const T c = (m - dt) * iv[wi];
+                    best = c > best ? c : best;
+                }
+            } else if (wmask & (1u << wi)) {
+                constexpr int k = bx_parts(W);
+                constexpr float cm = (float)((double)(4 * (k + 2) * (NB + WC + 2 * k + 4)) *
+                                             0x1p-24);
+                float sb = 0.0f;
+                sfor<bx_count(W, 42)>([&](auto) { sb = sb + ma42; });
+                sfor<bx_count(W, 13)>([&](auto) { sb = sb + ma13; });
+                sfor<bx_count(W, 4)>([&](auto) { sb = sb + ma4; });
+                sfor<bx_count(W, 1)>([&](auto) { sb = sb + ma1; });
+                const float bnd = __builtin_fmaf(pmax, cm, sb);
+                const float cb = (bnd - dt) * iv[wi];
+                if (!(iv[wi] > 0.0f && (double)cb <= pthr)) {
+                    const T m = winmax<NB, W>(P);
+                    const T c = (m - dt) * iv[wi];
+                    best = c > best ? c : best;
+                }
+            }
+        });
+    } else {
+        sfor<NC>([&](auto I) {
+            constexpr int wi = decltype(I)::value;
+            constexpr int W = wseq(wi);
+            if (wmask & (1u << wi)) {
+                const T m = winmax<NB, W>(P);
+                constexpr T duty = (T)W / (T)NB;
+                const T dt = duty * stot;
+                const T c = (m - dt) * invsw[wi];
+                best = c > best ? c : best;
+            }
+        });
+    }
     return valid ? best : T(0);
 }
 
@@ -458,6 +545,9 @@ __device__ __forceinline__ void search_tail(const cf (&X)[NH + 1], const FastArg
     }
 
     const double thr = a.threshold;
+    /* the boxcar may skip widths that cannot exceed the threshold unless
+     * every trial's metric is kept */
+    const double pthr = a.mout ? -(double)INFINITY : thr;
     /* the c2r twiddles of this precision */
     using TwP = std::conditional_t<sizeof(T) == 4, TwF, TwD>;
     const TwP twp{};
@@ -476,7 +566,7 @@ __device__ __forceinline__ void search_tail(const cf (&X)[NH + 1], const FastArg
                                        slot, a);
                 } else {
                     const T mv = boxcar<2 * Hk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
-                                                C->valid[slot]);
+                                                C->valid[slot], pthr);
                     if (a.mout)
                         metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
                     else
@@ -513,7 +603,7 @@ __device__ __forceinline__ void search_tail(const cf (&X)[NH + 1], const FastArg
                     sample_out<nk>(xk, active, lj, C->nvalid[slot], chunk, slot, a);
                 } else {
                     const T mv = boxcar<nk>(xk, a.wmask[slot], invsw_of<T>(C, slot),
-                                            C->valid[slot]);
+                                            C->valid[slot], pthr);
                     if (a.mout)
                         metric_out<T>(active && lj < C->nvalid[slot], slot, i0, lj, mv, a);
                     else
@@ -528,7 +618,7 @@ __device__ __forceinline__ void search_tail(const cf (&X)[NH + 1], const FastArg
         sample_out<2 * NH>(x, active, lj, C->nvalid[0], chunk, 0, a);
         return;
     }
-    const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0]);
+    const T mb = boxcar<2 * NH>(x, a.wmask[0], invsw_of<T>(C, 0), C->valid[0], pthr);
     if (a.mout) metric_out<T>(active, 0, i0, lj, mb, a);
     else emit(active && (double)mb > thr, chunk, 0, lj, mb, a);
 }
