@@ -269,6 +269,9 @@ dm_kernel(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, i
 
 constexpr int kRows = 256;            /* raw rows per subband tile: one per lane */
 constexpr int kTileBytes = 32 * 1024; /* LDS budget of a subband tile */
+constexpr int kLoadBatch = 8;          /* tile loads a lane keeps in flight */
+constexpr int kSpanSub = 4;            /* DM tiles: subband spans loaded per step */
+constexpr int kSpanLoads = 4;          /* DM tiles: samples per lane per span in flight */
 constexpr int kTU = 512;              /* DM samples per DM tile: two per lane, 256 apart */
 constexpr int kDmThreads = 256;
 constexpr int kGMax = 32;             /* DMs per DM tile (register accumulators) */
@@ -316,16 +319,24 @@ template <typename T, int VEC, bool CLIP>
 __global__ void __launch_bounds__(kRows)
 subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chunk, int nchan,
               int cps, int gs, int nsub, int ds, const int *__restrict__ idisp, int maxd,
-              int pitch, long long rows, int nu, float *__restrict__ out, int dsc, ClipView cv)
+              int pitch, long long rows, int nu, float *__restrict__ out, int dsc, ClipView cv,
+              int ngrp, int ntile)
 {
 
     extern __shared__ __align__(16) unsigned char tile[];
     using W = typename Vec<VEC>::type;
-    /* x: subband group (fast, so one row tile's groups run together and
-     * read whole rows), y: row tile */
-    const int s0 = (int)blockIdx.x * gs, ns = min(gs, nsub - s0);
+    /* XCD-aware order: workgroup b runs on XCD b % 8, and the grid (a
+     * multiple of 8) gives each XCD a contiguous run of (row tile, subband
+     * group) pairs, groups fastest.  A row tile's groups and the next row
+     * tile (whose first rows are this one's delay rows) then share one L2:
+     * the row segments two groups share and the delay rows are read from
+     * HBM once instead of once per XCD. */
+    const int wg = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    if (wg >= ngrp * ntile) return;
+    const int gi = wg % ngrp, ti = wg / ngrp;
+    const int s0 = gi * gs, ns = min(gs, nsub - s0);
     const int c0 = s0 * cps;
-    const long long t0 = (long long)blockIdx.y * kRows;
+    const long long t0 = (long long)ti * kRows;
     const int nr = kRows + maxd;
     /* with clipping: the rows' clip flags in LDS after the tile; a clipped
      * row's channel levels replace its samples where the sums read them */
@@ -349,15 +360,37 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
         const int rpi = nv <= kRows ? kRows / nv : 0;
         if (rpi) {
             const int lr = (int)threadIdx.x / nv, v = (int)threadIdx.x - lr * nv;
+            /* batches of kLoadBatch loads in flight before their LDS stores:
+             * one memory round trip per batch instead of one per row */
+            /* row t's segment sits at ra + t*rowb + col while t < chunk and
+             * at rb + (t - chunk)*rowb + col after: per load one offset add
+             * of a scalar step and a choice of base, instead of a 64-bit
+             * multiply.  Rows are compared as 32-bit tile rows: r < rlim
+             * exists, r < rch lies in chunk ra. */
+            const long long rowb = (long long)nchan * (long long)sizeof(T);
+            const long long col = (long long)c0 * (long long)sizeof(T) + (long long)v * VEC;
+            const long long step = (long long)rpi * rowb, cb = chunk * rowb;
+            const int rlim = (int)min((long long)nr, rows - t0);
+            const int rch = (int)max(-1ll, min((long long)nr, chunk - t0));
+            const unsigned char *rab = (const unsigned char *)ra, *rbb = (const unsigned char *)rb;
             if (lr < rpi)
-                for (int r = lr; r < nr; r += rpi) {
-                    const long long t = t0 + r;
-                    W w{};
-                    if (t < rows) {
-                        const T *row = t < chunk ? ra + t * nchan : rb + (t - chunk) * nchan;
-                        w = *(const W *)((const unsigned char *)(row + c0) + (size_t)v * VEC);
+                for (int rb0 = lr; rb0 < nr; rb0 += kLoadBatch * rpi) {
+                    W w[kLoadBatch];
+                    const long long o0 = (t0 + rb0) * rowb + col;
+#pragma unroll
+                    for (int i = 0; i < kLoadBatch; i++) {
+                        const int r = rb0 + i * rpi;
+                        w[i] = W{};
+                        if (r < rlim) {
+                            const long long o = o0 + (long long)i * step;
+                            w[i] = *(const W *)(r < rch ? rab + o : rbb + (o - cb));
+                        }
                     }
-                    tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w);
+#pragma unroll
+                    for (int i = 0; i < kLoadBatch; i++) {
+                        const int r = rb0 + i * rpi;
+                        if (r < nr) tile_store<VEC>(tile + (size_t)r * pitch + (size_t)v * VEC, w[i]);
+                    }
                 }
         } else {
             for (int idx = (int)threadIdx.x; idx < nr * nv; idx += kRows) {
@@ -376,29 +409,52 @@ subband_tiled(const T *__restrict__ ra, const T *__restrict__ rb, long long chun
     const int r = (int)threadIdx.x;
     const int k = r % ds, base = (int)(threadIdx.x & 63) - k;
     const long long u = (t0 + r) / ds;
+    /* lane r's row in the tile; a channel's sample sits d*pitch + ch*size
+     * further on, a uniform offset (scalar arithmetic), so each sample costs
+     * one vector add for its address */
+    const float rds = 1.0f / (float)ds;
+    int lb = r * pitch;
+    /* opaque to the optimiser, which would otherwise fold lb + d*pitch back
+     * into (r + d)*pitch: a quarter-rate vector multiply per sample */
+    asm volatile("" : "+v"(lb));
     /* the sums, with or without the clip test (the workgroup takes one
      * version: its loop stays as tight as the unclipped kernel's) */
     auto sums = [&](auto withclip) {
         for (int s = 0; s < ns; s++) {
             float v = 0.0f;
-            for (int j = 0; j < cps; j++) {
-                const int ch = s * cps + j;
-                const int d = idisp[c0 + ch];
-                float x =
-                    (float)*(const T *)(tile + (size_t)(r + d) * pitch + (size_t)ch * sizeof(T));
-                if constexpr (decltype(withclip)::value) {
-                    if (rflag[r + d]) {
-                        const long long b = (cv.row0 + t0 + r + d) / cv.P;
-                        x = cv.levels[b * nchan + c0 + ch];
-                    }
+            if (sizeof(T) == 1 && !decltype(withclip)::value && cps <= 65535) {
+                /* 8-bit samples: every partial sum of the float loop is an
+                 * integer below 255 * 65536 < 2^24, so each float addition is
+                 * exact and the sum equals the integer sum: accumulate in
+                 * integers and convert once (the same bits) */
+                unsigned vi = 0;
+                for (int j = 0; j < cps; j++) {
+                    const int ch = s * cps + j;
+                    vi += (unsigned)*(const T *)(tile + lb + idisp[c0 + ch] * pitch + ch);
                 }
-                v += x;
+                v = (float)vi;
+            } else {
+                for (int j = 0; j < cps; j++) {
+                    const int ch = s * cps + j;
+                    const int d = idisp[c0 + ch];
+                    float x = (float)*(const T *)(tile + lb + (d * pitch + ch * (int)sizeof(T)));
+                    if constexpr (decltype(withclip)::value) {
+                        if (rflag[r + d]) {
+                            const long long b = (cv.row0 + t0 + r + d) / cv.P;
+                            x = cv.levels[b * nchan + c0 + ch];
+                        }
+                    }
+                    v += x;
+                }
             }
             float acc = v;
             if (ds > 1) {
                 acc = 0.0f;
                 for (int q = 0; q < ds; q++) acc += __shfl(v, base + q, 64);
-                acc /= (float)ds;
+                /* ds divides 64 here, a power of two: the quotient by ds
+                 * and the product by its exact reciprocal round the same
+                 * real number */
+                acc *= rds;
             }
             if (k == 0 && u < nu) out[(long long)(s0 + s) * dsc + u] = acc;
         }
@@ -424,14 +480,19 @@ __global__ void __launch_bounds__(kDmThreads)
 dm_tiled(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, int nsub,
          int sbatch, const int *__restrict__ gmin, const int *__restrict__ glen,
          const int *__restrict__ gbase, const int *__restrict__ sofs, int numdms, int nu,
-         int tlim, float *__restrict__ out, long long numout, long long v0)
+         int tlim, float *__restrict__ out, long long numout, long long v0, int ngrp,
+         int ntile)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     float *span = (float *)smem;
-    /* x: DM group (fast: the groups of one sample tile read the same
-     * subband samples), y: sample tile */
-    const int g = (int)blockIdx.x, i0 = g * G, ng = min(G, numdms - i0);
-    const int u0 = (int)blockIdx.y * kTU, tid = (int)threadIdx.x;
+    /* XCD-aware order (the grid a multiple of 8, workgroup b on XCD b % 8):
+     * each XCD takes a contiguous run of (sample tile, DM group) pairs,
+     * groups fastest, so the groups of one sample tile -- which read the
+     * same subband samples -- share an L2 */
+    const int wg = (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+    if (wg >= ngrp * ntile) return;
+    const int g = wg % ngrp, i0 = g * G, ng = min(G, numdms - i0);
+    const int u0 = (wg / ngrp) * kTU, tid = (int)threadIdx.x;
     const int *mn = gmin + (size_t)g * nsub, *ln = glen + (size_t)g * nsub,
               *bs = gbase + (size_t)g * nsub;
     float acc[G], acc2[G];
@@ -440,13 +501,44 @@ dm_tiled(const float *__restrict__ sa, const float *__restrict__ sb, int dsc, in
     for (int sA = 0; sA < nsub; sA += sbatch) {
         const int sZ = min(nsub, sA + sbatch);
         if (sA) __syncthreads();
-        for (int s = sA; s < sZ; s++) {
-            const int len = ln[s], b = bs[s], tb = u0 + mn[s];
-            for (int q = tid; q < len; q += kDmThreads) {
-                const int t = tb + q;
-                float v = 0.0f;
-                if (t < tlim) v = t < dsc ? sa[(size_t)s * dsc + t] : sb[(size_t)s * dsc + (t - dsc)];
-                span[b + q] = v;
+        /* kSpanSub subbands at a time, each lane's first kSpanLoads samples
+         * of each loaded before any is stored: one memory round trip per
+         * step instead of one per sample (longer spans finish in a loop) */
+        for (int s0 = sA; s0 < sZ; s0 += kSpanSub) {
+            float v[kSpanSub][kSpanLoads];
+            int lj[kSpanSub], bj[kSpanSub], tj[kSpanSub];
+#pragma unroll
+            for (int j = 0; j < kSpanSub; j++) {
+                const int s = s0 + j;
+                lj[j] = s < sZ ? ln[s] : 0;
+                bj[j] = s < sZ ? bs[s] : 0;
+                tj[j] = s < sZ ? u0 + mn[s] : 0;
+            }
+#pragma unroll
+            for (int j = 0; j < kSpanSub; j++) {
+                const size_t so = (size_t)(s0 + j) * dsc;
+#pragma unroll
+                for (int k = 0; k < kSpanLoads; k++) {
+                    const int q = tid + k * kDmThreads, t = tj[j] + q;
+                    v[j][k] = 0.0f;
+                    if (q < lj[j] && t < tlim) v[j][k] = t < dsc ? sa[so + t] : sb[so + (t - dsc)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kSpanSub; j++)
+#pragma unroll
+                for (int k = 0; k < kSpanLoads; k++) {
+                    const int q = tid + k * kDmThreads;
+                    if (q < lj[j]) span[bj[j] + q] = v[j][k];
+                }
+            for (int j = 0; j < kSpanSub; j++) {
+                const size_t so = (size_t)(s0 + j) * dsc;
+                for (int q = tid + kSpanLoads * kDmThreads; q < lj[j]; q += kDmThreads) {
+                    const int t = tj[j] + q;
+                    float x = 0.0f;
+                    if (t < tlim) x = t < dsc ? sa[so + t] : sb[so + (t - dsc)];
+                    span[bj[j] + q] = x;
+                }
             }
         }
         __syncthreads();
@@ -1599,14 +1691,16 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             /* rows the tiles may load: those the pass reads (its samples'
              * rows plus the deepest delay), never past the data */
             const long long nrow = nu * ds, here = std::min(rows - j * C, nrow + t.maxd);
-            const dim3 grid((unsigned)((nsub + sb_gs - 1) / sb_gs),
-                            (unsigned)((nrow + kRows - 1) / kRows));
+            const int ngrp = (nsub + sb_gs - 1) / sb_gs;
+            const int ntile = (int)((nrow + kRows - 1) / kRows);
+            const dim3 grid((unsigned)(((long long)ngrp * ntile + 7) / 8 * 8));
             const size_t lds = (size_t)(kRows + t.maxd) * sb_pitch +
                                (clip ? (size_t)(kRows + t.maxd + 3) / 4 * 4 : 0);
 #define CS_SUB(V, K)                                                                       \
     subband_tiled<T, V, K><<<grid, kRows, lds, d->st>>>(ra, rb, C, nch, cps, sb_gs, nsub, ds, \
                                                         d->d_idisp, t.maxd, sb_pitch, here, \
-                                                        (int)nu, sub, (int)dsc, clip_view(j))
+                                                        (int)nu, sub, (int)dsc, clip_view(j), \
+                                                        ngrp, ntile)
             if (clip) {
                 if (sb_vec == 16) CS_SUB(16, true);
                 else if (sb_vec == 4 || sizeof(T) != 1) CS_SUB(4, true);
@@ -1642,11 +1736,13 @@ static int run_passes(CsDedisp *d, const T *spectra, bool on_device)
             const int *gmin = d->d_gtab, *glen = gmin + (size_t)ng * nsub,
                       *gbase = glen + (size_t)ng * nsub, *sofs = gbase + (size_t)ng * nsub;
             const int tlim = (int)(nv + t.maxoff);
-            const dim3 grid((unsigned)ng, (unsigned)((nv + kTU - 1) / kTU));
+            const int ntile = (int)((nv + kTU - 1) / kTU);
+            const dim3 grid((unsigned)(((long long)ng * ntile + 7) / 8 * 8));
 #define CS_DM(G)                                                                           \
     dm_tiled<G><<<grid, kDmThreads, (size_t)d->dm_lds, d->st>>>(sa, sb, (int)dsc, nsub, d->dm_sbatch, \
                                                          gmin, glen, gbase, sofs, c.numdms,  \
-                                                         (int)nv, tlim, d->d_out, t.numout, v0)
+                                                         (int)nv, tlim, d->d_out, t.numout, v0, \
+                                                         ng, ntile)
             switch (d->dm_g) {
             case 32: CS_DM(32); break;
             case 16: CS_DM(16); break;
