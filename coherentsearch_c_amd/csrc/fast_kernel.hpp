@@ -73,6 +73,13 @@ constexpr int modinv(int a, int m)
 }
 constexpr int twoff(int N) { return N * (N - 1) / 2; }
 
+/* largest power of two <= n (n >= 1) */
+constexpr int pow2_floor(int n)
+{
+    int p = 1;
+    while (2 * p <= n) p *= 2;
+    return p;
+}
 /* width candidates: the fsp = 1.5 bank 1,2,3,4,6,9,13,... (src/boxcar.c:78-92) */
 constexpr int wseq(int i)
 {
@@ -727,17 +734,32 @@ void fast_search_kernel(FastArgs a)
      * suffices (a batched loop would let the compiler hoist every harmonic's
      * residue walk out of it and hold it in registers). */
     const int cnt = s_off[NH];
-    for (int idx = tid; idx < cnt; idx += CS_TILE) {
-        int lo = 0, hi = NH - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= idx) lo = mid;
-            else hi = mid - 1;
+    /* every bin a lane stages in flight before its LDS stores: one memory
+     * round trip for the tile's windows (SB covers CS_WIN_CAP, so the loop
+     * runs once); each bin's harmonic is the last window starting at or
+     * before it, found by a fixed-step binary search so that the searches
+     * interleave */
+    constexpr int SB = (CS_WIN_CAP(NH, M) + CS_TILE - 1) / CS_TILE;
+    for (int base = tid; base < cnt; base += SB * CS_TILE) {
+        float2 v[SB];
+#pragma unroll
+        for (int k = 0; k < SB; k++) {
+            const int idx = base + k * CS_TILE;
+            int lo = 0;
+#pragma unroll
+            for (int st = pow2_floor(NH); st >= 1; st >>= 1) {
+                const int c = min(lo + st, NH - 1);
+                lo = s_off[c] <= idx ? c : lo;
+            }
+            const long long ai = (long long)s_lo[lo] + (idx - s_off[lo]);
+            v[k] = make_float2(0.0f, 0.0f);
+            if (idx < cnt && ai >= 0 && ai < a.namps) v[k] = a.amps[ai];
         }
-        const long long ai = (long long)s_lo[lo] + (idx - s_off[lo]);
-        float2 v = make_float2(0.0f, 0.0f);
-        if (ai >= 0 && ai < a.namps) v = a.amps[ai];
-        win[idx] = v;
+#pragma unroll
+        for (int k = 0; k < SB; k++) {
+            const int idx = base + k * CS_TILE;
+            if (idx < cnt) win[idx] = v[k];
+        }
     }
     __syncthreads();
 

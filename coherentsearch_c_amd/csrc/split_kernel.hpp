@@ -27,6 +27,9 @@
 
 namespace csk {
 
+/* window bins a stack-pass lane stages per memory round trip */
+constexpr int kSplitStage = 12;
+
 struct SplitArgs {
     float2 *stk;          /* stack of the slice's tiles */
     long long tstride;    /* float2 elements from one tile's stack to the next */
@@ -141,26 +144,26 @@ __global__ __launch_bounds__(CS_TILE) void stack_pass_kernel(FastArgs a, SplitAr
         __syncthreads();
         if (tid < gn) s_C[tid] += s_off[tid];
         const int cnt = s_off[gn];
-        /* four bins per thread in flight: every load issued before the LDS
-         * stores */
-        for (int base = tid; base < cnt; base += 4 * CS_TILE) {
-            float2 v[4];
-            sfor<4>([&](auto U) {
+        /* kSplitStage bins per thread in flight (a group's windows up to
+         * 24 KB in one memory round trip): every load issued before the LDS
+         * stores; each bin's harmonic by a fixed-step binary search (gn <=
+         * 64), so that the searches interleave */
+        for (int base = tid; base < cnt; base += kSplitStage * CS_TILE) {
+            float2 v[kSplitStage];
+            sfor<kSplitStage>([&](auto U) {
                 constexpr int uu = decltype(U)::value;
                 const int idx = base + uu * CS_TILE;
                 v[uu] = make_float2(0.0f, 0.0f);
-                if (idx < cnt) {
-                    int lo = 0, hi = gn - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (s_off[mid] <= idx) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    const long long ai = (long long)s_lo[lo] + (idx - s_off[lo]);
-                    if (ai >= 0 && ai < a.namps) v[uu] = a.amps[ai];
+                int lo = 0;
+#pragma unroll
+                for (int st = 32; st >= 1; st >>= 1) {
+                    const int c = min(lo + st, gn - 1);
+                    lo = s_off[c] <= idx ? c : lo;
                 }
+                const long long ai = (long long)s_lo[lo] + (idx - s_off[lo]);
+                if (idx < cnt && ai >= 0 && ai < a.namps) v[uu] = a.amps[ai];
             });
-            sfor<4>([&](auto U) {
+            sfor<kSplitStage>([&](auto U) {
                 constexpr int uu = decltype(U)::value;
                 const int idx = base + uu * CS_TILE;
                 if (idx < cnt) win[idx] = v[uu];
