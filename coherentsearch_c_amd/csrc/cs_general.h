@@ -93,6 +93,8 @@ struct GenArgs {
                                    the work buffer already (the stack pass of
                                    split_kernel.hpp), workgroup u takes tile tile0 + u */
     int tile0, nslice;
+    int fslot;                  /* first of nh / 2 value slots a divisor decimation's fold of
+                                   the base profile is written to before its boxcar */
 };
 
 /* work buffer bytes for nvslots value slots and nh harmonics over nlanes lanes */

@@ -15,9 +15,11 @@
  *   3. per decimation slot the boxcar matched filter over the enabled widths
  *      (src/boxcar.c:203-252): prefix sums computed in order through a
  *      register block of B + WC + 1 values that slides along the profile, so
- *      every prefix value and window difference is the reference's own;
- *      divisor decimations fold the base profile on the fly
- *      (src/search.c:294-331);
+ *      every prefix value and window difference is the reference's own,
+ *      read a complex slot (two profile values) per load; a divisor
+ *      decimation's fold of the base profile (src/search.c:294-331) is
+ *      written to its own value slots first, eight bin pairs' loads in
+ *      flight per term;
  *   4. above-threshold records (one atomic per wave), or the metric
  *      collection / measured-sigma sample outputs of the other passes.
  */
@@ -386,6 +388,24 @@ struct Profile {
         for (int q = 1; q < fold; q++) s = s + raw(i + q * nk);
         return s * invk;
     }
+    /* values i and i + 1 for an even i from one load of each complex slot
+     * they share: nk is even, so i + q*nk and i + 1 + q*nk are the two parts
+     * of one slot for every q, and each sum keeps the order of
+     * operator() */
+    __host__ __device__ __forceinline__ void pair(int i, T &e, T &o) const
+    {
+        const cx<T> z = L->v(zout + (i >> 1));
+        e = z.x;
+        o = z.y;
+        if (!fold) return;
+        for (int q = 1; q < fold; q++) {
+            const cx<T> w = L->v(zout + ((i + q * nk) >> 1));
+            e = e + w.x;
+            o = o + w.y;
+        }
+        e = e * invk;
+        o = o * invk;
+    }
 };
 
 /* Boxcar S/N of an n-bin profile over the enabled width candidates:
@@ -400,15 +420,29 @@ __host__ __device__ __forceinline__ T boxcar_stream(const PX &px, int n, unsigne
 {
     constexpr int NC = ncand_upto(WC);
     constexpr int LEN = B + WC + 1;
+    static_assert(B % 2 == 0, "blocks of whole value pairs");
     T Pw[LEN];
     T m[NC];
     sfor<NC>([&](auto I) { m[decltype(I)::value] = -(T)INFINITY; });
     Pw[0] = T(0);
     T Pn = T(0);
     int idx = 0;              /* (I - 1) mod n of the next prefix index I */
+    /* n is even and idx starts at 0, so the c-th value read (from 0) has an
+     * idx of c's parity: an even idx loads the pair (idx, idx + 1) and keeps
+     * the odd value for the next read */
+    T odd = T(0);
+    auto next = [&](auto C2) -> T {
+        if constexpr (decltype(C2)::value == 0) {
+            T e;
+            px.pair(idx, e, odd);
+            return e;
+        } else {
+            return odd;
+        }
+    };
     sfor<LEN - 1>([&](auto J) {
         constexpr int j = decltype(J)::value + 1;
-        Pw[j] = Pw[j - 1] + px(idx);
+        Pw[j] = Pw[j - 1] + next(std::integral_constant<int, (j - 1) & 1>{});
         if (j == n) Pn = Pw[j];
         idx = idx + 1 == n ? 0 : idx + 1;
     });
@@ -428,9 +462,10 @@ __host__ __device__ __forceinline__ T boxcar_stream(const PX &px, int n, unsigne
                 constexpr int j = decltype(J)::value;
                 Pw[j] = Pw[j + B];
             });
+            /* LEN - 1 + (a multiple of B) values read before this block */
             sfor<B>([&](auto J) {
                 constexpr int j = decltype(J)::value + WC + 1;
-                Pw[j] = Pw[j - 1] + px(idx);
+                Pw[j] = Pw[j - 1] + next(std::integral_constant<int, (LEN - 1 + j - WC - 1) & 1>{});
                 if (p0 + B - 1 + j == n) Pn = Pw[j];
                 idx = idx + 1 == n ? 0 : idx + 1;
             });
@@ -449,6 +484,42 @@ __host__ __device__ __forceinline__ T boxcar_stream(const PX &px, int n, unsigne
         }
     });
     return valid ? best : T(0);
+}
+
+/* The fold of the base profile by a divisor decimation k (src/search.c:298-331)
+ * into value slots zout .. zout + Hk - 1: bin pair j of the fold is
+ * (sum_q z[j + q*Hk]) * (1/k), q = 0 .. k-1 in order, both parts summed as
+ * the profile reader's operator() sums them.  Eight pairs at a time, so each
+ * term's loads are in flight together instead of one dependent load per
+ * term and bin. */
+template <typename T>
+__host__ __device__ __forceinline__ void fold_into(const Lane<T> &L, int zin, int k, int Hk,
+                                                   int zout)
+{
+    constexpr int U = 8;
+    const T invk = T(1) / (T)k;
+    for (int j0 = 0; j0 < Hk; j0 += U) {
+        cx<T> acc[U];
+        sfor<U>([&](auto I) {
+            constexpr int u = decltype(I)::value;
+            acc[u] = L.v(zin + (j0 + u < Hk ? j0 + u : Hk - 1));
+        });
+        for (int q = 1; q < k; q++) {
+            cx<T> w[U];
+            sfor<U>([&](auto I) {
+                constexpr int u = decltype(I)::value;
+                w[u] = L.v(zin + (j0 + u < Hk ? j0 + u : Hk - 1) + q * Hk);
+            });
+            sfor<U>([&](auto I) {
+                constexpr int u = decltype(I)::value;
+                acc[u] = {acc[u].x + w[u].x, acc[u].y + w[u].y};
+            });
+        }
+        sfor<U>([&](auto I) {
+            constexpr int u = decltype(I)::value;
+            if (j0 + u < Hk) L.v(zout + j0 + u) = {acc[u].x * invk, acc[u].y * invk};
+        });
+    }
 }
 
 /* the radix-R butterfly of the project c2r for a small odd prime R, with
@@ -677,9 +748,8 @@ __global__ __launch_bounds__(CS_TILE) void general_search_kernel(GenArgs a)
             if (a.slot_c2r[s] >= 0) {
                 px.zout = a.c2r[a.slot_c2r[s]].zout;
             } else {
-                px.zout = a.c2r[0].zout;
-                px.fold = k;
-                px.invk = T(1) / (T)k;
+                fold_into<T>(L, a.c2r[0].zout, k, Hk, a.fslot);
+                px.zout = a.fslot;
             }
             const int nv = s == 0 ? C->n : C->nvalid[s];
             if (a.samp) {

@@ -600,7 +600,9 @@ static int gen_setup(GpuSession *S)
         fprintf(stderr, "Error: c2r plan upload failed\n");
         return -1;
     }
-    const int nv = P.nslots + P.nscratch;
+    /* value slots, butterfly scratch, then the fold of the base profile
+     * (nh / 2 complex slots: a fold by k has nh / k bin pairs) */
+    const int nv = P.nslots + P.nscratch + S->nh / 2;
     const int grid = gen_grid(nv, S->nh, S->f64, S->args.ntiles);
     g.nlanes = (long long)grid * CS_TILE;
     const size_t wbytes = cs_gen_work_bytes(nv, S->nh, g.nlanes, S->f64);
@@ -630,6 +632,7 @@ static int gen_setup(GpuSession *S)
     for (int c = 0; c < g.nc2r; c++) g.c2r[c] = P.c2r[(size_t)c];
     g.nslots = P.nslots;
     g.nvslots = nv;
+    g.fslot = P.nslots + P.nscratch;
     g.work = S->d_work;
     return 0;
 }
