@@ -18,7 +18,7 @@
  *      every prefix value and window difference is the reference's own,
  *      read a complex slot (two profile values) per load; a divisor
  *      decimation's fold of the base profile (src/search.c:294-331) is
- *      written to its own value slots first, eight bin pairs' loads in
+ *      written to its own value slots first, 32 bin pairs' loads in
  *      flight per term;
  *   4. above-threshold records (one atomic per wave), or the metric
  *      collection / measured-sigma sample outputs of the other passes.
@@ -258,6 +258,25 @@ int cs_gen_build_plan(int nh, int nextra, const int *Hk, const int *ks, CsGenPla
         }
         P->ops.push_back(g);
     }
+    /* run lengths: op i leads the ops i .. i + pad - 1 when they are all
+     * butterflies of its kind and radix and none reads a value another of
+     * them writes (the kernel loads such a run's operands together) */
+    for (size_t i = 0; i < b.ops.size(); i++) {
+        const SOp &o = b.ops[i];
+        if (o.kind != CS_OP_BF2 && o.kind != CS_OP_BF4 && o.kind != CS_OP_BFP) continue;
+        std::vector<int> outs(o.out);
+        size_t j = i + 1;
+        for (; j < b.ops.size() && j - i < 64; j++) {
+            const SOp &n = b.ops[j];
+            if (n.kind != o.kind || n.R != o.R) break;
+            bool dep = false;
+            for (int v : n.in)
+                if (std::find(outs.begin(), outs.end(), v) != outs.end()) dep = true;
+            if (dep) break;
+            outs.insert(outs.end(), n.out.begin(), n.out.end());
+        }
+        P->ops[i].pad = (int)(j - i);
+    }
     nres = 0;
     for (const Top &t : tops) {
         CsGenC2R c;
@@ -446,7 +465,26 @@ __host__ __device__ __forceinline__ T boxcar_stream(const PX &px, int n, unsigne
         if (j == n) Pn = Pw[j];
         idx = idx + 1 == n ? 0 : idx + 1;
     });
+    /* the prefetch's B registers cost no occupancy for f32 at WC 63 (two
+     * waves per SIMD either way) and 211 (one); at 141 they would halve it,
+     * and in f64 they spill */
+    constexpr bool PF = sizeof(T) == 4 && (WC < 100 || WC > 200);
     for (int p0 = 1; p0 <= n; p0 += B) {
+        /* the next block's pair loads issued before this block's windows
+         * (read jj of a block has idx + jj's parity, that of LEN - 1 + jj,
+         * and the even reads are jj / 2 = 0 .. B/2 - 1 in order) */
+        const bool more = p0 + B <= n;
+        cx<T> nx[B / 2];
+        if (PF && more) {
+            sfor<B>([&](auto J) {
+                constexpr int jj = decltype(J)::value;
+                if constexpr (((LEN - 1 + jj) & 1) == 0) {
+                    int ii = idx + jj;
+                    while (ii >= n) ii -= n;
+                    px.pair(ii, nx[jj / 2].x, nx[jj / 2].y);
+                }
+            });
+        }
         sfor<NC>([&](auto I) {
             constexpr int wi = decltype(I)::value;
             constexpr int W = wseq(wi);
@@ -457,15 +495,24 @@ __host__ __device__ __forceinline__ T boxcar_stream(const PX &px, int n, unsigne
                 });
             }
         });
-        if (p0 + B <= n) {
+        if (more) {
             sfor<WC + 1>([&](auto J) {
                 constexpr int j = decltype(J)::value;
                 Pw[j] = Pw[j + B];
             });
-            /* LEN - 1 + (a multiple of B) values read before this block */
             sfor<B>([&](auto J) {
-                constexpr int j = decltype(J)::value + WC + 1;
-                Pw[j] = Pw[j - 1] + next(std::integral_constant<int, (LEN - 1 + j - WC - 1) & 1>{});
+                constexpr int jj = decltype(J)::value;
+                constexpr int j = jj + WC + 1;
+                T v;
+                if constexpr (!PF) {
+                    v = next(std::integral_constant<int, (LEN - 1 + jj) & 1>{});
+                } else if constexpr (((LEN - 1 + jj) & 1) == 0) {
+                    v = nx[jj / 2].x;
+                    odd = nx[jj / 2].y;
+                } else {
+                    v = odd;
+                }
+                Pw[j] = Pw[j - 1] + v;
                 if (p0 + B - 1 + j == n) Pn = Pw[j];
                 idx = idx + 1 == n ? 0 : idx + 1;
             });
@@ -489,14 +536,14 @@ __host__ __device__ __forceinline__ T boxcar_stream(const PX &px, int n, unsigne
 /* The fold of the base profile by a divisor decimation k (src/search.c:298-331)
  * into value slots zout .. zout + Hk - 1: bin pair j of the fold is
  * (sum_q z[j + q*Hk]) * (1/k), q = 0 .. k-1 in order, both parts summed as
- * the profile reader's operator() sums them.  Eight pairs at a time, so each
+ * the profile reader's operator() sums them.  32 pairs at a time, so each
  * term's loads are in flight together instead of one dependent load per
  * term and bin. */
 template <typename T>
 __host__ __device__ __forceinline__ void fold_into(const Lane<T> &L, int zin, int k, int Hk,
                                                    int zout)
 {
-    constexpr int U = 8;
+    constexpr int U = 32;
     const T invk = T(1) / (T)k;
     for (int j0 = 0; j0 < Hk; j0 += U) {
         cx<T> acc[U];
@@ -522,51 +569,96 @@ __host__ __device__ __forceinline__ void fold_into(const Lane<T> &L, int zin, in
     }
 }
 
-/* the radix-R butterfly of the project c2r for a small odd prime R, with
- * its input twiddles (the oracle's dft_ for R odd) */
+/* A radix-R butterfly op of the project c2r with its input twiddles (the
+ * oracle's dft_ step), R = 2, 4 or a small odd prime, in two halves: the
+ * operand loads (each times its twiddle) and the outputs. */
 template <int R, typename T>
-__host__ __device__ __forceinline__ void bfp_small(const Lane<T> &L, const int *__restrict__ pl,
-                                          const cx<T> *__restrict__ tw)
+__host__ __device__ __forceinline__ void bf_load(const Lane<T> &L, const int *__restrict__ pl,
+                                                 const cx<T> *__restrict__ tw, cx<T> (&u)[R])
 {
-    constexpr int hh = (R - 1) / 2;
-    cx<T> u[R];
     sfor<R>([&](auto I) {
         constexpr int r = decltype(I)::value;
         const cx<T> y = L.v(pl[r]);
         const int t = pl[R + r];
         u[r] = t < 0 ? y : cmul(y, tw[t]);
     });
-    const int rtw = pl[3 * R];
-    cx<T> sp[hh + 1], sm[hh + 1];
-    sfor<hh>([&](auto I) {
-        constexpr int r = decltype(I)::value + 1;
-        sp[r] = {u[r].x + u[R - r].x, u[r].y + u[R - r].y};
-        sm[r] = {u[r].x - u[R - r].x, u[r].y - u[R - r].y};
-    });
-    cx<T> o[R];
-    sfor<hh>([&](auto S) {
-        constexpr int s = decltype(S)::value + 1;
-        cx<T> A = u[0], Bv = {T(0), T(0)};
+}
+
+template <int R, typename T>
+__host__ __device__ __forceinline__ void bf_store(const Lane<T> &L, const int *__restrict__ pl,
+                                                  const cx<T> *__restrict__ tw,
+                                                  const cx<T> (&u)[R])
+{
+    if constexpr (R == 2) {
+        L.v(pl[4]) = {u[0].x + u[1].x, u[0].y + u[1].y};
+        L.v(pl[5]) = {u[0].x - u[1].x, u[0].y - u[1].y};
+    } else if constexpr (R == 4) {
+        const cx<T> t0 = {u[0].x + u[2].x, u[0].y + u[2].y};
+        const cx<T> t1 = {u[0].x - u[2].x, u[0].y - u[2].y};
+        const cx<T> t2 = {u[1].x + u[3].x, u[1].y + u[3].y};
+        const cx<T> t3 = {u[1].x - u[3].x, u[1].y - u[3].y};
+        L.v(pl[8]) = {t0.x + t2.x, t0.y + t2.y};
+        L.v(pl[10]) = {t0.x - t2.x, t0.y - t2.y};
+        L.v(pl[9]) = {t1.x - t3.y, t1.y + t3.x};
+        L.v(pl[11]) = {t1.x + t3.y, t1.y - t3.x};
+    } else {
+        /* odd prime: sums and differences of mirrored inputs, then per
+         * output pair A = u0 + sum sp_r c (fused, left to right) and
+         * B = sum sm_r sn (first term a product, the rest fused) */
+        constexpr int hh = (R - 1) / 2;
+        const int rtw = pl[3 * R];
+        cx<T> sp[hh + 1], sm[hh + 1];
         sfor<hh>([&](auto I) {
             constexpr int r = decltype(I)::value + 1;
-            constexpr int j = (r * s) % R;
-            const cx<T> w = tw[rtw + j];
-            A = {fm(sp[r].x, w.x, A.x), fm(sp[r].y, w.x, A.y)};
-            if constexpr (r == 1) Bv = {sm[r].x * w.y, sm[r].y * w.y};
-            else Bv = {fm(sm[r].x, w.y, Bv.x), fm(sm[r].y, w.y, Bv.y)};
+            sp[r] = {u[r].x + u[R - r].x, u[r].y + u[R - r].y};
+            sm[r] = {u[r].x - u[R - r].x, u[r].y - u[R - r].y};
         });
-        o[s] = {A.x - Bv.y, A.y + Bv.x};
-        o[R - s] = {A.x + Bv.y, A.y - Bv.x};
+        cx<T> o[R];
+        sfor<hh>([&](auto S) {
+            constexpr int s = decltype(S)::value + 1;
+            cx<T> A = u[0], Bv = {T(0), T(0)};
+            sfor<hh>([&](auto I) {
+                constexpr int r = decltype(I)::value + 1;
+                constexpr int j = (r * s) % R;
+                const cx<T> w = tw[rtw + j];
+                A = {fm(sp[r].x, w.x, A.x), fm(sp[r].y, w.x, A.y)};
+                if constexpr (r == 1) Bv = {sm[r].x * w.y, sm[r].y * w.y};
+                else Bv = {fm(sm[r].x, w.y, Bv.x), fm(sm[r].y, w.y, Bv.y)};
+            });
+            o[s] = {A.x - Bv.y, A.y + Bv.x};
+            o[R - s] = {A.x + Bv.y, A.y - Bv.x};
+        });
+        cx<T> x0 = u[0];
+        sfor<hh>([&](auto I) {
+            constexpr int r = decltype(I)::value + 1;
+            x0 = {x0.x + sp[r].x, x0.y + sp[r].y};
+        });
+        o[0] = x0;
+        sfor<R>([&](auto I) {
+            constexpr int s = decltype(I)::value;
+            L.v(pl[2 * R + s]) = o[s];
+        });
+    }
+}
+
+/* G consecutive radix-R ops from op i, none reading another's outputs (the
+ * plan marks such runs): every operand loaded before any output is stored,
+ * so the G ops' loads are in flight together.  An op's reads of a slot a
+ * later op of the run reuses for its outputs still come first. */
+template <int R, int G, typename T>
+__host__ __device__ __forceinline__ void bf_group(const Lane<T> &L, const GenArgs &a, int i,
+                                                  const cx<T> *__restrict__ tw)
+{
+    cx<T> u[G][R];
+    const int *pls[G];
+    sfor<G>([&](auto Gi) {
+        constexpr int g = decltype(Gi)::value;
+        pls[g] = a.pool + a.ops[i + g].pool;
+        bf_load<R>(L, pls[g], tw, u[g]);
     });
-    cx<T> x0 = u[0];
-    sfor<hh>([&](auto I) {
-        constexpr int r = decltype(I)::value + 1;
-        x0 = {x0.x + sp[r].x, x0.y + sp[r].y};
-    });
-    o[0] = x0;
-    sfor<R>([&](auto I) {
-        constexpr int s = decltype(I)::value;
-        L.v(pl[2 * R + s]) = o[s];
+    sfor<G>([&](auto Gi) {
+        constexpr int g = decltype(Gi)::value;
+        bf_store<R>(L, pls[g], tw, u[g]);
     });
 }
 
@@ -610,13 +702,33 @@ __host__ __device__ void bfp_any(const Lane<T> &L, int R, const int *__restrict_
     L.v(pl[2 * R]) = x0;
 }
 
-template <typename T>
-__host__ __device__ void run_c2r(const Lane<T> &L, const GenArgs &a, const CsGenC2R &c)
+/* run op i and the ops it leads: up to the plan's run length (op.pad) of
+ * radix-R ops in groups of 4, 2 or 1; returns the ops done */
+template <int R, typename T>
+__host__ __device__ __forceinline__ int bf_run(const Lane<T> &L, const GenArgs &a, int i, int run,
+                                               const cx<T> *__restrict__ tw)
 {
+    if (run >= 4) {
+        bf_group<R, 4>(L, a, i, tw);
+        return 4;
+    }
+    if (run >= 2) {
+        bf_group<R, 2>(L, a, i, tw);
+        return 2;
+    }
+    bf_group<R, 1>(L, a, i, tw);
+    return 1;
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void run_c2r(const Lane<T> &L, const GenArgs &a, const CsGenC2R &c)
+{
+    constexpr int PB = 16;    /* pre-step outputs per batch of loads */
     const cx<T> *__restrict__ tw = (const cx<T> *)a.tw;
-    for (int i = c.op0; i < c.op0 + c.nop; i++) {
+    for (int i = c.op0; i < c.op0 + c.nop;) {
         const CsGenOp op = a.ops[i];
         const int *__restrict__ pl = a.pool + op.pool;
+        const int run = op.pad;
         if (op.kind == CS_OP_PRE) {
             const int H = op.R, XS = pl[0], twb = pl[1];
             const int *outs = pl + 2;
@@ -624,47 +736,46 @@ __host__ __device__ void run_c2r(const Lane<T> &L, const GenArgs &a, const CsGen
                 const T x0 = (T)L.x(0).x, xh = (T)L.x(H * XS).x;
                 L.v(outs[0]) = {x0 + xh, x0 - xh};
             }
-            for (int k = 1; k < H; k++) {
-                const float2 fa = L.x(k * XS), fb = L.x((H - k) * XS);
-                const cx<T> A = {(T)fa.x, (T)fa.y}, Bv = {(T)fb.x, (T)fb.y};
-                const T er = A.x + Bv.x, ei = A.y - Bv.y;
-                const cx<T> o = cmul(cx<T>{A.x - Bv.x, A.y + Bv.y}, tw[twb + k - 1]);
-                L.v(outs[k]) = {er - o.y, ei + o.x};
+            /* k = 1 .. H-1 in batches: every stack read of a batch before
+             * its stores (the stack and the value slots do not overlap) */
+            for (int k0 = 1; k0 < H; k0 += PB) {
+                float2 fa[PB], fb[PB];
+                cx<T> w[PB];
+                sfor<PB>([&](auto U) {
+                    constexpr int u = decltype(U)::value;
+                    const int k = k0 + u < H ? k0 + u : H - 1;
+                    fa[u] = L.x(k * XS);
+                    fb[u] = L.x((H - k) * XS);
+                    w[u] = tw[twb + k - 1];
+                });
+                sfor<PB>([&](auto U) {
+                    constexpr int u = decltype(U)::value;
+                    const int k = k0 + u;
+                    if (k < H) {
+                        const cx<T> A = {(T)fa[u].x, (T)fa[u].y}, Bv = {(T)fb[u].x, (T)fb[u].y};
+                        const T er = A.x + Bv.x, ei = A.y - Bv.y;
+                        const cx<T> o = cmul(cx<T>{A.x - Bv.x, A.y + Bv.y}, w[u]);
+                        L.v(outs[k]) = {er - o.y, ei + o.x};
+                    }
+                });
             }
+            i++;
         } else if (op.kind == CS_OP_BF2) {
-            cx<T> u[2];
-            for (int r = 0; r < 2; r++) {
-                const cx<T> y = L.v(pl[r]);
-                u[r] = pl[2 + r] < 0 ? y : cmul(y, tw[pl[2 + r]]);
-            }
-            L.v(pl[4]) = {u[0].x + u[1].x, u[0].y + u[1].y};
-            L.v(pl[5]) = {u[0].x - u[1].x, u[0].y - u[1].y};
+            i += bf_run<2>(L, a, i, run, tw);
         } else if (op.kind == CS_OP_BF4) {
-            cx<T> u[4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const cx<T> y = L.v(pl[r]);
-                u[r] = pl[4 + r] < 0 ? y : cmul(y, tw[pl[4 + r]]);
-            }
-            const cx<T> t0 = {u[0].x + u[2].x, u[0].y + u[2].y};
-            const cx<T> t1 = {u[0].x - u[2].x, u[0].y - u[2].y};
-            const cx<T> t2 = {u[1].x + u[3].x, u[1].y + u[3].y};
-            const cx<T> t3 = {u[1].x - u[3].x, u[1].y - u[3].y};
-            L.v(pl[8]) = {t0.x + t2.x, t0.y + t2.y};
-            L.v(pl[10]) = {t0.x - t2.x, t0.y - t2.y};
-            L.v(pl[9]) = {t1.x - t3.y, t1.y + t3.x};
-            L.v(pl[11]) = {t1.x + t3.y, t1.y - t3.x};
+            i += bf_run<4>(L, a, i, run, tw);
         } else if (op.kind == CS_OP_BFP) {
             switch (op.R) {
-            case 3: bfp_small<3>(L, pl, tw); break;
-            case 5: bfp_small<5>(L, pl, tw); break;
-            case 7: bfp_small<7>(L, pl, tw); break;
-            case 11: bfp_small<11>(L, pl, tw); break;
-            case 13: bfp_small<13>(L, pl, tw); break;
-            default: bfp_any(L, op.R, pl, tw, a.nslots); break;
+            case 3: i += bf_run<3>(L, a, i, run, tw); break;
+            case 5: i += bf_run<5>(L, a, i, run, tw); break;
+            case 7: bf_group<7, 1>(L, a, i, tw); i++; break;
+            case 11: bf_group<11, 1>(L, a, i, tw); i++; break;
+            case 13: bf_group<13, 1>(L, a, i, tw); i++; break;
+            default: bfp_any(L, op.R, pl, tw, a.nslots); i++; break;
             }
         } else {
             L.v(pl[1]) = L.v(pl[0]);
+            i++;
         }
     }
 }
