@@ -104,7 +104,7 @@ static inline size_t cs_gen_work_bytes(int nvslots, int nh, long long nlanes, bo
 }
 
 /* workgroups of the persistent grid (each loops over tiles) */
-#define CS_GEN_GRID 1024
+#define CS_GEN_GRID 2048
 
 extern "C" __attribute__((visibility("hidden"))) hipError_t cs_gen_launch(const GenArgs *a,
                                                                           bool f64,

@@ -170,8 +170,12 @@ __global__ __launch_bounds__(256) void sigma_kernel(const T *__restrict__ samp,
     }
 }
 
-/* HBM the general kernel's work buffer may take per session */
-#define CS_GEN_WORK_BUDGET (1536ll << 20)
+/* HBM the general kernel's work buffer may take per session: with 4 GiB a
+ * slice of the two-pass search at nharms 240 is 2048 workgroups, four whole
+ * rounds of the chip's wave slots at two waves per SIMD (at 1.5 GiB it was
+ * 790, and each launch's second round ran half full; 8 GiB and 4096
+ * workgroups measured no faster) */
+#define CS_GEN_WORK_BUDGET (4096ll << 20)
 /* HBM the two-pass search's stack buffer may take per session (one slice) */
 #define CS_SPLIT_BUDGET (1024ll << 20)
 /* dynamic LDS of the stack pass: a group's windows up to the first bound
