@@ -257,7 +257,7 @@ def metric_stats_timing(spec, params, reps=3):
     return out
 
 
-def wide_harmonics(spec, steps=5, general=True):
+def wide_harmonics(spec, steps=5, general=True, only=None):
     """The reference README's wider configurations on the same resident
     spectrum, timed like the headline step: nharms 120 with maxdecim 16 and
     nharms 128 (the register kernels' two-word filled mask, one wave per
@@ -269,6 +269,9 @@ def wide_harmonics(spec, steps=5, general=True):
     if general:
         cases += [(240, 16, 16, 2), (60, 6, 64, 2)]
     for nh, md, m, nst in cases:
+        key = "nharms%d_maxdecim%d" % (nh, md) + ("" if m == 16 else "_m%d" % m)
+        if only and key not in only:
+            continue
         p = cs.make_params(nharms=nh, m=m, maxdecim=md)
         sess = cs.Session(spec, p, blocksize=65536)
         ht, trials, _ = sess.work()
@@ -281,7 +284,6 @@ def wide_harmonics(spec, steps=5, general=True):
             kms.append(sess.kernel_ms())
         wall = (time.perf_counter() - t) / nst
         sess.close()
-        key = "nharms%d_maxdecim%d" % (nh, md) + ("" if m == 16 else "_m%d" % m)
         out[key] = {"trials": trials, "harmonic_trials": ht, "ms_per_search": 1e3 * wall,
                     "kernel_ms": float(np.mean(kms)), "ht_s": ht / wall}
         log("nharms %d maxdecim %d m %d: %.2f ms per search, %.4g ht/s"
@@ -484,6 +486,9 @@ def main():
     ap.add_argument("--wide-only", action="store_true",
                     help="only time the nharms 120 / maxdecim 16 and nharms 128 / maxdecim 6 "
                          "searches of the 2^24-sample spectrum (prints one JSON line)")
+    ap.add_argument("--wide-case", action="append", default=None, metavar="KEY",
+                    help="with --wide-only, time only this case (e.g. nharms240_maxdecim16); "
+                         "repeatable")
     ap.add_argument("--no-general", action="store_true",
                     help="with --wide-only: only the register-kernel cases (nharms 120/128)")
     ap.add_argument("--c2-only", action="store_true",
@@ -535,7 +540,8 @@ def main():
     log("staging (synth + realfft + rednoise): %.2f s" % (time.perf_counter() - t))
     if args.wide_only:
         print(json.dumps({"wide_harmonics": wide_harmonics(spec, steps=max(1, args.steps),
-                                                           general=not args.no_general)}),
+                                                           general=not args.no_general,
+                                                           only=args.wide_case)}),
               flush=True)
         return
     pkw = dict(nharms=60, m=16, maxdecim=6)
